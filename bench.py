@@ -1,0 +1,136 @@
+"""Benchmark of the implicit diffusion solve on one MI355X.
+
+Workload: the 4096x4096 deck (two seeded rectangles and a circle on [0,10]^2,
+density 0.1 / 100), FP64 CG with strip-4 block-Jacobi preconditioning,
+tl_eps 1e-15, dt 0.004. A step is one implicit timestep (one tea_leaf solve to
+convergence) run by the device-resident driver; the timed region covers
+exactly `--steps` timesteps after `--warmup` untimed ones, with all fields
+resident in HBM. `value` is CG iterations per second over the timed region.
+
+Printed as one JSON line; see the fields for the solve time per step, the
+per-kernel roofline of the dominant kernel (HIP events on the library's own
+stream) and a CPU baseline (the FP64 oracle on the host cores, a bounded
+sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12
+METRIC = "solve s to 1e-10 rel resid @4096² (FP64 vs mixed); HBM GB/s % peak; iters/s"
+
+# algorithmic FP64 bytes per cell (each distinct array read and/or written once)
+ALG_BYTES = {
+    # p_new = z + beta p_old; w = A p_new; p.w : R z, p_old, Kx, Ky  W p_new
+    "k_fused_cg_w": 5 * 8,
+    # u += a p; r -= a A p; z = M_blk^-1 r; r.z : R u, p, r, Kx, Ky  W u, r, z
+    "k_fused_cg_ur": 8 * 8,
+}
+REF_BYTES_PER_ITER = 144  # reference CG + strip-4 block Jacobi (calc_w + calc_ur + calc_p)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--cpu-iters", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", type=int, default=1, help="1 native (default), 0 drop-in")
+    return ap.parse_args()
+
+
+def cpu_baseline(deck, iters):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    pyoracle.load()
+    secs, done = pyoracle.time_cg_iterations(deck, iters)
+    return {"value": done / secs if secs > 0 else None, "unit": "iters/s", "cores": pyoracle.num_threads(),
+            "kind": "port",
+            "sample": "%d FP64 CG + block-Jacobi iterations of the first timestep at %s, oracle/tl_oracle.c "
+                      "(OpenMP), timed after setup" % (done, os.path.basename(deck))}
+
+
+def load_traffic(n):
+    path = os.path.join(ROOT, "profiles", "traffic_k_fused_cg_ur_%d.json" % n)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs one chunk on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import tealeaf_opencl_amd as tl
+    from tealeaf_opencl_amd import decks
+
+    tl.load_library()
+    n = args.n
+    with tempfile.TemporaryDirectory() as d:
+        deck = decks.config2_deck(os.path.join(d, "bench.in"), n=n, steps=args.warmup + args.steps,
+                                  summary_frequency=0)
+        r = tl.bench_deck(deck, mode=args.mode, warmup=args.warmup, steps=args.steps)
+        # solve seconds to 1e-10 (one timestep, true residual reported)
+        deck10 = decks.config2_deck(os.path.join(d, "e10.in"), n=n, steps=1, eps=1e-10, summary_frequency=0,
+                                    extra=["tl_check_result"])
+        t0 = time.time()
+        r10 = tl.bench_deck(deck10, mode=args.mode, warmup=0, steps=1)
+        cpu = None if args.no_cpu else cpu_baseline(deck, args.cpu_iters)
+
+    cells = n * n
+    its = r["iterations"]
+    secs = r["seconds"]
+    it_s = its / secs
+    k1, k2 = r["k1_ms"], r["k2_ms"]
+    roof = None
+    if k2 and k2 > 0:
+        alg = ALG_BYTES["k_fused_cg_ur"] * cells
+        ach = alg / (k2 * 1e-3)
+        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "traffic": load_traffic(n), "kernel": "k_fused_cg_ur<block>",
+                "alg_bytes_per_launch": alg, "avg_ms": k2, "samples": r["k_samples"],
+                "k_fused_cg_w": {"avg_ms": k1, "alg_bytes_per_launch": ALG_BYTES["k_fused_cg_w"] * cells,
+                                 "achieved_GBs": ALG_BYTES["k_fused_cg_w"] * cells / (k1 * 1e-3) / 1e9}}
+    per_iter_bytes = (ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur"]) * cells
+    out = {
+        "metric": METRIC,
+        "value": it_s,
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "tealeaf %dx%d CG + strip-4 block Jacobi FP64, eps 1e-15, dt 0.004, "
+                               "2 rectangles + circle (seed 1)" % (n, n),
+                   "cells": cells, "mode": "native" if args.mode == 1 else "drop-in"},
+        "iterations": its,
+        "solve_s_per_step": secs / args.steps,
+        "hbm_GBs_per_iteration": per_iter_bytes * it_s / 1e9,
+        "hbm_frac_per_iteration": per_iter_bytes * it_s / PEAK_HBM,
+        "reference_equiv_GBs": REF_BYTES_PER_ITER * cells * it_s / 1e9,
+        "solve_to_1e-10": {"fp64_s": r10["seconds"], "fp64_iterations": r10["iterations"], "mixed_s": None},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "final_summary": r["final_summary"],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+/*
+ * tealeaf_mi355x.h -- C-ABI of libtealeaf_mi355x.so, the MI355X (gfx950) HIP
+ * implementation of TeaLeaf's implicit diffusion solve.
+ *
+ * Part 1 is the drop-in boundary: the Fortran-callable entry points that the
+ * reference's Fortran driver calls (lower case, trailing underscore, every
+ * argument by reference). Each one replaces the reference definition cited
+ * beside it and keeps its argument meaning, its synchronous/asynchronous
+ * behaviour (functions that return a scalar block until it is on the host)
+ * and its error behaviour (print "@@@@@ Fatal error ..." to stderr and abort
+ * the job: MPI_Abort when the process has MPI initialised, else exit(1)).
+ * They act on one chunk created by initialise_ocl_, which reads ./tea.in for
+ * the solver, tl_preconditioner_type=, halo_depth= and profiler_on with the
+ * reference's own matching rules.
+ *
+ * Part 2 is a superset API used by the native driver, the Python package and
+ * the tests: explicit chunk handles, status codes instead of aborts, field
+ * upload/download in the reference layout, and the device-resident driver.
+ *
+ * Reference layout of a 2-D field, as exchanged by tl_field_get/tl_field_set:
+ * (x_max + 2*hd) columns by (y_max + 2*hd) rows, row-major, interior cells at
+ * [hd, hd+x_max) x [hd, hd+y_max) (kernel_files/macros_cl.cl:20-22).
+ */
+#ifndef TEALEAF_MI355X_H
+#define TEALEAF_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================= */
+/* Part 1: reference entry points                                          */
+/* ======================================================================= */
+
+/* ocl_init.cpp:11-17 -- create the chunk; reads ./tea.in (side channel). */
+void initialise_ocl_(int *x_min, int *x_max, int *y_min, int *y_max);
+/* initialise_chunk_kernel_ocl.cpp:3-7 -- coordinates, volume, face areas. */
+void initialise_chunk_kernel_ocl_(double *xmin, double *ymin, double *dx, double *dy);
+/* generate_chunk_kernel_ocl.cpp:3-23 -- paint states 1..n, then u = u0 = e0*density. */
+void generate_chunk_kernel_ocl_(const int *number_of_states, const double *state_density,
+                                const double *state_energy, const double *state_xmin,
+                                const double *state_xmax, const double *state_ymin,
+                                const double *state_ymax, const double *state_radius,
+                                const int *state_geometry, const int *g_rect, const int *g_circ,
+                                const int *g_point);
+/* set_field_kernel_ocl.cpp:3-7 -- energy1 = energy0. */
+void set_field_kernel_ocl_(void);
+/* field_summary_kernel_ocl.cpp:4-8 -- blocking; volume, mass, mass*e1, mass*u sums. */
+void field_summary_kernel_ocl_(double *vol, double *mass, double *ie, double *temp);
+/* update_halo_kernel_ocl.cpp:6-12 -- reflective fill of density/energy0/energy1. */
+void update_halo_kernel_ocl_(const int *chunk_neighbours, const int *fields, const int *depth);
+/* tea_leaf_kernel_ocl.cpp:233-239 -- rx, ry are outputs; Kx, Ky; walls; u = u0 = e1*density. */
+void tea_leaf_common_init_kernel_ocl_(const int *coefficient, double *dt, double *rx, double *ry,
+                                      int *zero_boundary, int *reflective_boundary);
+/* tea_leaf_kernel_ocl.cpp:248-252 -- r = u0 - A u. */
+void tea_leaf_calc_residual_ocl_(void);
+/* tea_leaf_kernel_ocl.cpp:34-38 -- blocking; 0: u0.u0, 1: r.r, 2: r.z (r.r unpreconditioned). */
+void tea_leaf_calc_2norm_kernel_ocl_(int *norm_array, double *norm);
+/* tea_leaf_kernel_ocl.cpp:242-246 -- energy1 = u / density. */
+void tea_leaf_common_finalise_kernel_ocl_(void);
+/* tea_leaf_kernel_ocl.cpp:143-147 -- blocking; preconditioner setup, p, rro. */
+void tea_leaf_cg_init_kernel_ocl_(double *rro);
+/* tea_leaf_kernel_ocl.cpp:149-153 -- blocking; w = A p, pw = p.w. */
+void tea_leaf_cg_calc_w_kernel_ocl_(double *pw);
+/* tea_leaf_kernel_ocl.cpp:154-158 -- blocking; u += a p, r -= a w, z = M^-1 r, rrn. */
+void tea_leaf_cg_calc_ur_kernel_ocl_(double *alpha, double *rrn);
+/* tea_leaf_kernel_ocl.cpp:159-163 -- p = z + beta p (r + beta p unpreconditioned). */
+void tea_leaf_cg_calc_p_kernel_ocl_(double *beta);
+/* tea_leaf_kernel_ocl.cpp:42-48 -- copies n_coefs coefficients; runs init_p + calc_u. */
+void tea_leaf_cheby_init_kernel_ocl_(const double *ch_alphas, const double *ch_betas, int *n_coefs,
+                                     const double *rx, const double *ry, const double *theta);
+/* tea_leaf_kernel_ocl.cpp:50-54 -- one Chebyshev step with coefficient index step-1. */
+void tea_leaf_cheby_iterate_kernel_ocl_(const int *cheby_calc_step);
+/* tea_leaf_kernel_ocl.cpp:306-311 -- PPCG coefficients and theta. */
+void tea_leaf_ppcg_init_ocl_(const double *ch_alphas, const double *ch_betas, double *theta,
+                             int *n_inner_steps);
+/* tea_leaf_kernel_ocl.cpp:313-317 -- sd = M^-1 r / theta. */
+void tea_leaf_ppcg_init_sd_kernel_ocl_(void);
+/* tea_leaf_kernel_ocl.cpp:319-325 -- update_r + calc_sd on the matrix-powers extent. */
+void tea_leaf_ppcg_inner_kernel_ocl_(int *inner_step, int *bounds_extra, const int *chunk_neighbours);
+/* tea_leaf_kernel_ocl.cpp:216-220 -- blocking; one Jacobi sweep, returns sum |u - un|. */
+void tea_leaf_jacobi_solve_kernel_ocl_(double *error);
+/* ocl_pack.cpp:4-16 -- multi-chunk halo staging. One chunk per GPU never
+ * reaches these (tea.f90:310 returns first); they abort if called. */
+void ocl_pack_buffers_(int fields[7], int offsets[7], int *depth, int *face, double *host_buffer);
+void ocl_unpack_buffers_(int fields[7], int offsets[7], int *depth, int *face, double *host_buffer);
+/* ocl_errors.cpp:196-200 -- per-kernel time table when profiler_on. */
+void print_opencl_profiling_info_(void);
+
+/* ======================================================================= */
+/* Part 2: superset API (status codes; no aborts)                          */
+/* ======================================================================= */
+
+enum {
+    TL_OK = 0,
+    TL_ERR_ARG = 1,     /* bad argument */
+    TL_ERR_HIP = 2,     /* HIP runtime failure (no device, out of memory, launch) */
+    TL_ERR_CONFIG = 3,  /* deck or side-channel error */
+    TL_ERR_SOLVER = 4   /* solver failure (eigenvalue estimate) */
+};
+
+/* preconditioners, solvers, coefficients: kernel_files/definitions.hpp:14-56 */
+enum { TL_PREC_NONE = 1, TL_PREC_JAC_DIAG = 2, TL_PREC_JAC_BLOCK = 3 };
+enum { TL_SOLVER_JACOBI = 1, TL_SOLVER_CG = 2, TL_SOLVER_CHEBY = 3, TL_SOLVER_PPCG = 4 };
+enum { TL_COEF_CONDUCTIVITY = 1, TL_COEF_RECIP_CONDUCTIVITY = 2 };
+
+/* 2-D field ids */
+enum {
+    TL_F_DENSITY = 0, TL_F_ENERGY0, TL_F_ENERGY1, TL_F_VOLUME, TL_F_XAREA, TL_F_YAREA,
+    TL_F_U, TL_F_U0, TL_F_P, TL_F_R, TL_F_W, TL_F_MI, TL_F_KX, TL_F_KY, TL_F_SD, TL_F_Z,
+    TL_F_CP, TL_F_BFP, TL_NUM_FIELDS2D
+};
+/* 1-D field ids */
+enum {
+    TL_F_CELLX = 0, TL_F_CELLDX, TL_F_VERTEXX, TL_F_VERTEXDX,
+    TL_F_CELLY, TL_F_CELLDY, TL_F_VERTEXY, TL_F_VERTEXDY, TL_NUM_FIELDS1D
+};
+
+/* driver modes of tl_run_deck */
+enum {
+    TL_MODE_DROPIN = 0, /* reference call granularity, scalars on the host */
+    TL_MODE_NATIVE = 1  /* fused kernels, device-resident scalars, hipGraph batches */
+};
+
+typedef struct tl_chunk tl_chunk;
+
+const char *tl_last_error(void);
+int tl_device_count(int *count);
+
+int tl_chunk_create(int x_max, int y_max, int halo_depth, int preconditioner, int profiler_on,
+                    tl_chunk **out);
+int tl_chunk_destroy(tl_chunk *c);
+/* chunk the Part 1 entry points act on (initialise_ocl_ sets it too) */
+int tl_chunk_set_default(tl_chunk *c);
+tl_chunk *tl_chunk_default(void);
+int tl_chunk_info(const tl_chunk *c, int *x_max, int *y_max, int *halo_depth, int *preconditioner);
+
+int tl_field_get(tl_chunk *c, int field, double *host);
+int tl_field_set(tl_chunk *c, int field, const double *host);
+int tl_field1d_get(tl_chunk *c, int field, double *host, int n);
+
+/* the bodies of the Part 1 entry points, one per reference kernel call */
+int tl_initialise_chunk(tl_chunk *c, double xmin, double ymin, double dx, double dy);
+int tl_generate_chunk(tl_chunk *c, int nstates, const double *density, const double *energy,
+                      const double *xmin, const double *xmax, const double *ymin, const double *ymax,
+                      const double *radius, const int *geometry, int g_rect, int g_circ, int g_point);
+int tl_set_field(tl_chunk *c);
+int tl_field_summary(tl_chunk *c, double *vol, double *mass, double *ie, double *temp);
+int tl_update_halo(tl_chunk *c, const int *chunk_neighbours, const int *fields, int depth);
+int tl_common_init(tl_chunk *c, int coefficient, double dt, double *rx, double *ry,
+                   const int *zero_boundary, int reflective_boundary);
+int tl_calc_residual(tl_chunk *c);
+int tl_calc_2norm(tl_chunk *c, int norm_array, double *norm);
+int tl_finalise(tl_chunk *c);
+int tl_cg_init(tl_chunk *c, double *rro);
+int tl_cg_calc_w(tl_chunk *c, double *pw);
+int tl_cg_calc_ur(tl_chunk *c, double alpha, double *rrn);
+int tl_cg_calc_p(tl_chunk *c, double beta);
+int tl_cheby_init(tl_chunk *c, const double *ch_alphas, const double *ch_betas, int n_coefs,
+                  double rx, double ry, double theta);
+int tl_cheby_iterate(tl_chunk *c, int cheby_calc_step);
+int tl_ppcg_init(tl_chunk *c, const double *ch_alphas, const double *ch_betas, double theta,
+                 int n_inner_steps);
+int tl_ppcg_init_sd(tl_chunk *c);
+int tl_ppcg_inner(tl_chunk *c, int inner_step, int bounds_extra, const int *chunk_neighbours);
+int tl_jacobi_solve(tl_chunk *c, double *error);
+/* building blocks of tl_cg_init, exposed for preconditioner tests */
+int tl_block_init(tl_chunk *c);
+int tl_block_solve(tl_chunk *c);
+int tl_init_jac_diag(tl_chunk *c);
+int tl_synchronize(tl_chunk *c);
+int tl_print_profiling(tl_chunk *c);
+
+/* Whole-deck driver (start.f90 + diffuse.f90 + tea_solve.f90 restated).
+ * Writes a line-oriented log: SUMMARY step vol mass density ie temp /
+ * SOLVE step iters conduction_err exact_err ppcg_inner cg_steps cheby_steps
+ * switch_n est_itc / SWITCH / ESTITC / RESID / QA lines. max_steps > 0 caps
+ * the number of timesteps. */
+int tl_run_deck(const char *deck_path, const char *log_path, int mode, int max_steps);
+
+/* Timed run for benchmarks: builds the deck's chunk, runs `warmup` then
+ * `steps` timesteps with the given mode and reports wall seconds of the timed
+ * steps, iterations done in them, and the average device duration (ms) of the
+ * two CG iteration kernels measured with HIP events on the library's stream
+ * (-1 when the mode/solver has none). */
+typedef struct {
+    double seconds;
+    long iterations;        /* outer iterations summed over the timed steps */
+    long inner_iterations;  /* PPCG inner steps summed over the timed steps */
+    double k1_ms, k2_ms;    /* average durations of the two CG iteration kernels */
+    long k_samples;
+    double final_summary[5];/* vol, mass, density, ie, temp after the last step */
+} tl_bench_result;
+int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, tl_bench_result *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,79 @@
+// Private definition of the chunk: the device buffers of one TeaLeaf chunk
+// (types.hpp:42-367 of the reference), its stream, the reduction scratch and
+// the device-resident solver state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+
+#include "tl_common.h"
+
+struct tl_chunk {
+    tl::Geom g{};
+    int prec = tl::PREC_NONE;
+    int profiler_on = 0;
+    int solver = tl::SOLVER_JACOBI;  // side-channel solver (informational)
+    hipStream_t s = nullptr;
+
+    long elems2d = 0;                // doubles per 2-D allocation
+    long off = 0;                    // offset of element (0,0) in an allocation
+    double *base2[tl::NUM_FIELDS_2D] = {};
+    double *f2[tl::NUM_FIELDS_2D] = {};
+    double *f1[tl::NUM_FIELDS_1D] = {};
+    int len1d = 0;
+
+    int cap = 2048;                  // max blocks per launch
+    double *partials = nullptr;
+    unsigned *counter = nullptr;
+    double *result = nullptr;
+    tl::SolveState *st = nullptr;
+    double *hist_a = nullptr, *hist_b = nullptr, *hist_e = nullptr;
+    int hist_len = 0;
+    double *ch_a = nullptr, *ch_b = nullptr;
+    int ch_cap = 0;
+    int n_coefs = 0;
+
+    double *h_result = nullptr;      // pinned
+    tl::SolveState *h_st = nullptr;  // pinned, 2 slots
+
+    // host copies of scalars the reference keeps in kernel arguments
+    double rx = 0, ry = 0, theta = 0, dx = 0, dy = 0;
+
+    // profiler_on: per-kernel device time (ms) and calls (ocl_errors.cpp:64-121)
+    std::map<std::string, std::pair<double, long>> prof;
+    hipEvent_t pe0 = nullptr, pe1 = nullptr;
+
+    // captured iteration batches of the native driver
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        int key = -1;
+    };
+    GraphSlot g_cg, g_cheby, g_ppcg, g_jacobi;
+    // events around the first CG iteration of each captured batch
+    hipEvent_t ev_t[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_batch[2] = {nullptr, nullptr};
+    bool timing = false;
+    double t_k1_ms = 0, t_k2_ms = 0;
+    long t_samples = 0;
+};
+
+namespace tl {
+
+// interior and launch-bound extents (ocl_kernels.cpp:123-134)
+Ext make_ext(int j0, int j1, int k0, int k1);
+Ext ext_range(const tl_chunk *c, int xmin, int xmax, int ymin, int ymax);
+inline Ext ext_interior(const tl_chunk *c) { return ext_range(c, 0, 0, 0, 0); }
+RedArgs red_args(tl_chunk *c, int mode, bool with_state);
+void set_error(const char *fmt, ...);
+int hip_fail(const char *what, hipError_t e);
+// profiling wrapper bracket (no-ops unless profiler_on)
+void prof_begin(tl_chunk *c);
+void prof_end(tl_chunk *c, const char *name);
+int ensure_coef_capacity(tl_chunk *c, int n);
+int ensure_hist_capacity(tl_chunk *c, int n);
+int read_result(tl_chunk *c, double *out, int nq);
+
+}  // namespace tl

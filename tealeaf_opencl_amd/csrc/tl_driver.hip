@@ -1,0 +1,746 @@
+// Native driver: start.f90:24-115, diffuse.f90:43-101, tea_solve.f90:37-521
+// and field_summary.f90:27-110 restated over the chunk operations.
+//
+// Two modes share the same host logic:
+//  * drop-in: every step calls the reference-granularity operations and does
+//    the scalar algebra on the host, exactly as the Fortran does through the
+//    entry points (one blocking 8-byte read per dot product);
+//  * native: the scalars live on the device (SolveState), each solver
+//    iteration is a fixed sequence of fused kernels whose last reduction
+//    finishes the iteration's algebra, and batches of iterations are captured
+//    once into hipGraphs and replayed. The host only looks at the state after
+//    each batch (the next batch is already queued), at the CG -> Chebyshev /
+//    PPCG switch and at the end of the solve.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/tealeaf_mi355x.h"
+#include "tl_chunk.h"
+#include "tl_deck.h"
+#include "tl_kernels.h"
+
+using namespace tl;
+
+int tl_cg_init_device(tl_chunk *c);
+
+namespace {
+
+struct Save {  // SAVE'd variables of tea_solve.f90:54-57
+    int first = 1;
+    double eigmin = 0, eigmax = 0, theta = 0, cn = 0;
+};
+
+struct Stats {
+    long iterations = 0;
+    long inner = 0;
+};
+
+const double kat_temp[7] = {0.0, 157.550841832793, 116.067951160930, 95.4865103390698,
+                            166.838315378708, 116.482111627676, 103.88639125996923};
+
+#define RC(x)                          \
+    do {                               \
+        int rc_ = (x);                 \
+        if (rc_ != TL_OK) return rc_;  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host scalar routines (tea_leaf_cheby.f90:44-172)
+// ---------------------------------------------------------------------------
+
+// implicit QL on a symmetric tridiagonal matrix (tea_leaf_cheby.f90:44-94)
+int tqli(std::vector<double> &d, std::vector<double> &e, int n)
+{
+    for (int i = 0; i < n - 1; i++) e[i] = e[i + 1];
+    if (n > 0) e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0;
+        for (;;) {
+            int m;
+            for (m = l; m < n - 1; m++) {
+                const double dd = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) + dd == dd) break;
+            }
+            if (m == l) break;
+            if (iter == 30) return 1;
+            iter++;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = sqrt(g * g + 1.0 * 1.0);
+            g = d[m] - d[l] + e[l] / (g + copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool restart = false;
+            for (int i = m - 1; i >= l; i--) {
+                const double f = s * e[i];
+                const double b = c * e[i];
+                r = sqrt(f * f + g * g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] = d[i + 1] - p;
+                    e[m] = 0.0;
+                    restart = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+            }
+            if (restart) continue;
+            d[l] = d[l] - p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    return 0;
+}
+
+// tea_calc_eigenvalues (tea_leaf_cheby.f90:96-143); alphas/betas 0-based
+int calc_eigenvalues(const double *a, const double *b, int ns, double *eigmin, double *eigmax)
+{
+    if (ns < 1) return 1;
+    std::vector<double> d(ns, 0.0), e(ns, 0.0);
+    for (int n = 0; n < ns; n++) {
+        d[n] = 1.0 / a[n];
+        if (n > 0) d[n] = d[n] + b[n - 1] / a[n - 1];
+        if (n < ns - 1) e[n + 1] = sqrt(b[n]) / a[n];
+    }
+    int info = tqli(d, e, ns);
+    if (info) return info;
+    bool swapped;
+    do {
+        swapped = false;
+        for (int n = 0; n < ns - 1; n++)
+            if (d[n] > d[n + 1]) {
+                std::swap(d[n], d[n + 1]);
+                swapped = true;
+            }
+    } while (swapped);
+    *eigmin = d[0];
+    *eigmax = d[ns - 1];
+    if (*eigmin < 0.0 || *eigmax < 0.0) info = 1;
+    return info;
+}
+
+// tea_calc_ch_coefs (tea_leaf_cheby.f90:145-172)
+void calc_ch_coefs(std::vector<double> &ca, std::vector<double> &cb, double eigmin, double eigmax, double *theta,
+                   int n)
+{
+    const double th = (eigmax + eigmin) / 2;
+    const double delta = (eigmax - eigmin) / 2;
+    const double sigma = th / delta;
+    double rho_old = 1.0 / sigma;
+    ca.assign(n > 0 ? n : 1, 0.0);
+    cb.assign(n > 0 ? n : 1, 0.0);
+    for (int i = 0; i < n; i++) {
+        const double rho_new = 1.0 / (2.0 * sigma - rho_old);
+        ca[i] = rho_new * rho_old;
+        cb[i] = 2.0 * rho_new / delta;
+        rho_old = rho_new;
+    }
+    *theta = th;
+}
+
+void emit_summary(FILE *lg, tl_chunk *c, int step, bool complete, const Config &cfg, double *out5)
+{
+    double vol, mass, ie, temp;
+    if (tl_field_summary(c, &vol, &mass, &ie, &temp) != TL_OK) return;
+    if (out5) {
+        out5[0] = vol; out5[1] = mass; out5[2] = mass / vol; out5[3] = ie; out5[4] = temp;
+    }
+    if (lg) fprintf(lg, "SUMMARY %d %.17e %.17e %.17e %.17e %.17e\n", step, vol, mass, mass / vol, ie, temp);
+    if (complete && cfg.test_problem >= 1 && cfg.test_problem <= 6 && lg) {
+        const double qa = fabs((100.0 * (temp / kat_temp[cfg.test_problem])) - 100.0);
+        fprintf(lg, "QA %d %.7e %s\n", cfg.test_problem, qa, qa < 0.001 ? "PASSED" : "FAILED");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// drop-in mode: tea_solve.f90:37-420 over the reference-granularity calls
+// ---------------------------------------------------------------------------
+int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, FILE *lg, Stats *stats)
+{
+    const int max_iters = cfg.max_iters;
+    const bool use_cg = cfg.solver == SOLVER_CG, use_ch = cfg.solver == SOLVER_CHEBY;
+    const bool use_ppcg = cfg.solver == SOLVER_PPCG, use_jac = cfg.solver == SOLVER_JACOBI;
+    const int nb[4] = {-1, -1, -1, -1};
+    std::vector<double> cg_alphas(max_iters + 2, 0.0), cg_betas(max_iters + 2, 0.0);
+    std::vector<double> ch_alphas, ch_betas;
+    double old_error, error = 0.0, exact_error = 0.0, initial_residual;
+    double rro = 0.0, pw, rrn, alpha, beta;
+    int est_itc = 0, cheby_calc_steps = 0, max_cheby_iters = 0, ppcg_inner_iters = 0;
+    int cg_calc_steps = 0, switch_n = 0;
+    bool ch_switch_check = false;
+    double rx, ry;
+
+    int fields[7] = {1, 0, 1, 0, 0, 0, 0};
+    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    const int zb[4] = {1, 1, 1, 1};
+    RC(tl_common_init(c, cfg.coefficient, dt, &rx, &ry, zb, cfg.reflective_boundary));
+    RC(tl_calc_residual(c));
+    RC(tl_calc_2norm(c, 1, &initial_residual));
+    old_error = initial_residual;
+    initial_residual = sqrt(initial_residual);
+    if (lg && cfg.verbose_on) fprintf(lg, "INITRES %d %.17e\n", step, initial_residual);
+    if (use_cg || use_ch || use_ppcg) RC(tl_cg_init(c, &rro));
+
+    int n;
+    for (n = 1; n <= max_iters; n++) {
+        if (!ch_switch_check) {
+            if (cheby_calc_steps > 0) ch_switch_check = true;
+            else if (!sv.first && use_ppcg && n > 1) ch_switch_check = true;
+            else if (fabs(old_error) <= cfg.tl_ch_cg_epslim && n >= cfg.tl_ch_cg_presteps) ch_switch_check = true;
+            else ch_switch_check = false;
+        }
+        if ((use_ch || use_ppcg) && ch_switch_check) {
+            if (cheby_calc_steps == 0) {
+                max_cheby_iters = max_iters - n + 2;
+                if (sv.first) {
+                    const int info = calc_eigenvalues(&cg_alphas[1], &cg_betas[1], n - 1, &sv.eigmin, &sv.eigmax);
+                    sv.first = 0;
+                    if (info != 0) {
+                        set_error("tea_leaf: Error in calculating eigenvalues");
+                        return TL_ERR_SOLVER;
+                    }
+                    sv.eigmin = sv.eigmin * 0.95;
+                    sv.eigmax = sv.eigmax * 1.05;
+                }
+                calc_ch_coefs(ch_alphas, ch_betas, sv.eigmin, sv.eigmax, &sv.theta,
+                              use_ch ? max_cheby_iters : cfg.tl_ppcg_inner_steps);
+                sv.cn = sv.eigmax / sv.eigmin;
+                switch_n = n;
+                if (lg)
+                    fprintf(lg, "SWITCH %d %d %.17e %.17e %.17e %.17e %.17e\n", step, n, rro, sv.eigmin, sv.eigmax,
+                            sv.cn, old_error);
+            }
+            if (use_ch) {
+                if (cheby_calc_steps == 0) {
+                    double bb;
+                    RC(tl_calc_2norm(c, 0, &bb));
+                    RC(tl_cheby_init(c, ch_alphas.data(), ch_betas.data(), max_cheby_iters, rx, ry, sv.theta));
+                    RC(tl_cheby_iterate(c, 1));
+                    RC(tl_calc_2norm(c, 1, &old_error));
+                    const double it_alpha = cfg.eps * bb / (4.0 * old_error);
+                    const double gamm = (sqrt(sv.cn) - 1.0) / (sqrt(sv.cn) + 1.0);
+                    est_itc = (int)lround(log(it_alpha) / (2.0 * log(gamm)));
+                    if (lg) fprintf(lg, "ESTITC %d %d\n", step, est_itc);
+                    cheby_calc_steps = 1;
+                } else {
+                    RC(tl_cheby_iterate(c, cheby_calc_steps));
+                    if (n >= est_itc && n % 10 == 0) RC(tl_calc_2norm(c, 1, &error));
+                }
+            } else {
+                if (cheby_calc_steps == 0)
+                    RC(tl_ppcg_init(c, ch_alphas.data(), ch_betas.data(), sv.theta, cfg.tl_ppcg_inner_steps));
+                RC(tl_cg_calc_w(c, &pw));
+                alpha = rro / pw;
+                RC(tl_cg_calc_ur(c, alpha, &rrn));
+                RC(tl_ppcg_init_sd(c));
+                for (int cur = 1; cur <= cfg.tl_ppcg_inner_steps; cur += cfg.halo_depth) {
+                    int inner_step = cur;
+                    for (int be = cfg.halo_depth - 1; be >= 0; be--) {
+                        RC(tl_ppcg_inner(c, inner_step, be, nb));
+                        inner_step++;
+                        if (inner_step > cfg.tl_ppcg_inner_steps) break;
+                    }
+                }
+                ppcg_inner_iters += cfg.tl_ppcg_inner_steps;
+                RC(tl_calc_2norm(c, 2, &rrn));
+                beta = rrn / rro;
+                RC(tl_cg_calc_p(c, beta));
+                error = rrn;
+                rro = rrn;
+            }
+            cheby_calc_steps++;
+        } else if (use_cg || use_ch || use_ppcg) {
+            cg_calc_steps++;
+            RC(tl_cg_calc_w(c, &pw));
+            alpha = rro / pw;
+            cg_alphas[n] = alpha;
+            RC(tl_cg_calc_ur(c, alpha, &rrn));
+            beta = rrn / rro;
+            cg_betas[n] = beta;
+            RC(tl_cg_calc_p(c, beta));
+            error = rrn;
+            rro = rrn;
+        } else if (use_jac) {
+            RC(tl_jacobi_solve(c, &error));
+        }
+        error = sqrt(error);
+        if (lg && cfg.verbose_on) fprintf(lg, "RESID %d %d %.17e\n", step, n, error);
+        if (fabs(error) < cfg.eps * initial_residual) break;
+        old_error = error;
+    }
+    if (cfg.tl_check_result) {
+        RC(tl_calc_residual(c));
+        RC(tl_calc_2norm(c, 1, &exact_error));
+        exact_error = sqrt(exact_error);
+    }
+    if (lg)
+        fprintf(lg, "SOLVE %d %d %.17e %.17e %d %d %d %d %d\n", step, n - 1, error / initial_residual,
+                cfg.tl_check_result ? exact_error / initial_residual : NAN, ppcg_inner_iters, cg_calc_steps,
+                cheby_calc_steps, switch_n, est_itc);
+    if (stats) {
+        stats->iterations += n > max_iters ? max_iters : n;
+        stats->inner += ppcg_inner_iters;
+    }
+    RC(tl_finalise(c));
+    fields[0] = 0;
+    fields[2] = 1;
+    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, 1));
+    return TL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// native mode
+// ---------------------------------------------------------------------------
+int state_download(tl_chunk *c, SolveState *h)
+{
+    hipError_t e = hipMemcpyAsync(&c->h_st[0], c->st, sizeof(SolveState), hipMemcpyDeviceToHost, c->s);
+    if (e != hipSuccess) return hip_fail("state download", e);
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("state download", e);
+    *h = c->h_st[0];
+    return TL_OK;
+}
+
+int state_upload(tl_chunk *c, const SolveState &h)
+{
+    hipError_t e = hipStreamSynchronize(c->s);
+    if (e != hipSuccess) return hip_fail("state upload", e);
+    c->h_st[0] = h;
+    e = hipMemcpyAsync(c->st, &c->h_st[0], sizeof(SolveState), hipMemcpyHostToDevice, c->s);
+    if (e != hipSuccess) return hip_fail("state upload", e);
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("state upload", e);
+    return TL_OK;
+}
+
+int capture(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body)
+{
+    if (gs.exec && gs.key == key) return TL_OK;
+    if (gs.exec) {
+        hipGraphExecDestroy(gs.exec);
+        gs.exec = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
+    body();
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(c->s, &graph);
+    if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
+    e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        gs.exec = nullptr;
+        return hip_fail("hipGraphInstantiate", e);
+    }
+    gs.key = key;
+    return TL_OK;
+}
+
+// Replays `exec` until the device state says halt. Without timing, batch i+1
+// is queued before the state of batch i is examined. With timing, batches run
+// one at a time and the events recorded around the first CG iteration of each
+// batch are read after it.
+int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
+{
+    hipError_t e;
+    int slot = 0;
+    auto enqueue = [&](int sl) -> hipError_t {
+        hipError_t ee = hipGraphLaunch(exec, c->s);
+        if (ee != hipSuccess) return ee;
+        ee = hipMemcpyAsync(&c->h_st[sl], c->st, sizeof(SolveState), hipMemcpyDeviceToHost, c->s);
+        if (ee != hipSuccess) return ee;
+        return hipEventRecord(c->ev_batch[sl], c->s);
+    };
+    if ((e = enqueue(0)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
+    for (long guard = 0;; ++guard) {
+        if (!timed) {
+            if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
+        }
+        if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
+        const SolveState &h = c->h_st[slot];
+        if (timed) {
+            float a = 0.f, b = 0.f;
+            if (hipEventElapsedTime(&a, c->ev_t[0], c->ev_t[1]) == hipSuccess &&
+                hipEventElapsedTime(&b, c->ev_t[1], c->ev_t[2]) == hipSuccess && !h.halt) {
+                c->t_k1_ms += a;
+                c->t_k2_ms += b;
+                c->t_samples += 1;
+            }
+        }
+        if (h.halt) break;
+        if (guard > max_batches) {
+            hipStreamSynchronize(c->s);
+            set_error("solver state did not halt after %ld batches", guard);
+            return TL_ERR_SOLVER;
+        }
+        if (timed) {
+            if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
+        }
+        slot = 1 - slot;
+    }
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("batch drain", e);
+    return TL_OK;
+}
+
+int cg_graph(tl_chunk *c, bool timed)
+{
+    const int B = 16;
+    const int key = c->prec * 10 + (timed ? 1 : 0);
+    const Ext in = ext_interior(c);
+    const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
+    return capture(c, c->g_cg, key, [&]() {
+        for (int b = 0; b < B; ++b) {
+            if (timed && b == 0) hipEventRecord(c->ev_t[0], c->s);
+            launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_KX], c->f2[F_KY],
+                              red_args(c, RED_CG_PW, true));
+            if (timed && b == 0) hipEventRecord(c->ev_t[1], c->s);
+            launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
+                               c->f2[F_Z], c->f2[F_SD], c->f2[F_KX], c->f2[F_KY], red_args(c, RED_CG_RRN, true));
+            if (timed && b == 0) hipEventRecord(c->ev_t[2], c->s);
+        }
+    });
+}
+
+int cheby_graph(tl_chunk *c)
+{
+    const int B = 16;
+    const Ext in = ext_interior(c);
+    return capture(c, c->g_cheby, c->prec, [&]() {
+        for (int b = 0; b < B; ++b) {
+            launch_cheby_p(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_U0], c->f2[F_P], c->f2[F_R],
+                           nullptr, c->f2[F_CP], c->f2[F_BFP], c->f2[F_MI], c->f2[F_KX], c->f2[F_KY], c->ch_a,
+                           c->ch_b, 0.0, 0, c->st);
+            launch_cheby_u(c->s, c->g, in, c->cap, c->f2[F_U], c->f2[F_P], c->f2[F_R], red_args(c, RED_CHEBY, true));
+        }
+    });
+}
+
+int ppcg_graph(tl_chunk *c, int inner)
+{
+    const int B = std::max(1, 32 / (inner + 2));
+    const Ext in = ext_interior(c);
+    const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
+    return capture(c, c->g_ppcg, c->prec * 100000 + inner, [&]() {
+        for (int b = 0; b < B; ++b) {
+            launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_KX], c->f2[F_KY],
+                              red_args(c, RED_PPCG_PW, true));
+            launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 1, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
+                               c->f2[F_Z], c->f2[F_SD], c->f2[F_KX], c->f2[F_KY], red_args(c, RED_STORE, true));
+            for (int i = 1; i <= inner; ++i) {
+                const double *sin = (i & 1) ? c->f2[F_SD] : c->f2[F_SD2];
+                double *sout = (i & 1) ? c->f2[F_SD2] : c->f2[F_SD];
+                launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner, c->f2[F_U], c->f2[F_R], sin, sout,
+                                        c->f2[F_KX], c->f2[F_KY], c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
+                                        red_args(c, RED_PPCG_RRN, true));
+            }
+        }
+    });
+}
+
+int jacobi_graph(tl_chunk *c)
+{
+    const int B = 16;
+    const Ext in = ext_interior(c);
+    return capture(c, c->g_jacobi, 1, [&]() {
+        for (int b = 0; b < B; ++b)
+            launch_fused_jacobi(c->s, c->g, in, c->cap, c->f2[F_KX], c->f2[F_KY], c->f2[F_U0], c->f2[F_U],
+                                c->f2[F_U2], red_args(c, RED_JACOBI, true));
+    });
+}
+
+int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, FILE *lg, Stats *stats)
+{
+    const int nb[4] = {-1, -1, -1, -1};
+    const bool use_ch = cfg.solver == SOLVER_CHEBY, use_ppcg = cfg.solver == SOLVER_PPCG;
+    const bool use_jac = cfg.solver == SOLVER_JACOBI;
+    int fields[7] = {1, 0, 1, 0, 0, 0, 0};
+    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    const int zb[4] = {1, 1, 1, 1};
+    double rx, ry;
+    RC(tl_common_init(c, cfg.coefficient, dt, &rx, &ry, zb, cfg.reflective_boundary));
+    RC(ensure_hist_capacity(c, cfg.max_iters + 2));
+
+    SolveState h;
+    memset(&h, 0, sizeof h);
+    h.eps = cfg.eps;
+    h.epslim = cfg.tl_ch_cg_epslim;
+    h.max_iters = cfg.max_iters;
+    h.presteps = cfg.tl_ch_cg_presteps;
+    h.solver = cfg.solver;
+    h.first = sv.first;
+    h.n = 1;
+    RC(state_upload(c, h));
+
+    RC(tl_calc_residual(c));
+    launch_dot(c->s, c->g, ext_interior(c), c->cap, c->f2[F_R], c->f2[F_R], red_args(c, RED_INIT_RESID, true));
+    if (!use_jac) RC(tl_cg_init_device(c));
+    if (cfg.max_iters < 1) {
+        RC(state_download(c, &h));
+        h.halt = h.done = 1;
+        h.exit_n = cfg.max_iters + 1;
+        RC(state_upload(c, h));
+    }
+
+    const bool timed = c->timing;
+    RC(state_download(c, &h));
+    if (!h.halt) {
+        if (use_jac) {
+            RC(jacobi_graph(c));
+            RC(run_batches(c, c->g_jacobi.exec, false, cfg.max_iters / 16 + 4));
+        } else {
+            RC(cg_graph(c, timed));
+            RC(run_batches(c, c->g_cg.exec, timed, cfg.max_iters / 16 + 4));
+        }
+        RC(state_download(c, &h));
+    }
+    int est_itc = 0;
+    if (!h.done && h.switch_n > 0 && (use_ch || use_ppcg)) {
+        const int n = h.switch_n;
+        const int max_cheby_iters = cfg.max_iters - n + 2;
+        if (sv.first) {
+            std::vector<double> a(n > 1 ? n - 1 : 1), b(n > 1 ? n - 1 : 1);
+            if (n > 1) {
+                hipMemcpy(a.data(), c->hist_a + 1, sizeof(double) * (n - 1), hipMemcpyDeviceToHost);
+                hipMemcpy(b.data(), c->hist_b + 1, sizeof(double) * (n - 1), hipMemcpyDeviceToHost);
+            }
+            const int info = calc_eigenvalues(a.data(), b.data(), n - 1, &sv.eigmin, &sv.eigmax);
+            sv.first = 0;
+            if (info != 0) {
+                set_error("tea_leaf: Error in calculating eigenvalues");
+                return TL_ERR_SOLVER;
+            }
+            sv.eigmin = sv.eigmin * 0.95;
+            sv.eigmax = sv.eigmax * 1.05;
+        }
+        std::vector<double> ca, cb;
+        calc_ch_coefs(ca, cb, sv.eigmin, sv.eigmax, &sv.theta, use_ch ? max_cheby_iters : cfg.tl_ppcg_inner_steps);
+        sv.cn = sv.eigmax / sv.eigmin;
+        if (lg)
+            fprintf(lg, "SWITCH %d %d %.17e %.17e %.17e %.17e %.17e\n", step, n, h.rro, sv.eigmin, sv.eigmax, sv.cn,
+                    h.old_error);
+        if (use_ch) {
+            // tea_leaf_cheby_first_step (tea_solve.f90:477-521)
+            double bb, err1;
+            RC(tl_calc_2norm(c, 0, &bb));
+            RC(tl_cheby_init(c, ca.data(), cb.data(), max_cheby_iters, rx, ry, sv.theta));
+            RC(tl_cheby_iterate(c, 1));
+            RC(tl_calc_2norm(c, 1, &err1));
+            const double it_alpha = cfg.eps * bb / (4.0 * err1);
+            const double gamm = (sqrt(sv.cn) - 1.0) / (sqrt(sv.cn) + 1.0);
+            est_itc = (int)lround(log(it_alpha) / (2.0 * log(gamm)));
+            if (lg) fprintf(lg, "ESTITC %d %d\n", step, est_itc);
+            // the rest of iteration n: cheby_calc_steps = 1 then +1, and the
+            // loop tail on the stale error (tea_solve.f90:269, 322-331)
+            RC(state_download(c, &h));
+            h.old_error = err1;
+            h.cheby_calc_steps = 2;
+            h.est_itc = est_itc;
+            h.theta = sv.theta;
+            h.error = sqrt(h.error);
+            hipMemcpy(c->hist_e + n, &h.error, sizeof(double), hipMemcpyHostToDevice);
+            h.halt = 0;
+            if (fabs(h.error) < h.eps * h.initial_residual) {
+                h.done = h.halt = 1;
+                h.exit_n = n;
+            } else {
+                h.old_error = h.error;
+                if (n >= cfg.max_iters) {
+                    h.done = h.halt = 1;
+                    h.exit_n = cfg.max_iters + 1;
+                } else {
+                    h.n = n + 1;
+                }
+            }
+            RC(state_upload(c, h));
+            if (!h.halt) {
+                RC(cheby_graph(c));
+                RC(run_batches(c, c->g_cheby.exec, false, cfg.max_iters / 16 + 4));
+            }
+        } else {
+            RC(tl_ppcg_init(c, ca.data(), cb.data(), sv.theta, cfg.tl_ppcg_inner_steps));
+            RC(state_download(c, &h));
+            h.theta = sv.theta;
+            h.halt = 0;
+            RC(state_upload(c, h));
+            RC(ppcg_graph(c, cfg.tl_ppcg_inner_steps));
+            RC(run_batches(c, c->g_ppcg.exec, false, cfg.max_iters + 4));
+        }
+        RC(state_download(c, &h));
+    }
+    const int executed = (h.exit_n > cfg.max_iters) ? cfg.max_iters : h.exit_n;
+    if (use_jac && (executed & 1)) {
+        // the last sweep wrote the second buffer
+        launch_copy(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U2], c->f2[F_U]);
+    }
+    double exact = NAN;
+    if (cfg.tl_check_result) {
+        RC(tl_calc_residual(c));
+        RC(tl_calc_2norm(c, 1, &exact));
+        exact = sqrt(exact) / h.initial_residual;
+    }
+    const int ppcg_inner = use_ppcg ? h.cheby_calc_steps * cfg.tl_ppcg_inner_steps : 0;
+    if (lg && cfg.verbose_on) {
+        fprintf(lg, "INITRES %d %.17e\n", step, h.initial_residual);
+        std::vector<double> he(executed + 1, 0.0);
+        if (executed > 0)
+            hipMemcpy(he.data() + 1, c->hist_e + 1, sizeof(double) * executed, hipMemcpyDeviceToHost);
+        for (int i = 1; i <= executed; ++i) fprintf(lg, "RESID %d %d %.17e\n", step, i, he[i]);
+    }
+    if (lg)
+        fprintf(lg, "SOLVE %d %d %.17e %.17e %d %d %d %d %d\n", step, h.exit_n - 1, h.error / h.initial_residual,
+                exact, ppcg_inner, h.cg_calc_steps, h.cheby_calc_steps, h.switch_n, est_itc);
+    if (stats) {
+        stats->iterations += executed;
+        stats->inner += ppcg_inner;
+    }
+    RC(tl_finalise(c));
+    fields[0] = 0;
+    fields[2] = 1;
+    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, 1));
+    return TL_OK;
+}
+
+// start.f90:24-115
+int start(tl_chunk *c, const Config &cfg)
+{
+    const double dx = (cfg.xmax - cfg.xmin) / (double)(float)cfg.x_cells;
+    const double dy = (cfg.ymax - cfg.ymin) / (double)(float)cfg.y_cells;
+    RC(tl_initialise_chunk(c, cfg.xmin, cfg.ymin, dx, dy));
+    RC(tl_generate_chunk(c, cfg.nstates, cfg.density.data(), cfg.energy.data(), cfg.sxmin.data(), cfg.sxmax.data(),
+                         cfg.symin.data(), cfg.symax.data(), cfg.radius.data(), cfg.geometry.data(), 1, 2, 3));
+    const int nb[4] = {-1, -1, -1, -1};
+    const int fields[7] = {1, 1, 1, 0, 0, 0, 0};
+    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    RC(tl_set_field(c));
+    return TL_OK;
+}
+
+struct RunOut {
+    double seconds = 0;
+    Stats stats;
+    double summary[5] = {0, 0, 0, 0, 0};
+    double k1_ms = 0, k2_ms = 0;  // summed over k_samples batches
+    long k_samples = 0;
+};
+
+// diffuse.f90:43-101; when warmup >= 0 the first `warmup` steps are untimed
+// and the rest are timed into `out`
+int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut *out)
+{
+    if (cfg.sc.halo_depth < 1) {
+        set_error("Halo exchange depth unspecified or was too small");
+        return TL_ERR_CONFIG;
+    }
+    tl_chunk *c = nullptr;
+    RC(tl_chunk_create(cfg.x_cells, cfg.y_cells, cfg.sc.halo_depth, cfg.sc.preconditioner, cfg.sc.profiler_on, &c));
+    c->solver = cfg.sc.solver;
+    c->timing = (warmup >= 0);
+    int rc = start(c, cfg);
+    if (rc == TL_OK) emit_summary(lg, c, 0, false, cfg, nullptr);
+    Save sv;
+    double time = 0.0, dt = cfg.dtinit;
+    const double g_small = (double)1.0e-16f;
+    int step = 0;
+    std::chrono::steady_clock::time_point t0;
+    Stats stats;
+    while (rc == TL_OK) {
+        step++;
+        if (warmup >= 0 && step == warmup + 1) {
+            hipDeviceSynchronize();
+            c->t_k1_ms = c->t_k2_ms = 0;
+            c->t_samples = 0;
+            stats = Stats();
+            t0 = std::chrono::steady_clock::now();
+        }
+        if (cfg.dtinit <= dt) dt = cfg.dtinit;
+        rc = (mode == TL_MODE_NATIVE) ? solve_native(c, cfg, sv, dt, step, lg, &stats)
+                                      : solve_dropin(c, cfg, sv, dt, step, lg, &stats);
+        if (rc != TL_OK) break;
+        time = time + dt;
+        bool done = (time + g_small > cfg.end_time || step >= cfg.end_step);
+        if (max_steps > 0 && step >= max_steps) done = true;
+        if (cfg.summary_frequency != 0 && step % cfg.summary_frequency == 0)
+            emit_summary(lg, c, step, false, cfg, nullptr);
+        if (done) {
+            if (warmup >= 0 && out) {
+                hipDeviceSynchronize();
+                out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                out->stats = stats;
+            }
+            emit_summary(lg, c, step, true, cfg, out ? out->summary : nullptr);
+            break;
+        }
+    }
+    if (rc == TL_OK && cfg.sc.profiler_on) tl_print_profiling(c);
+    if (out) {
+        out->k1_ms = c->t_k1_ms;
+        out->k2_ms = c->t_k2_ms;
+        out->k_samples = c->t_samples;
+    }
+    tl_chunk_destroy(c);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int tl_run_deck(const char *deck_path, const char *log_path, int mode, int max_steps)
+{
+    Config cfg;
+    std::string err;
+    if (parse_deck(deck_path, &cfg, &err)) {
+        set_error("%s", err.c_str());
+        return TL_ERR_CONFIG;
+    }
+    FILE *lg = nullptr;
+    if (log_path && log_path[0]) {
+        lg = fopen(log_path, "w");
+        if (!lg) {
+            set_error("cannot open %s", log_path);
+            return TL_ERR_ARG;
+        }
+    }
+    const int rc = run(cfg, lg, mode, max_steps, -1, nullptr);
+    if (lg) fclose(lg);
+    return rc;
+}
+
+extern "C" int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, tl_bench_result *out)
+{
+    Config cfg;
+    std::string err;
+    if (parse_deck(deck_path, &cfg, &err)) {
+        set_error("%s", err.c_str());
+        return TL_ERR_CONFIG;
+    }
+    if (warmup_steps < 0 || steps < 1) {
+        set_error("tl_bench_deck: warmup %d steps %d", warmup_steps, steps);
+        return TL_ERR_ARG;
+    }
+    cfg.end_step = warmup_steps + steps;
+    cfg.end_time = 1e300;
+    cfg.summary_frequency = 0;
+    cfg.verbose_on = 0;
+    RunOut ro;
+    const int rc = run(cfg, nullptr, mode, 0, warmup_steps, &ro);
+    if (rc != TL_OK) return rc;
+    memset(out, 0, sizeof *out);
+    out->seconds = ro.seconds;
+    out->iterations = ro.stats.iterations;
+    out->inner_iterations = ro.stats.inner;
+    out->k_samples = ro.k_samples;
+    out->k1_ms = ro.k_samples ? ro.k1_ms / ro.k_samples : -1.0;
+    out->k2_ms = ro.k_samples ? ro.k2_ms / ro.k_samples : -1.0;
+    for (int i = 0; i < 5; ++i) out->final_summary[i] = ro.summary[i];
+    return TL_OK;
+}

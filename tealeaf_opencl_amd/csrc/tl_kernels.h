@@ -1,0 +1,85 @@
+// Host-side launchers of the kernels in tl_kernels.hip. Every launcher
+// enqueues on the given stream and returns immediately; `cap` bounds the
+// number of blocks (and so the number of reduction partials).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tl_common.h"
+
+namespace tl {
+
+void launch_init_chunk_1d(hipStream_t s, const Geom &g, double xmin, double ymin, double dx, double dy,
+                          double *const *f1);
+void launch_init_chunk_2d(hipStream_t s, const Geom &g, const Ext &e, int cap, double dx, double dy,
+                          double *vol, double *xa, double *ya);
+// sarr: density, energy, xmin, xmax, ymin, ymax, radius (device arrays)
+void launch_generate(hipStream_t s, const Geom &g, const Ext &e, int cap, double *const *f1, double *rho,
+                     double *e0, double *u, double *u0, int nstates, const double *const *sarr,
+                     const int *sgeom, int g_rect, int g_circ, int g_point);
+void launch_copy(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src, double *dst);
+void launch_field_summary(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *vol,
+                          const double *rho, const double *en, const double *u, const RedArgs &ra);
+// face: 0 left, 1 right, 2 bottom, 3 top
+void launch_halo(hipStream_t s, const Geom &g, double *a, int depth, int face);
+void launch_init_common(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, double *Kx,
+                        double *Ky, double rx, double ry, int coef);
+void launch_zero_boundary(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Kx, double *Ky,
+                          const int *zb);
+void launch_init_u(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, const double *en,
+                   double *u, double *u0);
+void launch_init_jac_diag(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Mi, const double *Kx,
+                          const double *Ky);
+void launch_calc_residual(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *u,
+                          const double *u0, double *r, const double *Kx, const double *Ky);
+void launch_dot(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *a, const double *b,
+                const RedArgs &ra);
+void launch_finalise(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho,
+                     const double *u, double *en);
+void launch_block_init(hipStream_t s, const Geom &g, const Ext &e, int cap, double *cp, double *bfp,
+                       const double *Kx, const double *Ky);
+void launch_block_solve(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *r, double *z,
+                        const double *cp, const double *bfp, const double *Kx, const double *Ky);
+void launch_cg_init_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double *p,
+                      const double *r, double *z, const double *Mi, const RedArgs &ra);
+void launch_cg_calc_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *p, double *w,
+                      const double *Kx, const double *Ky, const RedArgs &ra);
+void launch_cg_calc_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double alpha,
+                       double *u, const double *p, double *r, const double *w, double *z,
+                       const double *cp, const double *bfp, const double *Mi, const double *Kx,
+                       const double *Ky, const RedArgs &ra);
+void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double beta,
+                      double *p, const double *r, const double *z);
+void launch_cheby_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int init,
+                    const double *u, const double *u0, double *p, double *r, double *w,
+                    const double *cp, const double *bfp, const double *Mi, const double *Kx,
+                    const double *Ky, const double *ch_a, const double *ch_b, double theta, int step,
+                    const SolveState *st);
+void launch_cheby_u(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
+                    const double *r, const RedArgs &ra);
+void launch_ppcg_init_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
+                         double *sd, const double *cp, const double *bfp, const double *Mi,
+                         const double *Kx, const double *Ky, double theta);
+void launch_ppcg_update_r(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, double *r,
+                          const double *Kx, const double *Ky, const double *sd);
+void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
+                         double *sd, const double *cp, const double *bfp, const double *Mi,
+                         const double *Kx, const double *Ky, const double *ch_a, const double *ch_b,
+                         int step);
+void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
+                   const double *u0, double *u1, const double *un, const RedArgs &ra);
+// p is double-buffered (pA, pB); the iteration counter in the state picks
+// which buffer is read and which is written
+void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
+                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra);
+void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
+                        const double *pA, const double *pB, double *r, double *z, double *sd,
+                        const double *Kx, const double *Ky, const RedArgs &ra);
+void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
+                         const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra);
+void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
+                             double *u, double *r, const double *sd_in, double *sd_out,
+                             const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                             const double *ch_b, int step, const RedArgs &ra);
+
+}  // namespace tl

@@ -1,0 +1,1418 @@
+// HIP kernels for gfx950 (MI355X): the TeaLeaf implicit diffusion solve.
+//
+// Work decomposition (every kernel): a 256-thread block is four wavefronts;
+// a tile is 64 columns x 16 rows; wavefront w of the block owns the 4-row
+// strip w of the tile and lane l owns column l. A thread therefore handles one
+// column of one 4-row strip, which is exactly the y-strip of the strip-4
+// block-Jacobi preconditioner (kernel_files/tea_block_jacobi.cl), so the
+// tridiagonal solve of a strip stays in one thread's registers. Each row of a
+// strip is one fully coalesced 512-byte load per wavefront. Blocks stride over
+// tiles; the grid is capped so reductions produce few partials.
+//
+// Reductions: wavefront shuffle sum, then the four wavefronts in fixed order,
+// then one partial per block; the last block to finish (atomic ticket) sums
+// the partials in block order and runs the scalar algebra of the iteration.
+// The result is deterministic for a given grid.
+//
+// Two families:
+//  * reference-granularity kernels (one per kernel_files/*.cl kernel, same
+//    inputs and outputs) behind the drop-in entry points;
+//  * fused iteration kernels for the device-resident driver: the CG search
+//    direction update is folded into the next matvec (p is double-buffered so
+//    neighbours are read before they are overwritten), w = A p is recomputed
+//    instead of stored, and the diagonal / block-Jacobi factors are
+//    recomputed in registers from Kx, Ky with the same expressions as
+//    tea_leaf_init_jac_diag and tea_leaf_block_init.
+#include <hip/hip_runtime.h>
+
+#include "tl_common.h"
+#include "tl_kernels.h"
+
+namespace tl {
+
+// ---------------------------------------------------------------------------
+// tiling helpers
+// ---------------------------------------------------------------------------
+#define TL_TILE_LOOP(e)                                                          \
+    const int tx_ = threadIdx.x & (TILE_X - 1);                                  \
+    const int ty_ = threadIdx.x / TILE_X;                                        \
+    const int nt_ = (e).ntx * (e).nty;                                           \
+    for (int t_ = blockIdx.x; t_ < nt_; t_ += gridDim.x) {                       \
+        const int j = (e).j0 + (t_ % (e).ntx) * TILE_X + tx_;                    \
+        const int ks = (e).k0 + ((t_ / (e).ntx) * STRIPS_PER_TILE + ty_) * STRIP; \
+        if (j > (e).j1 || ks > (e).k1) continue;
+#define TL_TILE_LOOP_END }
+
+__device__ __forceinline__ int IX(const Geom &g, int j, int k) { return k * g.pitch + j; }
+
+// 5-point operator of tea_leaf_common_cl.cl:28-32 (same association)
+__device__ __forceinline__ double opA(double kx0, double kx1, double ky0, double ky1, double vc,
+                                      double vw, double ve, double vs, double vn)
+{
+    return (1.0 + (ky1 + ky0) + (kx1 + kx0)) * vc - (ky1 * vn + ky0 * vs) - (kx1 * ve + kx0 * vw);
+}
+
+__device__ __forceinline__ double diag_of(double kx0, double kx1, double ky0, double ky1)
+{
+    return (1.0 + (ky1 + ky0) + (kx1 + kx0));
+}
+
+// BLOCK_TOP of macros_cl.cl:6-10 (BLOCK_TOP_CHECK only when y_max % 4 != 0)
+__device__ __forceinline__ int block_top(const Geom &g, int row)
+{
+    if (g.y_max % STRIP) {
+        const int t = g.y_max + 2 - row;
+        return t < STRIP ? t : STRIP;
+    }
+    return STRIP;
+}
+
+// block_solve_func (tea_block_jacobi.cl:6-42) on one strip held in registers.
+// b[k] = COEF_B, a[k] = COEF_A of strip row k; cp/bfp as produced by
+// tea_leaf_block_init; rows >= upper are left at zero.
+__device__ __forceinline__ void strip_solve(int upper, const double *r, const double *b,
+                                            const double *a, const double *cp,
+                                            const double *bfp, double *z)
+{
+    double dp[STRIP];
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) z[k] = 0.0;
+    if (upper <= 0) return;
+    dp[0] = r[0] / b[0];
+#pragma unroll
+    for (int k = 1; k < STRIP; ++k)
+        if (k < upper) dp[k] = (r[k] - a[k] * dp[k - 1]) * bfp[k];
+    // back substitution from row upper-1
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; --k) {
+        if (k > upper - 1) continue;
+        if (k == upper - 1) z[k] = dp[k];
+        else z[k] = dp[k] - cp[k] * z[k + 1];
+    }
+}
+
+// Thomas factors of tea_leaf_block_init (tea_block_jacobi.cl:78-105)
+__device__ __forceinline__ void strip_factors(int upper, const double *b, const double *a,
+                                              const double *c, double *cp, double *bfp)
+{
+    cp[0] = c[0] / b[0];
+    bfp[0] = 0.0;
+#pragma unroll
+    for (int k = 1; k < STRIP; ++k) {
+        if (k < upper) {
+            bfp[k] = 1.0 / (b[k] - a[k] * cp[k - 1]);
+            cp[k] = c[k] * bfp[k];
+        } else {
+            bfp[k] = 0.0;
+            cp[k] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// tea_solve.f90:322-331: error = SQRT(error), exit test, old_error, then the
+// solver-switch test at the top of the next iteration (:151-165)
+__device__ void iteration_tail(SolveState *s, double *hist_e)
+{
+    s->error = sqrt(s->error);
+    if (hist_e) hist_e[s->n] = s->error;
+    if (fabs(s->error) < s->eps * s->initial_residual) {
+        s->done = 1;
+        s->halt = 1;
+        s->exit_n = s->n;
+        return;
+    }
+    s->old_error = s->error;
+    if (s->n >= s->max_iters) {
+        s->done = 1;
+        s->halt = 1;
+        s->exit_n = s->max_iters + 1;
+        return;
+    }
+    s->n += 1;
+    if (s->cheby_calc_steps == 0 && (s->solver == SOLVER_CHEBY || s->solver == SOLVER_PPCG)) {
+        const bool sw = (!s->first && s->solver == SOLVER_PPCG && s->n > 1) ||
+                        (fabs(s->old_error) <= s->epslim && s->n >= s->presteps);
+        if (sw) {
+            s->halt = 1;
+            s->switch_n = s->n;
+        }
+    }
+}
+
+__device__ __forceinline__ bool cheby_norm_due(const SolveState *s)
+{
+    return s->n >= s->est_itc && (s->n % 10) == 0;
+}
+
+__device__ void finish_scalars(const RedArgs &ra, const double *tot)
+{
+#pragma unroll
+    for (int q = 0; q < MAX_RED; ++q) ra.result[q] = tot[q];
+    SolveState *s = ra.st;
+    if (!s) return;
+    switch (ra.mode) {
+    case RED_INIT_RESID:
+        s->old_error = tot[0];
+        s->initial_residual = sqrt(tot[0]);
+        s->n = 1;
+        break;
+    case RED_INIT_RRO: {
+        s->rro = tot[0];
+        s->beta = 0.0;
+        s->n = 1;
+        if (s->solver == SOLVER_CHEBY || s->solver == SOLVER_PPCG) {
+            const bool sw = fabs(s->old_error) <= s->epslim && s->n >= s->presteps;
+            if (sw) {
+                s->halt = 1;
+                s->switch_n = 1;
+            }
+        }
+        break;
+    }
+    case RED_CG_PW:
+        s->pw = tot[0];
+        s->alpha = s->rro / s->pw;
+        if (ra.hist_a) ra.hist_a[s->n] = s->alpha;
+        break;
+    case RED_CG_RRN:
+        s->rrn = tot[0];
+        s->beta = s->rrn / s->rro;
+        if (ra.hist_b) ra.hist_b[s->n] = s->beta;
+        s->error = s->rrn;
+        s->rro = s->rrn;
+        s->cg_calc_steps += 1;
+        iteration_tail(s, ra.hist_e);
+        break;
+    case RED_PPCG_PW:
+        s->pw = tot[0];
+        s->alpha = s->rro / s->pw;
+        break;
+    case RED_PPCG_RRN:
+        s->rrn = tot[0];
+        s->beta = s->rrn / s->rro;
+        s->error = s->rrn;
+        s->rro = s->rrn;
+        s->cheby_calc_steps += 1;
+        iteration_tail(s, ra.hist_e);
+        break;
+    case RED_JACOBI:
+        s->error = tot[0];
+        iteration_tail(s, ra.hist_e);
+        break;
+    case RED_CHEBY:
+        if (cheby_norm_due(s)) s->error = tot[0];
+        s->cheby_calc_steps += 1;
+        iteration_tail(s, ra.hist_e);
+        break;
+    case RED_EXACT:
+        s->exact_error = sqrt(tot[0]);
+        break;
+    default:
+        break;
+    }
+}
+
+// Block reduction of NR sums plus the last-block finish. Must be reached by
+// every thread of every block that has not returned early as a whole grid.
+template <int NR>
+__device__ __forceinline__ void reduce_finish(const double (&v)[NR], const RedArgs &ra)
+{
+    __shared__ double lds[STRIPS_PER_TILE][MAX_RED];
+    __shared__ int am_last;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) lds[w][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < MAX_RED; ++q) {
+            double s = 0.0;
+            if (q < NR) s = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+            __hip_atomic_store(&ra.partials[blockIdx.x * MAX_RED + q], s, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __threadfence();
+        const unsigned ticket = atomicAdd(ra.counter, 1u);
+        am_last = (ticket == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!am_last) return;
+    __threadfence();
+    double acc[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            acc[q] += __hip_atomic_load(&ra.partials[b * MAX_RED + q], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const double s = wave_sum(acc[q]);
+        if (lane == 0) lds[w][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < NR; ++q) tot[q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+        *ra.counter = 0u;
+        finish_scalars(ra, tot);
+    }
+}
+
+__device__ __forceinline__ bool halted(const SolveState *st) { return st && st->halt; }
+
+// ===========================================================================
+// reference-granularity kernels
+// ===========================================================================
+
+// initialise_chunk_first (initialise_chunk_cl.cl:3-49): 1-D coordinate arrays
+__global__ void k_init_chunk_1d(Geom g, double xmin, double ymin, double dx, double dy,
+                                double *vx, double *vdx, double *vy, double *vdy, double *cx,
+                                double *cdx, double *cy, double *cdy)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int hd = g.hd;
+    if (i <= g.x_max + hd) {
+        vx[i] = xmin + dx * (double)(((i)-1) - hd + 1);
+        vdx[i] = dx;
+        const double vplus = xmin + dx * (double)((i)-hd + 1);
+        cx[i] = 0.5 * (vx[i] + vplus);
+        cdx[i] = dx;
+    }
+    if (i <= g.y_max + hd) {
+        vy[i] = ymin + dy * (double)(((i)-1) - hd + 1);
+        vdy[i] = dy;
+        const double vplus = ymin + dy * (double)((i)-hd + 1);
+        cy[i] = 0.5 * (vy[i] + vplus);
+        cdy[i] = dy;
+    }
+}
+
+// initialise_chunk_second (initialise_chunk_cl.cl:51-68)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_chunk_2d(Geom g, Ext e, double dx, double dy,
+                                                                 double *vol, double *xa, double *ya)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        vol[IX(g, j, k)] = dx * dy;
+        xa[IX(g, j, k)] = dy;  // xarea lives in its own pitched buffer
+        ya[IX(g, j, k)] = dx;
+    }
+    TL_TILE_LOOP_END
+}
+
+// generate_chunk_init (generate_chunk_cl.cl:82-95) + every later state in
+// order (:3-65, one pass per cell instead of one launch per state) +
+// generate_chunk_init_u from energy0 (:67-80)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_generate(
+    Geom g, Ext e, const double *vx, const double *vy, const double *cx, const double *cy,
+    double *rho, double *e0, double *u, double *u0, int nstates, const double *sdens,
+    const double *sener, const double *sxmin, const double *sxmax, const double *symin,
+    const double *symax, const double *srad, const int *sgeom, int g_rect, int g_circ, int g_point)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        double en = sener[0], de = sdens[0];
+        for (int s = 1; s < nstates; ++s) {
+            bool hit = false;
+            if (g_rect == sgeom[s]) {
+                hit = (vx[1 + j] >= sxmin[s] && vx[j] < sxmax[s] && vy[1 + k] >= symin[s] &&
+                       vy[k] < symax[s]);
+            } else if (sgeom[s] == g_circ) {
+                const double xp = cx[j] - sxmin[s], yp = cy[k] - symin[s];
+                const double rad = sqrt(xp * xp + yp * yp);
+                hit = (rad <= srad[s]);
+            } else if (sgeom[s] == g_point) {
+                hit = (vx[j] == sxmin[s] && vy[k] == symin[s]);
+            }
+            if (hit) {
+                en = sener[s];
+                de = sdens[s];
+            }
+        }
+        const int i = IX(g, j, k);
+        e0[i] = en;
+        rho[i] = de;
+        u[i] = en * de;
+        u0[i] = en * de;
+    }
+    TL_TILE_LOOP_END
+}
+
+// set_field (set_field_cl.cl:3-13) and generic interior copy
+__global__ void __launch_bounds__(BLOCK_THREADS) k_copy(Geom g, Ext e, const double *src, double *dst)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        dst[IX(g, j, k)] = src[IX(g, j, k)];
+    }
+    TL_TILE_LOOP_END
+}
+
+// field_summary (field_summary_cl.cl:3-43), energy1 bound to `energy`
+__global__ void __launch_bounds__(BLOCK_THREADS) k_field_summary(Geom g, Ext e, const double *vol,
+                                                                 const double *rho, const double *en,
+                                                                 const double *u, RedArgs ra)
+{
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const double cv = vol[i];
+        const double cm = cv * rho[i];
+        acc[0] += cv;
+        acc[1] += cm;
+        acc[2] += cm * en[i];
+        acc[3] += cm * u[i];
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<4>(acc, ra);
+}
+
+// update_halo_{bottom,top} (update_halo_cl.cl:43-69): one thread per column
+// of the depth rows; face 2 = bottom, 3 = top
+__global__ void k_halo_bt(Geom g, double *a, int depth, int face)
+{
+    const int j = (g.hd - depth) + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > g.x_max + g.hd - 1 + depth) return;
+    for (int k = g.hd - depth; k <= g.hd - 1; ++k) {
+        if (face == 2) a[IX(g, j, k)] = 1 * a[IX(g, j, 2 * g.hd - 1 - k)];
+        else a[IX(g, j, g.y_max + 2 * g.hd - 1 - k)] = a[IX(g, j, g.y_max + k)];
+    }
+}
+
+// update_halo_{left,right} (update_halo_cl.cl:3-41); face 0 = left, 1 = right
+__global__ void k_halo_lr(Geom g, double *a, int depth, int face)
+{
+    const int k = (g.hd - depth) + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > g.y_max + g.hd - 1 + depth) return;
+    for (int j = g.hd - depth; j <= g.hd - 1; ++j) {
+        if (face == 0) a[IX(g, j, k)] = 1 * a[IX(g, 2 * g.hd - 1 - j, k)];
+        else a[IX(g, g.x_max + 2 * g.hd - 1 - j, k)] = a[IX(g, g.x_max + j, k)];
+    }
+}
+
+// tea_leaf_init_common (tea_leaf_common_cl.cl:56-90), launch bounds (1-hd, hd)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_common(Geom g, Ext e, const double *rho,
+                                                               double *Kx, double *Ky, double rx,
+                                                               double ry, int coef)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        double dc, dl, dd;
+        if (coef == COEF_CONDUCTIVITY) {
+            dc = rho[i];
+            dl = rho[i - 1];
+            dd = rho[i - g.pitch];
+        } else {
+            dc = 1.0 / rho[i];
+            dl = 1.0 / rho[i - 1];
+            dd = 1.0 / rho[i - g.pitch];
+        }
+        double kx = (dl + dc) / (2.0 * dl * dc);
+        kx *= rx;
+        double ky = (dd + dc) / (2.0 * dd * dc);
+        ky *= ry;
+        Kx[i] = kx;
+        Ky[i] = ky;
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_zero_boundary (tea_leaf_common_cl.cl:92-125), launch bounds (-hd, hd)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_zero_boundary(Geom g, Ext e, double *Kx, double *Ky,
+                                                                 int zl, int zr, int zb, int zt)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const int hd = g.hd;
+        if (zl && j <= hd) {
+            Kx[i] = 0;
+            if (j < hd) Ky[i] = 0;
+        }
+        if (zr && j > (g.x_max - 1) + hd) {
+            Kx[i] = 0;
+            Ky[i] = 0;
+        }
+        if (zb && k <= hd) {
+            Ky[i] = 0;
+            if (k < hd) Kx[i] = 0;
+        }
+        if (zt && k > (g.y_max - 1) + hd) {
+            Ky[i] = 0;
+            Kx[i] = 0;
+        }
+    }
+    TL_TILE_LOOP_END
+}
+
+// generate_chunk_init_u with energy1 (tea_leaf_kernel_ocl.cpp:287-288)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_u(Geom g, Ext e, const double *rho,
+                                                          const double *en, double *u, double *u0)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        u[i] = en[i] * rho[i];
+        u0[i] = en[i] * rho[i];
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_init_jac_diag (tea_leaf_common_cl.cl:127-142), launch bounds (-hd, hd)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_init_jac_diag(Geom g, Ext e, double *Mi,
+                                                                 const double *Kx, const double *Ky)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const double d = (1.0 + (Ky[i + g.pitch] + Ky[i]) + (Kx[i + 1] + Kx[i]));
+        Mi[i] = 1.0 / d;
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_calc_residual (tea_leaf_common_cl.cl:17-36)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_calc_residual(Geom g, Ext e, const double *u,
+                                                                 const double *u0, double *r,
+                                                                 const double *Kx, const double *Ky)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const double s = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], u[i], u[i - 1], u[i + 1],
+                             u[i - g.pitch], u[i + g.pitch]);
+        r[i] = u0[i] - s;
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_calc_2norm (tea_leaf_common_cl.cl:38-54)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_dot(Geom g, Ext e, const double *a, const double *b,
+                                                       RedArgs ra)
+{
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        acc[0] += a[i] * b[i];
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+// tea_leaf_finalise (tea_leaf_common_cl.cl:4-15)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_finalise(Geom g, Ext e, const double *rho,
+                                                            const double *u, double *en)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        en[i] = u[i] / rho[i];
+    }
+    TL_TILE_LOOP_END
+}
+
+// load the strip's tridiagonal coefficients (COEF_A/B/C, tea_block_jacobi.cl:1-3)
+__device__ __forceinline__ void strip_coefs(const Geom &g, const double *Kx, const double *Ky, int j,
+                                            int ks, double *a, double *b, double *c)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        const int i = IX(g, j, ks + q);
+        const double ky0 = Ky[i], ky1 = Ky[i + g.pitch];
+        a[q] = -ky0;
+        c[q] = -ky1;
+        b[q] = (1.0 + (ky1 + ky0) + (Kx[i + 1] + Kx[i]));
+    }
+}
+
+// tea_leaf_block_init (tea_block_jacobi.cl:78-105)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_block_init(Geom g, Ext e, double *cp, double *bfp,
+                                                              const double *Kx, const double *Ky)
+{
+    TL_TILE_LOOP(e)
+    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
+    strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+    const int upper = block_top(g, ks);
+    strip_factors(upper, b, a, c, cpv, bfv);
+    cp[IX(g, j, ks)] = cpv[0];
+#pragma unroll
+    for (int q = 1; q < STRIP; ++q)
+        if (q < upper) {
+            bfp[IX(g, j, ks + q)] = bfv[q];
+            cp[IX(g, j, ks + q)] = cpv[q];
+        }
+    TL_TILE_LOOP_END
+}
+
+__device__ __forceinline__ void load_factors(const Geom &g, const double *cp, const double *bfp, int j,
+                                             int ks, double *cpv, double *bfv)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        cpv[q] = cp[IX(g, j, ks + q)];
+        bfv[q] = bfp[IX(g, j, ks + q)];
+    }
+}
+
+// gather a strip of `src` with rows above k1 read as zero (the LDS copy of
+// the reference only holds in-bounds rows)
+__device__ __forceinline__ void strip_load(const Geom &g, const double *src, int j, int ks, int k1,
+                                           double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) v[q] = (ks + q <= k1) ? src[IX(g, j, ks + q)] : 0.0;
+}
+
+// tea_leaf_block_solve (tea_block_jacobi.cl:44-76)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_block_solve(Geom g, Ext e, const double *r,
+                                                               double *z, const double *cp,
+                                                               const double *bfp, const double *Kx,
+                                                               const double *Ky)
+{
+    TL_TILE_LOOP(e)
+    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], rv[STRIP], zv[STRIP];
+    strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+    load_factors(g, cp, bfp, j, ks, cpv, bfv);
+    strip_load(g, r, j, ks, e.k1, rv);
+    strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) z[IX(g, j, ks + q)] = zv[q];
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_cg_solve_init_p (tea_leaf_cg_cl.cl:8-41)
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cg_init_p(Geom g, Ext e, double *p, const double *r,
+                                                             double *z, const double *Mi, RedArgs ra)
+{
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        double pv;
+        if (PREC == PREC_BLOCK) {
+            pv = z[i];
+        } else if (PREC == PREC_DIAG) {
+            const double zv = r[i] * Mi[i];
+            z[i] = zv;
+            pv = zv;
+        } else {
+            pv = r[i];
+        }
+        p[i] = pv;
+        acc[0] += r[i] * pv;
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+// tea_leaf_cg_solve_calc_w (tea_leaf_cg_cl.cl:45-69)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_w(Geom g, Ext e, const double *p, double *w,
+                                                             const double *Kx, const double *Ky,
+                                                             RedArgs ra)
+{
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const double wv = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], p[i], p[i - 1], p[i + 1],
+                              p[i - g.pitch], p[i + g.pitch]);
+        w[i] = wv;
+        acc[0] += p[i] * wv;
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+// tea_leaf_cg_solve_calc_ur (tea_leaf_cg_cl.cl:73-137)
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, double alpha, double *u,
+                                                              const double *p, double *r,
+                                                              const double *w, double *z,
+                                                              const double *cp, const double *bfp,
+                                                              const double *Mi, const double *Kx,
+                                                              const double *Ky, RedArgs ra)
+{
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    double rv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        rv[q] = 0.0;
+        const int k = ks + q;
+        if (k > e.k1) continue;
+        const int i = IX(g, j, k);
+        u[i] += alpha * p[i];
+        const double rn = r[i] - alpha * w[i];
+        r[i] = rn;
+        rv[q] = rn;
+    }
+    if (PREC == PREC_BLOCK) {
+        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
+        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+        load_factors(g, cp, bfp, j, ks, cpv, bfv);
+        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q)
+            if (ks + q <= e.k1) {
+                z[IX(g, j, ks + q)] = zv[q];
+                acc[0] += rv[q] * zv[q];
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1) continue;
+            const int i = IX(g, j, ks + q);
+            if (PREC == PREC_DIAG) {
+                const double zv = rv[q] * Mi[i];
+                z[i] = zv;
+                acc[0] += rv[q] * zv;
+            } else {
+                acc[0] += rv[q] * rv[q];
+            }
+        }
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+// tea_leaf_cg_solve_calc_p (tea_leaf_cg_cl.cl:141-160)
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_p(Geom g, Ext e, double beta, double *p,
+                                                             const double *r, const double *z)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        if (PREC != PREC_NONE) p[i] = z[i] + beta * p[i];
+        else p[i] = r[i] + beta * p[i];
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_cheby_solve_init_p (INIT=1, tea_leaf_cheby_cl.cl:4-64) and
+// tea_leaf_cheby_solve_calc_p (INIT=0, :78-145). The coefficient index comes
+// from `step` or, when st is set, from the device-resident step counter.
+template <int PREC, int INIT>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_p(Geom g, Ext e, const double *u,
+                                                           const double *u0, double *p, double *r,
+                                                           double *w, const double *cp,
+                                                           const double *bfp, const double *Mi,
+                                                           const double *Kx, const double *Ky,
+                                                           const double *ch_a, const double *ch_b,
+                                                           double theta, int step,
+                                                           const SolveState *st)
+{
+    if (halted(st)) return;
+    if (st) step = st->cheby_calc_steps - 1;
+    const double ca = INIT ? 0.0 : ch_a[step], cb = INIT ? 0.0 : ch_b[step];
+    TL_TILE_LOOP(e)
+    double rv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        rv[q] = 0.0;
+        const int k = ks + q;
+        if (k > e.k1) continue;
+        const int i = IX(g, j, k);
+        const double wv = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], u[i], u[i - 1], u[i + 1],
+                              u[i - g.pitch], u[i + g.pitch]);
+        if (w) w[i] = wv;
+        const double rn = u0[i] - wv;
+        r[i] = rn;
+        rv[q] = rn;
+    }
+    if (PREC == PREC_BLOCK) {
+        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
+        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+        load_factors(g, cp, bfp, j, ks, cpv, bfv);
+        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q)
+            if (ks + q <= e.k1) {
+                const int i = IX(g, j, ks + q);
+                if (INIT) p[i] = zv[q] / theta;
+                else p[i] = ca * p[i] + cb * zv[q];
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1) continue;
+            const int i = IX(g, j, ks + q);
+            if (PREC == PREC_DIAG) {
+                if (INIT) p[i] = (Mi[i] * rv[q]) / theta;
+                else p[i] = ca * p[i] + cb * Mi[i] * rv[q];
+            } else {
+                if (INIT) p[i] = rv[q] / theta;
+                else p[i] = ca * p[i] + cb * rv[q];
+            }
+        }
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_cheby_solve_calc_u (tea_leaf_cheby_cl.cl:66-76). With st set this
+// is the last kernel of a device-resident Chebyshev iteration: it also sums
+// r.r when tea_solve.f90:223 asks for it and runs the iteration tail.
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_u(Geom g, Ext e, double *u, const double *p,
+                                                           const double *r, RedArgs ra)
+{
+    if (halted(ra.st)) return;
+    const bool want = ra.st && cheby_norm_due(ra.st);
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        u[i] += p[i];
+        if (want) acc[0] += r[i] * r[i];
+    }
+    TL_TILE_LOOP_END
+    if (ra.st) reduce_finish<1>(acc, ra);
+}
+
+// tea_leaf_ppcg_solve_init_sd (tea_leaf_ppcg_cl.cl:4-60)
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_init_sd(Geom g, Ext e, const double *r,
+                                                                double *sd, const double *cp,
+                                                                const double *bfp, const double *Mi,
+                                                                const double *Kx, const double *Ky,
+                                                                double theta)
+{
+    TL_TILE_LOOP(e)
+    if (PREC == PREC_BLOCK) {
+        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], rv[STRIP], zv[STRIP];
+        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+        load_factors(g, cp, bfp, j, ks, cpv, bfv);
+        strip_load(g, r, j, ks, e.k1, rv);
+        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q)
+            if (ks + q <= e.k1) sd[IX(g, j, ks + q)] = zv[q] / theta;
+    } else {
+        for (int q = 0; q < STRIP; ++q) {
+            const int k = ks + q;
+            if (k > e.k1) break;
+            const int i = IX(g, j, k);
+            if (PREC == PREC_DIAG) sd[i] = r[i] * Mi[i] / theta;
+            else sd[i] = r[i] / theta;
+        }
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_ppcg_solve_update_r (tea_leaf_ppcg_cl.cl:62-88) on the extent e
+__global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_update_r(Geom g, Ext e, double *u, double *r,
+                                                                 const double *Kx, const double *Ky,
+                                                                 const double *sd)
+{
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        u[i] += sd[i];
+        const double res = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], sd[i], sd[i - 1], sd[i + 1],
+                               sd[i - g.pitch], sd[i + g.pitch]);
+        r[i] -= res;
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_ppcg_solve_calc_sd (tea_leaf_ppcg_cl.cl:90-155) on the extent e
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_calc_sd(Geom g, Ext e, const double *r,
+                                                                double *sd, const double *cp,
+                                                                const double *bfp, const double *Mi,
+                                                                const double *Kx, const double *Ky,
+                                                                const double *ch_a,
+                                                                const double *ch_b, int step)
+{
+    const double ca = ch_a[step], cb = ch_b[step];
+    TL_TILE_LOOP(e)
+    if (PREC == PREC_BLOCK) {
+        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], rv[STRIP], zv[STRIP];
+        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+        load_factors(g, cp, bfp, j, ks, cpv, bfv);
+        strip_load(g, r, j, ks, e.k1, rv);
+        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q)
+            if (ks + q <= e.k1) {
+                const int i = IX(g, j, ks + q);
+                sd[i] = ca * sd[i] + cb * zv[q];
+            }
+    } else {
+        for (int q = 0; q < STRIP; ++q) {
+            const int k = ks + q;
+            if (k > e.k1) break;
+            const int i = IX(g, j, k);
+            if (PREC == PREC_DIAG) sd[i] = ca * sd[i] + cb * r[i] * Mi[i];
+            else sd[i] = ca * sd[i] + cb * r[i];
+        }
+    }
+    TL_TILE_LOOP_END
+}
+
+// tea_leaf_jacobi_solve (tea_leaf_jacobi_cl.cl:15-42); `un` holds the
+// previous iterate (the reference copies u into the Mi buffer first)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const double *Kx,
+                                                          const double *Ky, const double *u0,
+                                                          double *u1, const double *un, RedArgs ra)
+{
+    if (halted(ra.st)) return;
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const int P = g.pitch;
+        const double v = (u0[i] + Kx[i + 1] * un[i + 1] + Kx[i] * un[i - 1] + Ky[i + P] * un[i + P] +
+                          Ky[i] * un[i - P]) /
+                         (1.0 + (Kx[i] + Kx[i + 1]) + (Ky[i] + Ky[i + P]));
+        u1[i] = v;
+        acc[0] += fabs(v - un[i]);
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+// ===========================================================================
+// fused iteration kernels (device-resident driver)
+// ===========================================================================
+// Jacobi sweep of the device-resident driver: the previous iterate is the
+// other buffer of a pair (iteration n reads buffer (n-1)&1), which replaces
+// the reference's copy of u into the Mi buffer (tea_leaf_jacobi_cl.cl:3-13).
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_jacobi(Geom g, Ext e, const double *Kx,
+                                                                const double *Ky, const double *u0,
+                                                                double *uA, double *uB, RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    const bool odd = (st->n & 1) != 0;
+    const double *un = odd ? uA : uB;
+    double *u1 = odd ? uB : uA;
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    for (int q = 0; q < STRIP; ++q) {
+        const int k = ks + q;
+        if (k > e.k1) break;
+        const int i = IX(g, j, k);
+        const int P = g.pitch;
+        const double v = (u0[i] + Kx[i + 1] * un[i + 1] + Kx[i] * un[i - 1] + Ky[i + P] * un[i + P] +
+                          Ky[i] * un[i - P]) /
+                         (1.0 + (Kx[i] + Kx[i + 1]) + (Ky[i] + Ky[i + P]));
+        u1[i] = v;
+        acc[0] += fabs(v - un[i]);
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+
+// CG / PPCG outer step, first half: p_new = z + beta*p_old (r + beta*p_old
+// without preconditioner; tea_leaf_cg_cl.cl:141-160 of the previous
+// iteration), w = A p_new, p_new.w (tea_leaf_cg_cl.cl:45-69). p_new at the
+// four neighbours is formed from the old buffers, so no cell is read after
+// it is overwritten. Stores p_new only; w is recomputed by the second half.
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, const double *src,
+                                                              double *pA, double *pB,
+                                                              const double *Kx, const double *Ky,
+                                                              RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    const double beta = st->beta;
+    // iteration n reads the previous direction from buffer (n-1)&1
+    const bool odd = (st->n & 1) != 0;
+    const double *p_old = odd ? pA : pB;
+    double *p_new = odd ? pB : pA;
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    const int P = g.pitch;
+    const int i0 = IX(g, j, ks);
+    // column j, rows ks-1 .. ks+4
+    double pc[STRIP + 2];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) {
+        const int i = i0 + (q - 1) * P;
+        pc[q] = src[i] + beta * p_old[i];
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        const double pw_ = src[i - 1] + beta * p_old[i - 1];
+        const double pe_ = src[i + 1] + beta * p_old[i + 1];
+        const double wv = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + P], pc[q + 1], pw_, pe_, pc[q], pc[q + 2]);
+        p_new[i] = pc[q + 1];
+        acc[0] += pc[q + 1] * wv;
+    }
+    TL_TILE_LOOP_END
+    reduce_finish<1>(acc, ra);
+}
+
+// CG / PPCG outer step, second half (tea_leaf_cg_cl.cl:73-137): w = A p
+// recomputed, u += alpha p, r -= alpha w, z = M^-1 r, r.z. PPCG=1 also
+// writes sd = z/theta (tea_leaf_ppcg_cl.cl:4-60 on the same r) and skips the
+// unused r.z sum.
+template <int PREC, int PPCG>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, double *u,
+                                                               const double *pA, const double *pB,
+                                                               double *r, double *z,
+                                                               double *sd, const double *Kx,
+                                                               const double *Ky, RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    const double *p = (st->n & 1) ? pB : pA;
+    const double alpha = st->alpha;
+    const double theta = PPCG ? st->theta : 1.0;
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    const int P = g.pitch;
+    const int i0 = IX(g, j, ks);
+    double pc[STRIP + 2], kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) pc[q] = p[i0 + (q - 1) * P];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        kx0[q] = Kx[i0 + q * P];
+        kx1[q] = Kx[i0 + q * P + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = Ky[i0 + q * P];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        rv[q] = 0.0;
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        const double wv = opA(kx0[q], kx1[q], ky[q], ky[q + 1], pc[q + 1], p[i - 1], p[i + 1], pc[q], pc[q + 2]);
+        u[i] += alpha * pc[q + 1];
+        const double rn = r[i] - alpha * wv;
+        r[i] = rn;
+        rv[q] = rn;
+    }
+    if (PREC == PREC_BLOCK) {
+        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            a[q] = -ky[q];
+            c[q] = -ky[q + 1];
+            b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
+        }
+        const int upper = block_top(g, ks);
+        strip_factors(upper, b, a, c, cpv, bfv);
+        strip_solve(upper, rv, b, a, cpv, bfv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q)
+            if (ks + q <= e.k1) {
+                const int i = i0 + q * P;
+                z[i] = zv[q];
+                if (PPCG) sd[i] = zv[q] / theta;
+                else acc[0] += rv[q] * zv[q];
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1) continue;
+            const int i = i0 + q * P;
+            if (PREC == PREC_DIAG) {
+                const double mi = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
+                const double zv = rv[q] * mi;
+                z[i] = zv;
+                if (PPCG) sd[i] = zv / theta;
+                else acc[0] += rv[q] * zv;
+            } else {
+                if (PPCG) sd[i] = rv[q] / theta;
+                else acc[0] += rv[q] * rv[q];
+            }
+        }
+    }
+    TL_TILE_LOOP_END
+    if (!PPCG) reduce_finish<1>(acc, ra);
+}
+
+// PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
+// sd is double-buffered. LAST=1 also forms r.z with the z of the outer step
+// (tea_leaf_ppcg.f90:41-60 via calc_2norm(2)) and runs the iteration tail.
+template <int PREC, int LAST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
+    Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *Kx,
+    const double *Ky, const double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    const double ca = ch_a[step], cb = ch_b[step];
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    const int P = g.pitch;
+    const int i0 = IX(g, j, ks);
+    double sc[STRIP + 2], kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) sc[q] = sd_in[i0 + (q - 1) * P];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        kx0[q] = Kx[i0 + q * P];
+        kx1[q] = Kx[i0 + q * P + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = Ky[i0 + q * P];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        rv[q] = 0.0;
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        u[i] += sc[q + 1];
+        const double res = opA(kx0[q], kx1[q], ky[q], ky[q + 1], sc[q + 1], sd_in[i - 1], sd_in[i + 1], sc[q], sc[q + 2]);
+        const double rn = r[i] - res;
+        r[i] = rn;
+        rv[q] = rn;
+    }
+    if (PREC == PREC_BLOCK) {
+        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            a[q] = -ky[q];
+            c[q] = -ky[q + 1];
+            b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
+        }
+        const int upper = block_top(g, ks);
+        strip_factors(upper, b, a, c, cpv, bfv);
+        strip_solve(upper, rv, b, a, cpv, bfv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q)
+            if (ks + q <= e.k1) {
+                const int i = i0 + q * P;
+                sd_out[i] = ca * sc[q + 1] + cb * zv[q];
+                if (LAST) acc[0] += rv[q] * z[i];
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1) continue;
+            const int i = i0 + q * P;
+            if (PREC == PREC_DIAG) {
+                const double mi = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
+                sd_out[i] = ca * sc[q + 1] + cb * rv[q] * mi;
+                if (LAST) acc[0] += rv[q] * z[i];
+            } else {
+                sd_out[i] = ca * sc[q + 1] + cb * rv[q];
+                if (LAST) acc[0] += rv[q] * rv[q];
+            }
+        }
+    }
+    TL_TILE_LOOP_END
+    if (LAST) reduce_finish<1>(acc, ra);
+}
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+static inline int grid_for(const Ext &e, int cap)
+{
+    const int tiles = e.ntx * e.nty;
+    return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
+}
+
+#define TL_LAUNCH(kern, e, cap, stream, ...)                                                  \
+    hipLaunchKernelGGL(kern, dim3(grid_for(e, cap)), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__)
+
+void launch_init_chunk_1d(hipStream_t s, const Geom &g, double xmin, double ymin, double dx, double dy,
+                          double *const *f1)
+{
+    const int n = (g.x_max > g.y_max ? g.x_max : g.y_max) + 2 * g.hd + 1;
+    hipLaunchKernelGGL(k_init_chunk_1d, dim3((n + 255) / 256), dim3(256), 0, s, g, xmin, ymin, dx, dy,
+                       f1[F_VERTEXX], f1[F_VERTEXDX], f1[F_VERTEXY], f1[F_VERTEXDY], f1[F_CELLX],
+                       f1[F_CELLDX], f1[F_CELLY], f1[F_CELLDY]);
+}
+
+void launch_init_chunk_2d(hipStream_t s, const Geom &g, const Ext &e, int cap, double dx, double dy,
+                          double *vol, double *xa, double *ya)
+{
+    TL_LAUNCH(k_init_chunk_2d, e, cap, s, g, e, dx, dy, vol, xa, ya);
+}
+
+void launch_generate(hipStream_t s, const Geom &g, const Ext &e, int cap, double *const *f1, double *rho,
+                     double *e0, double *u, double *u0, int nstates, const double *const *sarr,
+                     const int *sgeom, int g_rect, int g_circ, int g_point)
+{
+    TL_LAUNCH(k_generate, e, cap, s, g, e, f1[F_VERTEXX], f1[F_VERTEXY], f1[F_CELLX], f1[F_CELLY], rho,
+              e0, u, u0, nstates, sarr[0], sarr[1], sarr[2], sarr[3], sarr[4], sarr[5], sarr[6], sgeom,
+              g_rect, g_circ, g_point);
+}
+
+void launch_copy(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src, double *dst)
+{
+    TL_LAUNCH(k_copy, e, cap, s, g, e, src, dst);
+}
+
+void launch_field_summary(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *vol,
+                          const double *rho, const double *en, const double *u, const RedArgs &ra)
+{
+    TL_LAUNCH(k_field_summary, e, cap, s, g, e, vol, rho, en, u, ra);
+}
+
+void launch_halo(hipStream_t s, const Geom &g, double *a, int depth, int face)
+{
+    if (face >= 2) {
+        const int n = g.x_max + 2 * depth;
+        hipLaunchKernelGGL(k_halo_bt, dim3((n + 255) / 256), dim3(256), 0, s, g, a, depth, face);
+    } else {
+        const int n = g.y_max + 2 * depth;
+        hipLaunchKernelGGL(k_halo_lr, dim3((n + 255) / 256), dim3(256), 0, s, g, a, depth, face);
+    }
+}
+
+void launch_init_common(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, double *Kx,
+                        double *Ky, double rx, double ry, int coef)
+{
+    TL_LAUNCH(k_init_common, e, cap, s, g, e, rho, Kx, Ky, rx, ry, coef);
+}
+
+void launch_zero_boundary(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Kx, double *Ky,
+                          const int *zb)
+{
+    TL_LAUNCH(k_zero_boundary, e, cap, s, g, e, Kx, Ky, zb[0], zb[1], zb[2], zb[3]);
+}
+
+void launch_init_u(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, const double *en,
+                   double *u, double *u0)
+{
+    TL_LAUNCH(k_init_u, e, cap, s, g, e, rho, en, u, u0);
+}
+
+void launch_init_jac_diag(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Mi, const double *Kx,
+                          const double *Ky)
+{
+    TL_LAUNCH(k_init_jac_diag, e, cap, s, g, e, Mi, Kx, Ky);
+}
+
+void launch_calc_residual(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *u,
+                          const double *u0, double *r, const double *Kx, const double *Ky)
+{
+    TL_LAUNCH(k_calc_residual, e, cap, s, g, e, u, u0, r, Kx, Ky);
+}
+
+void launch_dot(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *a, const double *b,
+                const RedArgs &ra)
+{
+    TL_LAUNCH(k_dot, e, cap, s, g, e, a, b, ra);
+}
+
+void launch_finalise(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho,
+                     const double *u, double *en)
+{
+    TL_LAUNCH(k_finalise, e, cap, s, g, e, rho, u, en);
+}
+
+void launch_block_init(hipStream_t s, const Geom &g, const Ext &e, int cap, double *cp, double *bfp,
+                       const double *Kx, const double *Ky)
+{
+    TL_LAUNCH(k_block_init, e, cap, s, g, e, cp, bfp, Kx, Ky);
+}
+
+void launch_block_solve(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *r, double *z,
+                        const double *cp, const double *bfp, const double *Kx, const double *Ky)
+{
+    TL_LAUNCH(k_block_solve, e, cap, s, g, e, r, z, cp, bfp, Kx, Ky);
+}
+
+void launch_cg_init_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double *p,
+                      const double *r, double *z, const double *Mi, const RedArgs &ra)
+{
+    if (prec == PREC_BLOCK) TL_LAUNCH(k_cg_init_p<PREC_BLOCK>, e, cap, s, g, e, p, r, z, Mi, ra);
+    else if (prec == PREC_DIAG) TL_LAUNCH(k_cg_init_p<PREC_DIAG>, e, cap, s, g, e, p, r, z, Mi, ra);
+    else TL_LAUNCH(k_cg_init_p<PREC_NONE>, e, cap, s, g, e, p, r, z, Mi, ra);
+}
+
+void launch_cg_calc_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *p, double *w,
+                      const double *Kx, const double *Ky, const RedArgs &ra)
+{
+    TL_LAUNCH(k_cg_calc_w, e, cap, s, g, e, p, w, Kx, Ky, ra);
+}
+
+void launch_cg_calc_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double alpha,
+                       double *u, const double *p, double *r, const double *w, double *z,
+                       const double *cp, const double *bfp, const double *Mi, const double *Kx,
+                       const double *Ky, const RedArgs &ra)
+{
+    if (prec == PREC_BLOCK)
+        TL_LAUNCH(k_cg_calc_ur<PREC_BLOCK>, e, cap, s, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+    else if (prec == PREC_DIAG)
+        TL_LAUNCH(k_cg_calc_ur<PREC_DIAG>, e, cap, s, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+    else
+        TL_LAUNCH(k_cg_calc_ur<PREC_NONE>, e, cap, s, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+}
+
+void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double beta,
+                      double *p, const double *r, const double *z)
+{
+    if (prec == PREC_NONE) TL_LAUNCH(k_cg_calc_p<PREC_NONE>, e, cap, s, g, e, beta, p, r, z);
+    else TL_LAUNCH(k_cg_calc_p<PREC_DIAG>, e, cap, s, g, e, beta, p, r, z);
+}
+
+void launch_cheby_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int init,
+                    const double *u, const double *u0, double *p, double *r, double *w,
+                    const double *cp, const double *bfp, const double *Mi, const double *Kx,
+                    const double *Ky, const double *ch_a, const double *ch_b, double theta, int step,
+                    const SolveState *st)
+{
+#define TL_CHP(P, I)                                                                               \
+    TL_LAUNCH((k_cheby_p<P, I>), e, cap, s, g, e, u, u0, p, r, w, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, \
+              theta, step, st)
+    if (init) {
+        if (prec == PREC_BLOCK) TL_CHP(PREC_BLOCK, 1);
+        else if (prec == PREC_DIAG) TL_CHP(PREC_DIAG, 1);
+        else TL_CHP(PREC_NONE, 1);
+    } else {
+        if (prec == PREC_BLOCK) TL_CHP(PREC_BLOCK, 0);
+        else if (prec == PREC_DIAG) TL_CHP(PREC_DIAG, 0);
+        else TL_CHP(PREC_NONE, 0);
+    }
+#undef TL_CHP
+}
+
+void launch_cheby_u(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
+                    const double *r, const RedArgs &ra)
+{
+    TL_LAUNCH(k_cheby_u, e, cap, s, g, e, u, p, r, ra);
+}
+
+void launch_ppcg_init_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
+                         double *sd, const double *cp, const double *bfp, const double *Mi,
+                         const double *Kx, const double *Ky, double theta)
+{
+    if (prec == PREC_BLOCK)
+        TL_LAUNCH(k_ppcg_init_sd<PREC_BLOCK>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, theta);
+    else if (prec == PREC_DIAG)
+        TL_LAUNCH(k_ppcg_init_sd<PREC_DIAG>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, theta);
+    else
+        TL_LAUNCH(k_ppcg_init_sd<PREC_NONE>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, theta);
+}
+
+void launch_ppcg_update_r(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, double *r,
+                          const double *Kx, const double *Ky, const double *sd)
+{
+    TL_LAUNCH(k_ppcg_update_r, e, cap, s, g, e, u, r, Kx, Ky, sd);
+}
+
+void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
+                         double *sd, const double *cp, const double *bfp, const double *Mi,
+                         const double *Kx, const double *Ky, const double *ch_a, const double *ch_b,
+                         int step)
+{
+    if (prec == PREC_BLOCK)
+        TL_LAUNCH(k_ppcg_calc_sd<PREC_BLOCK>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, step);
+    else if (prec == PREC_DIAG)
+        TL_LAUNCH(k_ppcg_calc_sd<PREC_DIAG>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, step);
+    else
+        TL_LAUNCH(k_ppcg_calc_sd<PREC_NONE>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, step);
+}
+
+void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
+                   const double *u0, double *u1, const double *un, const RedArgs &ra)
+{
+    TL_LAUNCH(k_jacobi, e, cap, s, g, e, Kx, Ky, u0, u1, un, ra);
+}
+
+void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
+                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra)
+{
+    TL_LAUNCH(k_fused_cg_w, e, cap, s, g, e, src, pA, pB, Kx, Ky, ra);
+}
+
+void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
+                         const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra)
+{
+    TL_LAUNCH(k_fused_jacobi, e, cap, s, g, e, Kx, Ky, u0, uA, uB, ra);
+}
+
+void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
+                        const double *pA, const double *pB, double *r, double *z, double *sd,
+                        const double *Kx, const double *Ky, const RedArgs &ra)
+{
+#define TL_FUR(P, M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra)
+    if (ppcg) {
+        if (prec == PREC_BLOCK) TL_FUR(PREC_BLOCK, 1);
+        else if (prec == PREC_DIAG) TL_FUR(PREC_DIAG, 1);
+        else TL_FUR(PREC_NONE, 1);
+    } else {
+        if (prec == PREC_BLOCK) TL_FUR(PREC_BLOCK, 0);
+        else if (prec == PREC_DIAG) TL_FUR(PREC_DIAG, 0);
+        else TL_FUR(PREC_NONE, 0);
+    }
+#undef TL_FUR
+}
+
+void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
+                             double *u, double *r, const double *sd_in, double *sd_out,
+                             const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                             const double *ch_b, int step, const RedArgs &ra)
+{
+#define TL_FPI(P, L) \
+    TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, Kx, Ky, z, ch_a, ch_b, step, ra)
+    if (last) {
+        if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 1);
+        else if (prec == PREC_DIAG) TL_FPI(PREC_DIAG, 1);
+        else TL_FPI(PREC_NONE, 1);
+    } else {
+        if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 0);
+        else if (prec == PREC_DIAG) TL_FPI(PREC_DIAG, 0);
+        else TL_FPI(PREC_NONE, 0);
+    }
+#undef TL_FPI
+}
+
+}  // namespace tl
