@@ -21,7 +21,7 @@ SUMMARY_TOL = 1e-11
 
 
 def _compare_summaries(lg, lo, tol=SUMMARY_TOL):
-    assert len(lg["summary"]) == len(lo["summary"])
+    assert lo["summary"] and len(lg["summary"]) == len(lo["summary"])
     for sg, so in zip(lg["summary"], lo["summary"]):
         assert sg[0] == so[0]
         for a, b in zip(sg[1:], so[1:]):
@@ -29,7 +29,7 @@ def _compare_summaries(lg, lo, tol=SUMMARY_TOL):
 
 
 def _compare_solves(lg, lo, iter_slack=1):
-    assert len(lg["solve"]) == len(lo["solve"])
+    assert lo["solve"] and len(lg["solve"]) == len(lo["solve"])
     for a, b in zip(lg["solve"], lo["solve"]):
         assert a[0] == b[0]
         assert abs(a[1] - b[1]) <= iter_slack, (a, b)
@@ -69,6 +69,7 @@ def test_config1_cg_diag_512(hiplib, oracle, tmp_path, mode):
     # residual trajectory of the first solve while far above round-off
     rg = [r[2] for r in lg["resid"] if r[0] == 1]
     ro = [r[2] for r in lo["resid"] if r[0] == 1]
+    assert len(rg) > 10 and len(rg) == len(ro)
     r0 = ro[0]
     for i, (a, b) in enumerate(zip(rg, ro)):
         if b < 1e-9 * r0:
