@@ -52,8 +52,9 @@ struct tl_chunk {
         int key = -1;
     };
     GraphSlot g_cg, g_cheby, g_ppcg, g_jacobi;
-    // events around the first CG iteration of each captured batch
-    hipEvent_t ev_t[3] = {nullptr, nullptr, nullptr};
+    // events around the eagerly launched first CG iteration of each batch
+    // (one set per pipeline slot)
+    hipEvent_t ev_t[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     bool timing = false;
     double t_k1_ms = 0, t_k2_ms = 0;

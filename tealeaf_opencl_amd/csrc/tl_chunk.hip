@@ -188,7 +188,8 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
         if (p) hipFree(p);
     if (c->h_result) hipHostFree(c->h_result);
     if (c->h_st) hipHostFree(c->h_st);
-    for (hipEvent_t ev : {c->pe0, c->pe1, c->ev_t[0], c->ev_t[1], c->ev_t[2], c->ev_batch[0], c->ev_batch[1]})
+    for (hipEvent_t ev : {c->pe0, c->pe1, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[1][0],
+                          c->ev_t[1][1], c->ev_t[1][2], c->ev_batch[0], c->ev_batch[1]})
         if (ev) hipEventDestroy(ev);
     if (c->s) hipStreamDestroy(c->s);
     if (g_default == c) g_default = nullptr;
@@ -257,7 +258,8 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     }
     hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 16, c->s);
     hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
-    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0], &c->ev_t[1], &c->ev_t[2]})
+    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[1][0],
+                           &c->ev_t[1][1], &c->ev_t[1][2]})
         if ((e = hipEventCreate(ev)) != hipSuccess) {
             tl_chunk_destroy(c);
             return hip_fail("hipEventCreate", e);

@@ -347,15 +347,32 @@ int capture(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<v
     return TL_OK;
 }
 
-// Replays `exec` until the device state says halt. Without timing, batch i+1
-// is queued before the state of batch i is examined. With timing, batches run
-// one at a time and the events recorded around the first CG iteration of each
-// batch are read after it.
+// One CG iteration of the device-resident driver: the fused search-direction
+// update + matvec + p.w, then the fused u/r update + preconditioner + r.z.
+void cg_iteration(tl_chunk *c, hipEvent_t *ev)
+{
+    const Ext in = ext_interior(c);
+    const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
+    if (ev) hipEventRecord(ev[0], c->s);
+    launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_KX], c->f2[F_KY],
+                      red_args(c, RED_CG_PW, true));
+    if (ev) hipEventRecord(ev[1], c->s);
+    launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
+                       c->f2[F_Z], c->f2[F_SD], c->f2[F_KX], c->f2[F_KY], red_args(c, RED_CG_RRN, true));
+    if (ev) hipEventRecord(ev[2], c->s);
+}
+
+// Replays `exec` until the device state says halt; batch i+1 is queued before
+// the state of batch i is examined. With `timed`, each batch starts with one
+// eagerly launched CG iteration bracketed by events (graph-recorded events
+// carry no timestamps); the events of a batch are read once it has finished,
+// while the other slot's set is in flight.
 int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
 {
     hipError_t e;
     int slot = 0;
     auto enqueue = [&](int sl) -> hipError_t {
+        if (timed) cg_iteration(c, c->ev_t[sl]);
         hipError_t ee = hipGraphLaunch(exec, c->s);
         if (ee != hipSuccess) return ee;
         ee = hipMemcpyAsync(&c->h_st[sl], c->st, sizeof(SolveState), hipMemcpyDeviceToHost, c->s);
@@ -364,15 +381,15 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
     };
     if ((e = enqueue(0)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
     for (long guard = 0;; ++guard) {
-        if (!timed) {
-            if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
-        }
+        if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
         if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
         const SolveState &h = c->h_st[slot];
         if (timed) {
+            // every examined batch follows a batch that did not halt, so its
+            // eager first iteration ran
             float a = 0.f, b = 0.f;
-            if (hipEventElapsedTime(&a, c->ev_t[0], c->ev_t[1]) == hipSuccess &&
-                hipEventElapsedTime(&b, c->ev_t[1], c->ev_t[2]) == hipSuccess && !h.halt) {
+            if (hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
+                hipEventElapsedTime(&b, c->ev_t[slot][1], c->ev_t[slot][2]) == hipSuccess) {
                 c->t_k1_ms += a;
                 c->t_k2_ms += b;
                 c->t_samples += 1;
@@ -384,31 +401,19 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
             set_error("solver state did not halt after %ld batches", guard);
             return TL_ERR_SOLVER;
         }
-        if (timed) {
-            if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
-        }
         slot = 1 - slot;
     }
     if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("batch drain", e);
+    (void)hipGetLastError();
     return TL_OK;
 }
 
 int cg_graph(tl_chunk *c, bool timed)
 {
-    const int B = 16;
+    const int B = timed ? 15 : 16;
     const int key = c->prec * 10 + (timed ? 1 : 0);
-    const Ext in = ext_interior(c);
-    const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
     return capture(c, c->g_cg, key, [&]() {
-        for (int b = 0; b < B; ++b) {
-            if (timed && b == 0) hipEventRecord(c->ev_t[0], c->s);
-            launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_KX], c->f2[F_KY],
-                              red_args(c, RED_CG_PW, true));
-            if (timed && b == 0) hipEventRecord(c->ev_t[1], c->s);
-            launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                               c->f2[F_Z], c->f2[F_SD], c->f2[F_KX], c->f2[F_KY], red_args(c, RED_CG_RRN, true));
-            if (timed && b == 0) hipEventRecord(c->ev_t[2], c->s);
-        }
+        for (int b = 0; b < B; ++b) cg_iteration(c, nullptr);
     });
 }
 
