@@ -27,7 +27,6 @@ struct tl_chunk {
 
     int cap = 2048;                  // max blocks per launch
     double *partials = nullptr;
-    unsigned *counter = nullptr;
     double *result = nullptr;
     tl::SolveState *st = nullptr;
     double *hist_a = nullptr, *hist_b = nullptr, *hist_e = nullptr;

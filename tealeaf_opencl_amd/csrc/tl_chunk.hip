@@ -69,7 +69,6 @@ RedArgs red_args(tl_chunk *c, int mode, bool with_state)
 {
     RedArgs ra;
     ra.partials = c->partials;
-    ra.counter = c->counter;
     ra.result = c->result;
     ra.st = with_state ? c->st : nullptr;
     ra.hist_a = with_state ? c->hist_a : nullptr;
@@ -182,7 +181,7 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
         if (c->base2[i]) hipFree(c->base2[i]);
     for (int i = 0; i < NUM_FIELDS_1D; ++i)
         if (c->f1[i]) hipFree(c->f1[i]);
-    for (void *p : {(void *)c->partials, (void *)c->counter, (void *)c->result, (void *)c->st,
+    for (void *p : {(void *)c->partials, (void *)c->result, (void *)c->st,
                     (void *)c->hist_a, (void *)c->hist_b, (void *)c->hist_e, (void *)c->ch_a,
                     (void *)c->ch_b})
         if (p) hipFree(p);
@@ -248,7 +247,6 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
         hipMemsetAsync(c->f1[i], 0, sizeof(double) * c->len1d, c->s);
     }
     if ((e = hipMalloc(&c->partials, sizeof(double) * MAX_RED * c->cap)) != hipSuccess ||
-        (e = hipMalloc(&c->counter, sizeof(unsigned) * 16)) != hipSuccess ||
         (e = hipMalloc(&c->result, sizeof(double) * MAX_RED)) != hipSuccess ||
         (e = hipMalloc(&c->st, sizeof(SolveState))) != hipSuccess ||
         (e = hipHostMalloc(&c->h_result, sizeof(double) * MAX_RED, hipHostMallocDefault)) != hipSuccess ||
@@ -256,7 +254,6 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
         tl_chunk_destroy(c);
         return hip_fail("hipMalloc scratch", e);
     }
-    hipMemsetAsync(c->counter, 0, sizeof(unsigned) * 16, c->s);
     hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
     for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[1][0],
                            &c->ev_t[1][1], &c->ev_t[1][2]})

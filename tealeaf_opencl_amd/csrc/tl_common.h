@@ -96,7 +96,6 @@ enum RedMode {
 
 struct RedArgs {
     double *partials;     // gridDim.x * MAX_RED
-    unsigned *counter;    // ticket for the last-block pattern
     double *result;       // MAX_RED totals
     SolveState *st;       // may be null (drop-in calls)
     double *hist_a;       // cg_alphas (1-based by iteration)

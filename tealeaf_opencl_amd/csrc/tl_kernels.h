@@ -66,8 +66,10 @@ void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
                          double *sd, const double *cp, const double *bfp, const double *Mi,
                          const double *Kx, const double *Ky, const double *ch_a, const double *ch_b,
                          int step);
+// without a solver state: u1 <- sweep(un); with one: the pair (u1, un) is
+// double-buffered by the iteration counter
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
-                   const double *u0, double *u1, const double *un, const RedArgs &ra);
+                   const double *u0, double *u1, double *un, const RedArgs &ra);
 // p is double-buffered (pA, pB); the iteration counter in the state picks
 // which buffer is read and which is written
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,

@@ -222,13 +222,13 @@ __device__ void finish_scalars(const RedArgs &ra, const double *tot)
     }
 }
 
-// Block reduction of NR sums plus the last-block finish. Must be reached by
-// every thread of every block that has not returned early as a whole grid.
+// Block reduction of NR sums into this block's partials (plain stores; the
+// finisher kernel that follows reads them after the kernel boundary). Must be
+// reached by every thread of the block.
 template <int NR>
-__device__ __forceinline__ void reduce_finish(const double (&v)[NR], const RedArgs &ra)
+__device__ __forceinline__ void block_partial(const double (&v)[NR], const RedArgs &ra)
 {
     __shared__ double lds[STRIPS_PER_TILE][MAX_RED];
-    __shared__ int am_last;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
@@ -236,33 +236,33 @@ __device__ __forceinline__ void reduce_finish(const double (&v)[NR], const RedAr
         if (lane == 0) lds[w][q] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < MAX_RED; ++q) {
-            double s = 0.0;
-            if (q < NR) s = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
-            __hip_atomic_store(&ra.partials[blockIdx.x * MAX_RED + q], s, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __threadfence();
-        const unsigned ticket = atomicAdd(ra.counter, 1u);
-        am_last = (ticket == gridDim.x - 1);
+    if (threadIdx.x < NR) {
+        const int q = threadIdx.x;
+        ra.partials[blockIdx.x * MAX_RED + q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
     }
-    __syncthreads();
-    if (!am_last) return;
-    __threadfence();
-    double acc[NR];
+}
+
+__device__ __forceinline__ bool iteration_mode(int mode)
+{
+    return mode == RED_CG_PW || mode == RED_CG_RRN || mode == RED_PPCG_PW || mode == RED_PPCG_RRN ||
+           mode == RED_JACOBI || mode == RED_CHEBY;
+}
+
+// Sums the partials of the preceding reduction kernel in block order and runs
+// the scalar algebra of its mode. One block of 256 threads. An iteration-mode
+// finisher of a halted solver does nothing, like the kernel before it.
+__global__ void __launch_bounds__(BLOCK_THREADS) k_finish(RedArgs ra, int nblocks, int nq)
+{
+    if (ra.st && iteration_mode(ra.mode) && ra.st->halt) return;
+    __shared__ double lds[STRIPS_PER_TILE][MAX_RED];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double acc[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
 #pragma unroll
-    for (int q = 0; q < NR; ++q) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        for (int q = 0; q < MAX_RED; ++q)
+            if (q < nq) acc[q] += ra.partials[b * MAX_RED + q];
 #pragma unroll
-        for (int q = 0; q < NR; ++q)
-            acc[q] += __hip_atomic_load(&ra.partials[b * MAX_RED + q], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
+    for (int q = 0; q < MAX_RED; ++q) {
         const double s = wave_sum(acc[q]);
         if (lane == 0) lds[w][q] = s;
     }
@@ -270,13 +270,108 @@ __device__ __forceinline__ void reduce_finish(const double (&v)[NR], const RedAr
     if (threadIdx.x == 0) {
         double tot[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int q = 0; q < NR; ++q) tot[q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
-        *ra.counter = 0u;
+        for (int q = 0; q < MAX_RED; ++q)
+            if (q < nq) tot[q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
         finish_scalars(ra, tot);
     }
 }
 
 __device__ __forceinline__ bool halted(const SolveState *st) { return st && st->halt; }
+
+// ---------------------------------------------------------------------------
+// strip helpers. Every load of a strip is issued before any of it is used;
+// rows past the extent are read too (the allocation margins hold them) and
+// only stores and sums are predicated on the row being inside the extent.
+// ---------------------------------------------------------------------------
+
+// rows ks .. ks+3 of column j (v[q] = row ks+q)
+__device__ __forceinline__ void col4(const double *f, int i0, int P, double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) v[q] = f[i0 + q * P];
+}
+
+// rows ks-1 .. ks+4 of column j (v[q] = row ks+q-1)
+__device__ __forceinline__ void col6(const double *f, int i0, int P, double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) v[q] = f[i0 + (q - 1) * P];
+}
+
+// face coefficients of a strip: Kx at the cell and its east neighbour, Ky at
+// rows ks .. ks+4
+__device__ __forceinline__ void strip_k(const double *Kx, const double *Ky, int i0, int P, double *kx0,
+                                        double *kx1, double *ky)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        kx0[q] = Kx[i0 + q * P];
+        kx1[q] = Kx[i0 + q * P + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = Ky[i0 + q * P];
+}
+
+// w = A v on the strip: c6 = centre column rows ks-1..ks+4, w4/e4 = west/east
+__device__ __forceinline__ void strip_apply(const double *c6, const double *w4, const double *e4,
+                                            const double *kx0, const double *kx1, const double *ky, double *out)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        out[q] = opA(kx0[q], kx1[q], ky[q], ky[q + 1], c6[q + 1], w4[q], e4[q], c6[q], c6[q + 2]);
+}
+
+// A v for a field read straight from memory
+__device__ __forceinline__ void strip_matvec(const double *v, const double *kx0, const double *kx1,
+                                             const double *ky, int i0, int P, double *out, double *centre)
+{
+    double c6[STRIP + 2], w4[STRIP], e4[STRIP];
+    col6(v, i0, P, c6);
+    col4(v, i0 - 1, P, w4);
+    col4(v, i0 + 1, P, e4);
+    strip_apply(c6, w4, e4, kx0, kx1, ky, out);
+    if (centre)
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) centre[q] = c6[q + 1];
+}
+
+// tridiagonal coefficients of the strip from its face coefficients
+__device__ __forceinline__ void strip_abc(const double *kx0, const double *kx1, const double *ky, double *a,
+                                          double *b, double *c)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        a[q] = -ky[q];
+        c[q] = -ky[q + 1];
+        b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
+    }
+}
+
+// z = M_blk^-1 r on the strip, with factors either read (cp/bfp non-null)
+// or recomputed from the coefficients
+__device__ __forceinline__ void strip_block(const Geom &g, int ks, int i0, const double *kx0,
+                                            const double *kx1, const double *ky, const double *cp,
+                                            const double *bfp, const double *rv, double *zv)
+{
+    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
+    strip_abc(kx0, kx1, ky, a, b, c);
+    const int upper = block_top(g, ks);
+    if (cp) {
+        col4(cp, i0, g.pitch, cpv);
+        col4(bfp, i0, g.pitch, bfv);
+    } else {
+        strip_factors(upper, b, a, c, cpv, bfv);
+    }
+    strip_solve(upper, rv, b, a, cpv, bfv, zv);
+}
+
+// the strip's right-hand side with rows past k1 zeroed (the reference's
+// work-group copy only holds in-bounds rows)
+__device__ __forceinline__ void mask_rows(int ks, int k1, double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) v[q] = (ks + q <= k1) ? v[q] : 0.0;
+}
 
 // ===========================================================================
 // reference-granularity kernels
@@ -310,12 +405,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_chunk_2d(Geom g, Ext e, 
                                                                  double *vol, double *xa, double *ya)
 {
     TL_TILE_LOOP(e)
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        vol[IX(g, j, k)] = dx * dy;
-        xa[IX(g, j, k)] = dy;  // xarea lives in its own pitched buffer
-        ya[IX(g, j, k)] = dx;
+        if (ks + q > e.k1) continue;
+        const int i = IX(g, j, ks + q);
+        vol[i] = dx * dy;
+        xa[i] = dy;  // xarea lives in its own pitched buffer
+        ya[i] = dx;
     }
     TL_TILE_LOOP_END
 }
@@ -364,11 +460,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_generate(
 __global__ void __launch_bounds__(BLOCK_THREADS) k_copy(Geom g, Ext e, const double *src, double *dst)
 {
     TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        dst[IX(g, j, k)] = src[IX(g, j, k)];
-    }
+    const int i0 = IX(g, j, ks);
+    double v[STRIP];
+    col4(src, i0, g.pitch, v);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) dst[i0 + q * g.pitch] = v[q];
     TL_TILE_LOOP_END
 }
 
@@ -379,19 +476,23 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_field_summary(Geom g, Ext e, 
 {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double v[STRIP], d[STRIP], x[STRIP], y[STRIP];
+    col4(vol, i0, P, v);
+    col4(rho, i0, P, d);
+    col4(en, i0, P, x);
+    col4(u, i0, P, y);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        const double cv = vol[i];
-        const double cm = cv * rho[i];
-        acc[0] += cv;
+        if (ks + q > e.k1) continue;
+        const double cm = v[q] * d[q];
+        acc[0] += v[q];
         acc[1] += cm;
-        acc[2] += cm * en[i];
-        acc[3] += cm * u[i];
+        acc[2] += cm * x[q];
+        acc[3] += cm * y[q];
     }
     TL_TILE_LOOP_END
-    reduce_finish<4>(acc, ra);
+    block_partial<4>(acc, ra);
 }
 
 // update_halo_{bottom,top} (update_halo_cl.cl:43-69): one thread per column
@@ -423,26 +524,30 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_common(Geom g, Ext e, co
                                                                double ry, int coef)
 {
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double c5[STRIP + 1], w4[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) c5[q] = rho[i0 + (q - 1) * P];  // rows ks-1 .. ks+3
+    col4(rho, i0 - 1, P, w4);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
+        if (ks + q > e.k1) continue;
         double dc, dl, dd;
         if (coef == COEF_CONDUCTIVITY) {
-            dc = rho[i];
-            dl = rho[i - 1];
-            dd = rho[i - g.pitch];
+            dc = c5[q + 1];
+            dl = w4[q];
+            dd = c5[q];
         } else {
-            dc = 1.0 / rho[i];
-            dl = 1.0 / rho[i - 1];
-            dd = 1.0 / rho[i - g.pitch];
+            dc = 1.0 / c5[q + 1];
+            dl = 1.0 / w4[q];
+            dd = 1.0 / c5[q];
         }
         double kx = (dl + dc) / (2.0 * dl * dc);
         kx *= rx;
         double ky = (dd + dc) / (2.0 * dd * dc);
         ky *= ry;
-        Kx[i] = kx;
-        Ky[i] = ky;
+        Kx[i0 + q * P] = kx;
+        Ky[i0 + q * P] = ky;
     }
     TL_TILE_LOOP_END
 }
@@ -482,12 +587,15 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_u(Geom g, Ext e, const d
                                                           const double *en, double *u, double *u0)
 {
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double d[STRIP], x[STRIP];
+    col4(rho, i0, P, d);
+    col4(en, i0, P, x);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        u[i] = en[i] * rho[i];
-        u0[i] = en[i] * rho[i];
+        if (ks + q > e.k1) continue;
+        u[i0 + q * P] = x[q] * d[q];
+        u0[i0 + q * P] = x[q] * d[q];
     }
     TL_TILE_LOOP_END
 }
@@ -497,13 +605,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_jac_diag(Geom g, Ext e, 
                                                                  const double *Kx, const double *Ky)
 {
     TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        const double d = (1.0 + (Ky[i + g.pitch] + Ky[i]) + (Kx[i + 1] + Kx[i]));
-        Mi[i] = 1.0 / d;
-    }
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) Mi[i0 + q * P] = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
     TL_TILE_LOOP_END
 }
 
@@ -513,14 +620,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_calc_residual(Geom g, Ext e, 
                                                                  const double *Kx, const double *Ky)
 {
     TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        const double s = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], u[i], u[i - 1], u[i + 1],
-                             u[i - g.pitch], u[i + g.pitch]);
-        r[i] = u0[i] - s;
-    }
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w[STRIP], b[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(u0, i0, P, b);
+    strip_matvec(u, kx0, kx1, ky, i0, P, w, nullptr);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) r[i0 + q * P] = b[q] - w[q];
     TL_TILE_LOOP_END
 }
 
@@ -530,14 +637,15 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_dot(Geom g, Ext e, const doub
 {
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        acc[0] += a[i] * b[i];
-    }
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double x[STRIP], y[STRIP];
+    col4(a, i0, P, x);
+    col4(b, i0, P, y);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) acc[0] += x[q] * y[q];
     TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
+    block_partial<1>(acc, ra);
 }
 
 // tea_leaf_finalise (tea_leaf_common_cl.cl:4-15)
@@ -545,27 +653,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_finalise(Geom g, Ext e, const
                                                             const double *u, double *en)
 {
     TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        en[i] = u[i] / rho[i];
-    }
-    TL_TILE_LOOP_END
-}
-
-// load the strip's tridiagonal coefficients (COEF_A/B/C, tea_block_jacobi.cl:1-3)
-__device__ __forceinline__ void strip_coefs(const Geom &g, const double *Kx, const double *Ky, int j,
-                                            int ks, double *a, double *b, double *c)
-{
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double d[STRIP], x[STRIP];
+    col4(rho, i0, P, d);
+    col4(u, i0, P, x);
 #pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        const int i = IX(g, j, ks + q);
-        const double ky0 = Ky[i], ky1 = Ky[i + g.pitch];
-        a[q] = -ky0;
-        c[q] = -ky1;
-        b[q] = (1.0 + (ky1 + ky0) + (Kx[i + 1] + Kx[i]));
-    }
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) en[i0 + q * P] = x[q] / d[q];
+    TL_TILE_LOOP_END
 }
 
 // tea_leaf_block_init (tea_block_jacobi.cl:78-105)
@@ -573,37 +668,20 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_block_init(Geom g, Ext e, dou
                                                               const double *Kx, const double *Ky)
 {
     TL_TILE_LOOP(e)
-    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
-    strip_coefs(g, Kx, Ky, j, ks, a, b, c);
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_abc(kx0, kx1, ky, a, b, c);
     const int upper = block_top(g, ks);
     strip_factors(upper, b, a, c, cpv, bfv);
-    cp[IX(g, j, ks)] = cpv[0];
+    cp[i0] = cpv[0];
 #pragma unroll
     for (int q = 1; q < STRIP; ++q)
         if (q < upper) {
-            bfp[IX(g, j, ks + q)] = bfv[q];
-            cp[IX(g, j, ks + q)] = cpv[q];
+            bfp[i0 + q * P] = bfv[q];
+            cp[i0 + q * P] = cpv[q];
         }
     TL_TILE_LOOP_END
-}
-
-__device__ __forceinline__ void load_factors(const Geom &g, const double *cp, const double *bfp, int j,
-                                             int ks, double *cpv, double *bfv)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        cpv[q] = cp[IX(g, j, ks + q)];
-        bfv[q] = bfp[IX(g, j, ks + q)];
-    }
-}
-
-// gather a strip of `src` with rows above k1 read as zero (the LDS copy of
-// the reference only holds in-bounds rows)
-__device__ __forceinline__ void strip_load(const Geom &g, const double *src, int j, int ks, int k1,
-                                           double *v)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) v[q] = (ks + q <= k1) ? src[IX(g, j, ks + q)] : 0.0;
 }
 
 // tea_leaf_block_solve (tea_block_jacobi.cl:44-76)
@@ -613,14 +691,15 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_block_solve(Geom g, Ext e, co
                                                                const double *Ky)
 {
     TL_TILE_LOOP(e)
-    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], rv[STRIP], zv[STRIP];
-    strip_coefs(g, Kx, Ky, j, ks, a, b, c);
-    load_factors(g, cp, bfp, j, ks, cpv, bfv);
-    strip_load(g, r, j, ks, e.k1, rv);
-    strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], zv[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(r, i0, P, rv);
+    mask_rows(ks, e.k1, rv);
+    strip_block(g, ks, i0, kx0, kx1, ky, cp, bfp, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q)
-        if (ks + q <= e.k1) z[IX(g, j, ks + q)] = zv[q];
+        if (ks + q <= e.k1) z[i0 + q * P] = zv[q];
     TL_TILE_LOOP_END
 }
 
@@ -631,25 +710,29 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_init_p(Geom g, Ext e, doub
 {
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double rv[STRIP], x[STRIP];
+    col4(r, i0, P, rv);
+    if (PREC == PREC_BLOCK) col4(z, i0, P, x);
+    else if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
         double pv;
         if (PREC == PREC_BLOCK) {
-            pv = z[i];
+            pv = x[q];
         } else if (PREC == PREC_DIAG) {
-            const double zv = r[i] * Mi[i];
-            z[i] = zv;
-            pv = zv;
+            pv = rv[q] * x[q];
+            z[i] = pv;
         } else {
-            pv = r[i];
+            pv = rv[q];
         }
         p[i] = pv;
-        acc[0] += r[i] * pv;
+        acc[0] += rv[q] * pv;
     }
     TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
+    block_partial<1>(acc, ra);
 }
 
 // tea_leaf_cg_solve_calc_w (tea_leaf_cg_cl.cl:45-69)
@@ -659,17 +742,18 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_w(Geom g, Ext e, cons
 {
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        const double wv = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], p[i], p[i - 1], p[i + 1],
-                              p[i - g.pitch], p[i + g.pitch]);
-        w[i] = wv;
-        acc[0] += p[i] * wv;
+        if (ks + q > e.k1) continue;
+        w[i0 + q * P] = wv[q];
+        acc[0] += pc[q] * wv[q];
     }
     TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
+    block_partial<1>(acc, ra);
 }
 
 // tea_leaf_cg_solve_calc_ur (tea_leaf_cg_cl.cl:73-137)
@@ -683,45 +767,41 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, dou
 {
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
-    double rv[STRIP];
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double uv[STRIP], pv[STRIP], rv[STRIP], wv[STRIP], x[STRIP], zv[STRIP];
+    col4(u, i0, P, uv);
+    col4(p, i0, P, pv);
+    col4(r, i0, P, rv);
+    col4(w, i0, P, wv);
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    if (PREC == PREC_BLOCK) strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        rv[q] = 0.0;
-        const int k = ks + q;
-        if (k > e.k1) continue;
-        const int i = IX(g, j, k);
-        u[i] += alpha * p[i];
-        const double rn = r[i] - alpha * w[i];
-        r[i] = rn;
-        rv[q] = rn;
+        uv[q] += alpha * pv[q];
+        rv[q] -= alpha * wv[q];
     }
-    if (PREC == PREC_BLOCK) {
-        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
-        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
-        load_factors(g, cp, bfp, j, ks, cpv, bfv);
-        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+    mask_rows(ks, e.k1, rv);
+    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, cp, bfp, rv, zv);
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q)
-            if (ks + q <= e.k1) {
-                z[IX(g, j, ks + q)] = zv[q];
-                acc[0] += rv[q] * zv[q];
-            }
-    } else {
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1) continue;
-            const int i = IX(g, j, ks + q);
-            if (PREC == PREC_DIAG) {
-                const double zv = rv[q] * Mi[i];
-                z[i] = zv;
-                acc[0] += rv[q] * zv;
-            } else {
-                acc[0] += rv[q] * rv[q];
-            }
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        u[i] = uv[q];
+        r[i] = rv[q];
+        if (PREC == PREC_BLOCK) {
+            z[i] = zv[q];
+            acc[0] += rv[q] * zv[q];
+        } else if (PREC == PREC_DIAG) {
+            const double zz = rv[q] * x[q];
+            z[i] = zz;
+            acc[0] += rv[q] * zz;
+        } else {
+            acc[0] += rv[q] * rv[q];
         }
     }
     TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
+    block_partial<1>(acc, ra);
 }
 
 // tea_leaf_cg_solve_calc_p (tea_leaf_cg_cl.cl:141-160)
@@ -730,19 +810,20 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_p(Geom g, Ext e, doub
                                                              const double *r, const double *z)
 {
     TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        if (PREC != PREC_NONE) p[i] = z[i] + beta * p[i];
-        else p[i] = r[i] + beta * p[i];
-    }
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double x[STRIP], pv[STRIP];
+    col4(PREC != PREC_NONE ? z : r, i0, P, x);
+    col4(p, i0, P, pv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) p[i0 + q * P] = x[q] + beta * pv[q];
     TL_TILE_LOOP_END
 }
 
 // tea_leaf_cheby_solve_init_p (INIT=1, tea_leaf_cheby_cl.cl:4-64) and
 // tea_leaf_cheby_solve_calc_p (INIT=0, :78-145). The coefficient index comes
-// from `step` or, when st is set, from the device-resident step counter.
+// from `step` or, when st is set, from the device-resident step counter; the
+// factors come from cp/bfp, or are recomputed when cp is null.
 template <int PREC, int INIT>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_p(Geom g, Ext e, const double *u,
                                                            const double *u0, double *p, double *r,
@@ -757,52 +838,36 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_p(Geom g, Ext e, const 
     if (st) step = st->cheby_calc_steps - 1;
     const double ca = INIT ? 0.0 : ch_a[step], cb = INIT ? 0.0 : ch_b[step];
     TL_TILE_LOOP(e)
-    double rv[STRIP];
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], b[STRIP], pv[STRIP], x[STRIP], rv[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(u0, i0, P, b);
+    if (!INIT) col4(p, i0, P, pv);
+    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
+    strip_matvec(u, kx0, kx1, ky, i0, P, wv, nullptr);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = b[q] - wv[q];
+    mask_rows(ks, e.k1, rv);
+    double zv[STRIP];
+    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, cp, bfp, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        rv[q] = 0.0;
-        const int k = ks + q;
-        if (k > e.k1) continue;
-        const int i = IX(g, j, k);
-        const double wv = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], u[i], u[i - 1], u[i + 1],
-                              u[i - g.pitch], u[i + g.pitch]);
-        if (w) w[i] = wv;
-        const double rn = u0[i] - wv;
-        r[i] = rn;
-        rv[q] = rn;
-    }
-    if (PREC == PREC_BLOCK) {
-        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
-        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
-        load_factors(g, cp, bfp, j, ks, cpv, bfv);
-        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q)
-            if (ks + q <= e.k1) {
-                const int i = IX(g, j, ks + q);
-                if (INIT) p[i] = zv[q] / theta;
-                else p[i] = ca * p[i] + cb * zv[q];
-            }
-    } else {
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1) continue;
-            const int i = IX(g, j, ks + q);
-            if (PREC == PREC_DIAG) {
-                if (INIT) p[i] = (Mi[i] * rv[q]) / theta;
-                else p[i] = ca * p[i] + cb * Mi[i] * rv[q];
-            } else {
-                if (INIT) p[i] = rv[q] / theta;
-                else p[i] = ca * p[i] + cb * rv[q];
-            }
-        }
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        if (w) w[i] = wv[q];
+        r[i] = rv[q];
+        double np;
+        if (PREC == PREC_BLOCK) np = INIT ? zv[q] / theta : ca * pv[q] + cb * zv[q];
+        else if (PREC == PREC_DIAG) np = INIT ? (x[q] * rv[q]) / theta : ca * pv[q] + cb * x[q] * rv[q];
+        else np = INIT ? rv[q] / theta : ca * pv[q] + cb * rv[q];
+        p[i] = np;
     }
     TL_TILE_LOOP_END
 }
 
 // tea_leaf_cheby_solve_calc_u (tea_leaf_cheby_cl.cl:66-76). With st set this
 // is the last kernel of a device-resident Chebyshev iteration: it also sums
-// r.r when tea_solve.f90:223 asks for it and runs the iteration tail.
+// r.r when tea_solve.f90:223 asks for it; its finisher runs the iteration tail.
 __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_u(Geom g, Ext e, double *u, const double *p,
                                                            const double *r, RedArgs ra)
 {
@@ -810,15 +875,19 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_u(Geom g, Ext e, double
     const bool want = ra.st && cheby_norm_due(ra.st);
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double uv[STRIP], pv[STRIP], rv[STRIP];
+    col4(u, i0, P, uv);
+    col4(p, i0, P, pv);
+    if (want) col4(r, i0, P, rv);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        u[i] += p[i];
-        if (want) acc[0] += r[i] * r[i];
+        if (ks + q > e.k1) continue;
+        u[i0 + q * P] = uv[q] + pv[q];
+        if (want) acc[0] += rv[q] * rv[q];
     }
     TL_TILE_LOOP_END
-    if (ra.st) reduce_finish<1>(acc, ra);
+    if (ra.st) block_partial<1>(acc, ra);
 }
 
 // tea_leaf_ppcg_solve_init_sd (tea_leaf_ppcg_cl.cl:4-60)
@@ -830,23 +899,22 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_init_sd(Geom g, Ext e, c
                                                                 double theta)
 {
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double rv[STRIP], x[STRIP], zv[STRIP], kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    col4(r, i0, P, rv);
+    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
     if (PREC == PREC_BLOCK) {
-        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], rv[STRIP], zv[STRIP];
-        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
-        load_factors(g, cp, bfp, j, ks, cpv, bfv);
-        strip_load(g, r, j, ks, e.k1, rv);
-        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+        strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+        mask_rows(ks, e.k1, rv);
+        strip_block(g, ks, i0, kx0, kx1, ky, cp, bfp, rv, zv);
+    }
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q)
-            if (ks + q <= e.k1) sd[IX(g, j, ks + q)] = zv[q] / theta;
-    } else {
-        for (int q = 0; q < STRIP; ++q) {
-            const int k = ks + q;
-            if (k > e.k1) break;
-            const int i = IX(g, j, k);
-            if (PREC == PREC_DIAG) sd[i] = r[i] * Mi[i] / theta;
-            else sd[i] = r[i] / theta;
-        }
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        if (PREC == PREC_BLOCK) sd[i] = zv[q] / theta;
+        else if (PREC == PREC_DIAG) sd[i] = rv[q] * x[q] / theta;
+        else sd[i] = rv[q] / theta;
     }
     TL_TILE_LOOP_END
 }
@@ -857,14 +925,17 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_update_r(Geom g, Ext e, 
                                                                  const double *sd)
 {
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(u, i0, P, uv);
+    col4(r, i0, P, rv);
+    strip_matvec(sd, kx0, kx1, ky, i0, P, res, sc);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        u[i] += sd[i];
-        const double res = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + g.pitch], sd[i], sd[i - 1], sd[i + 1],
-                               sd[i - g.pitch], sd[i + g.pitch]);
-        r[i] -= res;
+        if (ks + q > e.k1) continue;
+        u[i0 + q * P] = uv[q] + sc[q];
+        r[i0 + q * P] = rv[q] - res[q];
     }
     TL_TILE_LOOP_END
 }
@@ -880,92 +951,73 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_calc_sd(Geom g, Ext e, c
 {
     const double ca = ch_a[step], cb = ch_b[step];
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double rv[STRIP], sv[STRIP], x[STRIP], zv[STRIP], kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    col4(r, i0, P, rv);
+    col4(sd, i0, P, sv);
+    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
     if (PREC == PREC_BLOCK) {
-        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], rv[STRIP], zv[STRIP];
-        strip_coefs(g, Kx, Ky, j, ks, a, b, c);
-        load_factors(g, cp, bfp, j, ks, cpv, bfv);
-        strip_load(g, r, j, ks, e.k1, rv);
-        strip_solve(block_top(g, ks), rv, b, a, cpv, bfv, zv);
+        strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+        mask_rows(ks, e.k1, rv);
+        strip_block(g, ks, i0, kx0, kx1, ky, cp, bfp, rv, zv);
+    }
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q)
-            if (ks + q <= e.k1) {
-                const int i = IX(g, j, ks + q);
-                sd[i] = ca * sd[i] + cb * zv[q];
-            }
-    } else {
-        for (int q = 0; q < STRIP; ++q) {
-            const int k = ks + q;
-            if (k > e.k1) break;
-            const int i = IX(g, j, k);
-            if (PREC == PREC_DIAG) sd[i] = ca * sd[i] + cb * r[i] * Mi[i];
-            else sd[i] = ca * sd[i] + cb * r[i];
-        }
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        if (PREC == PREC_BLOCK) sd[i] = ca * sv[q] + cb * zv[q];
+        else if (PREC == PREC_DIAG) sd[i] = ca * sv[q] + cb * rv[q] * x[q];
+        else sd[i] = ca * sv[q] + cb * rv[q];
     }
     TL_TILE_LOOP_END
 }
 
 // tea_leaf_jacobi_solve (tea_leaf_jacobi_cl.cl:15-42); `un` holds the
-// previous iterate (the reference copies u into the Mi buffer first)
+// previous iterate. The device-resident driver passes the two buffers of a
+// pair and st: iteration n reads buffer (n-1)&1 (this replaces the
+// reference's copy of u into the Mi buffer, tea_leaf_jacobi_cl.cl:3-13).
 __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const double *Kx,
                                                           const double *Ky, const double *u0,
-                                                          double *u1, const double *un, RedArgs ra)
+                                                          double *uA, double *uB, RedArgs ra)
 {
     if (halted(ra.st)) return;
+    const double *un = uB;
+    double *u1 = uA;
+    if (ra.st && (ra.st->n & 1)) {
+        un = uA;
+        u1 = uB;
+    }
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], c6[STRIP + 2], w4[STRIP], e4[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(u0, i0, P, b);
+    col6(un, i0, P, c6);
+    col4(un, i0 - 1, P, w4);
+    col4(un, i0 + 1, P, e4);
+#pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        const int P = g.pitch;
-        const double v = (u0[i] + Kx[i + 1] * un[i + 1] + Kx[i] * un[i - 1] + Ky[i + P] * un[i + P] +
-                          Ky[i] * un[i - P]) /
-                         (1.0 + (Kx[i] + Kx[i + 1]) + (Ky[i] + Ky[i + P]));
-        u1[i] = v;
-        acc[0] += fabs(v - un[i]);
+        if (ks + q > e.k1) continue;
+        const double v = (b[q] + kx1[q] * e4[q] + kx0[q] * w4[q] + ky[q + 1] * c6[q + 2] + ky[q] * c6[q]) /
+                         (1.0 + (kx0[q] + kx1[q]) + (ky[q] + ky[q + 1]));
+        u1[i0 + q * P] = v;
+        acc[0] += fabs(v - c6[q + 1]);
     }
     TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
+    block_partial<1>(acc, ra);
 }
 
 // ===========================================================================
 // fused iteration kernels (device-resident driver)
 // ===========================================================================
-// Jacobi sweep of the device-resident driver: the previous iterate is the
-// other buffer of a pair (iteration n reads buffer (n-1)&1), which replaces
-// the reference's copy of u into the Mi buffer (tea_leaf_jacobi_cl.cl:3-13).
-__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_jacobi(Geom g, Ext e, const double *Kx,
-                                                                const double *Ky, const double *u0,
-                                                                double *uA, double *uB, RedArgs ra)
-{
-    const SolveState *st = ra.st;
-    if (st->halt) return;
-    const bool odd = (st->n & 1) != 0;
-    const double *un = odd ? uA : uB;
-    double *u1 = odd ? uB : uA;
-    double acc[1] = {0.0};
-    TL_TILE_LOOP(e)
-    for (int q = 0; q < STRIP; ++q) {
-        const int k = ks + q;
-        if (k > e.k1) break;
-        const int i = IX(g, j, k);
-        const int P = g.pitch;
-        const double v = (u0[i] + Kx[i + 1] * un[i + 1] + Kx[i] * un[i - 1] + Ky[i + P] * un[i + P] +
-                          Ky[i] * un[i - P]) /
-                         (1.0 + (Kx[i] + Kx[i + 1]) + (Ky[i] + Ky[i + P]));
-        u1[i] = v;
-        acc[0] += fabs(v - un[i]);
-    }
-    TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
-}
-
 
 // CG / PPCG outer step, first half: p_new = z + beta*p_old (r + beta*p_old
 // without preconditioner; tea_leaf_cg_cl.cl:141-160 of the previous
 // iteration), w = A p_new, p_new.w (tea_leaf_cg_cl.cl:45-69). p_new at the
-// four neighbours is formed from the old buffers, so no cell is read after
-// it is overwritten. Stores p_new only; w is recomputed by the second half.
+// four neighbours is formed from the old buffers, so no cell is read after it
+// is overwritten; iteration n reads buffer (n-1)&1 and writes buffer n&1.
+// Stores p_new only; w is recomputed by the second half.
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, const double *src,
                                                               double *pA, double *pB,
                                                               const double *Kx, const double *Ky,
@@ -974,39 +1026,45 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
     const SolveState *st = ra.st;
     if (st->halt) return;
     const double beta = st->beta;
-    // iteration n reads the previous direction from buffer (n-1)&1
     const bool odd = (st->n & 1) != 0;
     const double *p_old = odd ? pA : pB;
     double *p_new = odd ? pB : pA;
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
-    const int P = g.pitch;
-    const int i0 = IX(g, j, ks);
-    // column j, rows ks-1 .. ks+4
-    double pc[STRIP + 2];
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col6(src, i0, P, sc);
+    col6(p_old, i0, P, pc);
+    col4(src, i0 - 1, P, sw);
+    col4(p_old, i0 - 1, P, pw_);
+    col4(src, i0 + 1, P, se);
+    col4(p_old, i0 + 1, P, pe);
+    double c6[STRIP + 2], w4[STRIP], e4[STRIP], wv[STRIP];
 #pragma unroll
-    for (int q = 0; q < STRIP + 2; ++q) {
-        const int i = i0 + (q - 1) * P;
-        pc[q] = src[i] + beta * p_old[i];
+    for (int q = 0; q < STRIP + 2; ++q) c6[q] = sc[q] + beta * pc[q];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        w4[q] = sw[q] + beta * pw_[q];
+        e4[q] = se[q] + beta * pe[q];
     }
+    strip_apply(c6, w4, e4, kx0, kx1, ky, wv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
-        const int i = i0 + q * P;
-        const double pw_ = src[i - 1] + beta * p_old[i - 1];
-        const double pe_ = src[i + 1] + beta * p_old[i + 1];
-        const double wv = opA(Kx[i], Kx[i + 1], Ky[i], Ky[i + P], pc[q + 1], pw_, pe_, pc[q], pc[q + 2]);
-        p_new[i] = pc[q + 1];
-        acc[0] += pc[q + 1] * wv;
+        p_new[i0 + q * P] = c6[q + 1];
+        acc[0] += c6[q + 1] * wv[q];
     }
     TL_TILE_LOOP_END
-    reduce_finish<1>(acc, ra);
+    block_partial<1>(acc, ra);
 }
 
 // CG / PPCG outer step, second half (tea_leaf_cg_cl.cl:73-137): w = A p
-// recomputed, u += alpha p, r -= alpha w, z = M^-1 r, r.z. PPCG=1 also
-// writes sd = z/theta (tea_leaf_ppcg_cl.cl:4-60 on the same r) and skips the
-// unused r.z sum.
+// recomputed, u += alpha p, r -= alpha w, z = M^-1 r with the diagonal or
+// block factors recomputed in registers, r.z. PPCG=1 also writes
+// sd = z/theta (tea_leaf_ppcg_cl.cl:4-60 on the same r) and skips the unused
+// r.z sum.
 template <int PREC, int PPCG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, double *u,
                                                                const double *pA, const double *pB,
@@ -1021,72 +1079,39 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     const double theta = PPCG ? st->theta : 1.0;
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
-    const int P = g.pitch;
-    const int i0 = IX(g, j, ks);
-    double pc[STRIP + 2], kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP];
-#pragma unroll
-    for (int q = 0; q < STRIP + 2; ++q) pc[q] = p[i0 + (q - 1) * P];
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(u, i0, P, uv);
+    col4(r, i0, P, rv);
+    strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        kx0[q] = Kx[i0 + q * P];
-        kx1[q] = Kx[i0 + q * P + 1];
+        uv[q] += alpha * pc[q];
+        rv[q] -= alpha * wv[q];
     }
-#pragma unroll
-    for (int q = 0; q < STRIP + 1; ++q) ky[q] = Ky[i0 + q * P];
+    mask_rows(ks, e.k1, rv);
+    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        rv[q] = 0.0;
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        const double wv = opA(kx0[q], kx1[q], ky[q], ky[q + 1], pc[q + 1], p[i - 1], p[i + 1], pc[q], pc[q + 2]);
-        u[i] += alpha * pc[q + 1];
-        const double rn = r[i] - alpha * wv;
-        r[i] = rn;
-        rv[q] = rn;
-    }
-    if (PREC == PREC_BLOCK) {
-        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            a[q] = -ky[q];
-            c[q] = -ky[q + 1];
-            b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
-        }
-        const int upper = block_top(g, ks);
-        strip_factors(upper, b, a, c, cpv, bfv);
-        strip_solve(upper, rv, b, a, cpv, bfv, zv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q)
-            if (ks + q <= e.k1) {
-                const int i = i0 + q * P;
-                z[i] = zv[q];
-                if (PPCG) sd[i] = zv[q] / theta;
-                else acc[0] += rv[q] * zv[q];
-            }
-    } else {
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1) continue;
-            const int i = i0 + q * P;
-            if (PREC == PREC_DIAG) {
-                const double mi = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
-                const double zv = rv[q] * mi;
-                z[i] = zv;
-                if (PPCG) sd[i] = zv / theta;
-                else acc[0] += rv[q] * zv;
-            } else {
-                if (PPCG) sd[i] = rv[q] / theta;
-                else acc[0] += rv[q] * rv[q];
-            }
-        }
+        u[i] = uv[q];
+        r[i] = rv[q];
+        double zz = rv[q];
+        if (PREC == PREC_BLOCK) zz = zv[q];
+        if (PREC == PREC_DIAG) zz = rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+        if (PREC != PREC_NONE) z[i] = zz;
+        if (PPCG) sd[i] = zz / theta;
+        else acc[0] += rv[q] * zz;
     }
     TL_TILE_LOOP_END
-    if (!PPCG) reduce_finish<1>(acc, ra);
+    if (!PPCG) block_partial<1>(acc, ra);
 }
 
 // PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
 // sd is double-buffered. LAST=1 also forms r.z with the z of the outer step
-// (tea_leaf_ppcg.f90:41-60 via calc_2norm(2)) and runs the iteration tail.
+// (tea_leaf_ppcg.f90:41-60 via calc_2norm(2)); its finisher runs the tail.
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *Kx,
@@ -1097,64 +1122,32 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     const double ca = ch_a[step], cb = ch_b[step];
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
-    const int P = g.pitch;
-    const int i0 = IX(g, j, ks);
-    double sc[STRIP + 2], kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP];
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP],
+        zo[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(u, i0, P, uv);
+    col4(r, i0, P, rv);
+    if (LAST && PREC != PREC_NONE) col4(z, i0, P, zo);
+    strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
 #pragma unroll
-    for (int q = 0; q < STRIP + 2; ++q) sc[q] = sd_in[i0 + (q - 1) * P];
+    for (int q = 0; q < STRIP; ++q) rv[q] -= res[q];
+    mask_rows(ks, e.k1, rv);
+    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        kx0[q] = Kx[i0 + q * P];
-        kx1[q] = Kx[i0 + q * P + 1];
-    }
-#pragma unroll
-    for (int q = 0; q < STRIP + 1; ++q) ky[q] = Ky[i0 + q * P];
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        rv[q] = 0.0;
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        u[i] += sc[q + 1];
-        const double res = opA(kx0[q], kx1[q], ky[q], ky[q + 1], sc[q + 1], sd_in[i - 1], sd_in[i + 1], sc[q], sc[q + 2]);
-        const double rn = r[i] - res;
-        r[i] = rn;
-        rv[q] = rn;
-    }
-    if (PREC == PREC_BLOCK) {
-        double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP], zv[STRIP];
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            a[q] = -ky[q];
-            c[q] = -ky[q + 1];
-            b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
-        }
-        const int upper = block_top(g, ks);
-        strip_factors(upper, b, a, c, cpv, bfv);
-        strip_solve(upper, rv, b, a, cpv, bfv, zv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q)
-            if (ks + q <= e.k1) {
-                const int i = i0 + q * P;
-                sd_out[i] = ca * sc[q + 1] + cb * zv[q];
-                if (LAST) acc[0] += rv[q] * z[i];
-            }
-    } else {
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1) continue;
-            const int i = i0 + q * P;
-            if (PREC == PREC_DIAG) {
-                const double mi = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
-                sd_out[i] = ca * sc[q + 1] + cb * rv[q] * mi;
-                if (LAST) acc[0] += rv[q] * z[i];
-            } else {
-                sd_out[i] = ca * sc[q + 1] + cb * rv[q];
-                if (LAST) acc[0] += rv[q] * rv[q];
-            }
-        }
+        u[i] = uv[q] + sc[q];
+        r[i] = rv[q];
+        if (PREC == PREC_BLOCK) sd_out[i] = ca * sc[q] + cb * zv[q];
+        else if (PREC == PREC_DIAG)
+            sd_out[i] = ca * sc[q] + cb * rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+        else sd_out[i] = ca * sc[q] + cb * rv[q];
+        if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
     }
     TL_TILE_LOOP_END
-    if (LAST) reduce_finish<1>(acc, ra);
+    if (LAST) block_partial<1>(acc, ra);
 }
 
 // ===========================================================================
@@ -1168,6 +1161,14 @@ static inline int grid_for(const Ext &e, int cap)
 
 #define TL_LAUNCH(kern, e, cap, stream, ...)                                                  \
     hipLaunchKernelGGL(kern, dim3(grid_for(e, cap)), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__)
+
+// reduction kernel + the finisher that sums its per-block partials
+#define TL_LAUNCH_RED(kern, e, cap, stream, nq, ra, ...)                                           \
+    do {                                                                                           \
+        const int grid_ = grid_for(e, cap);                                                        \
+        hipLaunchKernelGGL(kern, dim3(grid_), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__);         \
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK_THREADS), 0, stream, ra, grid_, nq);      \
+    } while (0)
 
 void launch_init_chunk_1d(hipStream_t s, const Geom &g, double xmin, double ymin, double dx, double dy,
                           double *const *f1)
@@ -1201,7 +1202,7 @@ void launch_copy(hipStream_t s, const Geom &g, const Ext &e, int cap, const doub
 void launch_field_summary(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *vol,
                           const double *rho, const double *en, const double *u, const RedArgs &ra)
 {
-    TL_LAUNCH(k_field_summary, e, cap, s, g, e, vol, rho, en, u, ra);
+    TL_LAUNCH_RED(k_field_summary, e, cap, s, 4, ra, g, e, vol, rho, en, u, ra);
 }
 
 void launch_halo(hipStream_t s, const Geom &g, double *a, int depth, int face)
@@ -1248,7 +1249,7 @@ void launch_calc_residual(hipStream_t s, const Geom &g, const Ext &e, int cap, c
 void launch_dot(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *a, const double *b,
                 const RedArgs &ra)
 {
-    TL_LAUNCH(k_dot, e, cap, s, g, e, a, b, ra);
+    TL_LAUNCH_RED(k_dot, e, cap, s, 1, ra, g, e, a, b, ra);
 }
 
 void launch_finalise(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho,
@@ -1272,15 +1273,15 @@ void launch_block_solve(hipStream_t s, const Geom &g, const Ext &e, int cap, con
 void launch_cg_init_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double *p,
                       const double *r, double *z, const double *Mi, const RedArgs &ra)
 {
-    if (prec == PREC_BLOCK) TL_LAUNCH(k_cg_init_p<PREC_BLOCK>, e, cap, s, g, e, p, r, z, Mi, ra);
-    else if (prec == PREC_DIAG) TL_LAUNCH(k_cg_init_p<PREC_DIAG>, e, cap, s, g, e, p, r, z, Mi, ra);
-    else TL_LAUNCH(k_cg_init_p<PREC_NONE>, e, cap, s, g, e, p, r, z, Mi, ra);
+    if (prec == PREC_BLOCK) TL_LAUNCH_RED(k_cg_init_p<PREC_BLOCK>, e, cap, s, 1, ra, g, e, p, r, z, Mi, ra);
+    else if (prec == PREC_DIAG) TL_LAUNCH_RED(k_cg_init_p<PREC_DIAG>, e, cap, s, 1, ra, g, e, p, r, z, Mi, ra);
+    else TL_LAUNCH_RED(k_cg_init_p<PREC_NONE>, e, cap, s, 1, ra, g, e, p, r, z, Mi, ra);
 }
 
 void launch_cg_calc_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *p, double *w,
                       const double *Kx, const double *Ky, const RedArgs &ra)
 {
-    TL_LAUNCH(k_cg_calc_w, e, cap, s, g, e, p, w, Kx, Ky, ra);
+    TL_LAUNCH_RED(k_cg_calc_w, e, cap, s, 1, ra, g, e, p, w, Kx, Ky, ra);
 }
 
 void launch_cg_calc_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double alpha,
@@ -1289,11 +1290,11 @@ void launch_cg_calc_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int 
                        const double *Ky, const RedArgs &ra)
 {
     if (prec == PREC_BLOCK)
-        TL_LAUNCH(k_cg_calc_ur<PREC_BLOCK>, e, cap, s, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+        TL_LAUNCH_RED(k_cg_calc_ur<PREC_BLOCK>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
     else if (prec == PREC_DIAG)
-        TL_LAUNCH(k_cg_calc_ur<PREC_DIAG>, e, cap, s, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+        TL_LAUNCH_RED(k_cg_calc_ur<PREC_DIAG>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
     else
-        TL_LAUNCH(k_cg_calc_ur<PREC_NONE>, e, cap, s, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+        TL_LAUNCH_RED(k_cg_calc_ur<PREC_NONE>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
 }
 
 void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double beta,
@@ -1327,7 +1328,8 @@ void launch_cheby_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int pre
 void launch_cheby_u(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
                     const double *r, const RedArgs &ra)
 {
-    TL_LAUNCH(k_cheby_u, e, cap, s, g, e, u, p, r, ra);
+    if (ra.st) TL_LAUNCH_RED(k_cheby_u, e, cap, s, 1, ra, g, e, u, p, r, ra);
+    else TL_LAUNCH(k_cheby_u, e, cap, s, g, e, u, p, r, ra);
 }
 
 void launch_ppcg_init_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
@@ -1362,28 +1364,32 @@ void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
 }
 
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
-                   const double *u0, double *u1, const double *un, const RedArgs &ra)
+                   const double *u0, double *u1, double *un, const RedArgs &ra)
 {
-    TL_LAUNCH(k_jacobi, e, cap, s, g, e, Kx, Ky, u0, u1, un, ra);
+    TL_LAUNCH_RED(k_jacobi, e, cap, s, 1, ra, g, e, Kx, Ky, u0, u1, un, ra);
 }
 
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
                        double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra)
 {
-    TL_LAUNCH(k_fused_cg_w, e, cap, s, g, e, src, pA, pB, Kx, Ky, ra);
+    TL_LAUNCH_RED(k_fused_cg_w, e, cap, s, 1, ra, g, e, src, pA, pB, Kx, Ky, ra);
 }
 
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
                          const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra)
 {
-    TL_LAUNCH(k_fused_jacobi, e, cap, s, g, e, Kx, Ky, u0, uA, uB, ra);
+    TL_LAUNCH_RED(k_jacobi, e, cap, s, 1, ra, g, e, Kx, Ky, u0, uA, uB, ra);
 }
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, double *r, double *z, double *sd,
                         const double *Kx, const double *Ky, const RedArgs &ra)
 {
-#define TL_FUR(P, M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra)
+#define TL_FUR(P, M)                                                                               \
+    do {                                                                                           \
+        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra);  \
+        else TL_LAUNCH_RED((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra); \
+    } while (0)
     if (ppcg) {
         if (prec == PREC_BLOCK) TL_FUR(PREC_BLOCK, 1);
         else if (prec == PREC_DIAG) TL_FUR(PREC_DIAG, 1);
@@ -1401,8 +1407,13 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
                              const double *Kx, const double *Ky, const double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra)
 {
-#define TL_FPI(P, L) \
-    TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, Kx, Ky, z, ch_a, ch_b, step, ra)
+#define TL_FPI(P, L)                                                                                         \
+    do {                                                                                                     \
+        if (L) TL_LAUNCH_RED((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, g, e, u, r, sd_in, sd_out, Kx, Ky, z, \
+                             ch_a, ch_b, step, ra);                                                          \
+        else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, Kx, Ky, z, ch_a, ch_b, \
+                       step, ra);                                                                            \
+    } while (0)
     if (last) {
         if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 1);
         else if (prec == PREC_DIAG) TL_FPI(PREC_DIAG, 1);
