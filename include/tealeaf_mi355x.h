@@ -193,6 +193,36 @@ typedef struct {
 } tl_bench_result;
 int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, tl_bench_result *out);
 
+/* Refined solve of A u = u0 for the current timestep (call after
+ * tl_common_init; u holds the initial guess, the reference's u = u0), with the
+ * chunk's preconditioner. No reference counterpart: the reference solves in
+ * FP64 only (tea_solve.f90:37-420).
+ *   precision TL_PRECISION_FP64: inner CG with FP64 vectors;
+ *             TL_PRECISION_MIXED: inner CG with FP32 vectors and FP32
+ *             stencil coefficients, FP64 arithmetic, dot products and scalars.
+ * Either way the increment is folded into the FP64 solution and the residual
+ * is replaced by the true FP64 residual u0 - A u whenever the recursive
+ * residual norm has fallen below delta times its largest value since the last
+ * replacement (0 < delta < 1; 0 replaces it only at the tolerance), and the
+ * solve ends when ||u0 - A u||_2 <= tol * ||u0 - A u_init||_2 holds for the
+ * true residual, or after max_iters inner iterations (converged = 0, TL_OK).
+ * The result is left in field TL_F_U. */
+enum {
+    TL_PRECISION_FP64 = 0,
+    TL_PRECISION_MIXED = 1
+};
+typedef struct {
+    double seconds;           /* device time of the whole solve, setup included */
+    double initial_residual;  /* ||u0 - A u_init||_2 */
+    double rel_residual;      /* final true ||u0 - A u||_2 / initial_residual */
+    int iterations;           /* inner iterations */
+    int updates;              /* true residuals formed, the initial one included */
+    int converged;
+    int pad_;
+} tl_refine_result;
+int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_iters, double delta,
+                     tl_refine_result *out);
+
 #ifdef __cplusplus
 }
 #endif

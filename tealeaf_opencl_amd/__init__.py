@@ -21,12 +21,14 @@ __all__ = [
     "TealeafLibraryError", "PREC_NONE", "PREC_JAC_DIAG", "PREC_JAC_BLOCK",
     "SOLVER_JACOBI", "SOLVER_CG", "SOLVER_CHEBY", "SOLVER_PPCG",
     "COEF_CONDUCTIVITY", "COEF_RECIP_CONDUCTIVITY", "MODE_DROPIN", "MODE_NATIVE", "FIELDS", "FIELDS1D",
+    "PRECISION_FP64", "PRECISION_MIXED",
 ]
 
 PREC_NONE, PREC_JAC_DIAG, PREC_JAC_BLOCK = 1, 2, 3
 SOLVER_JACOBI, SOLVER_CG, SOLVER_CHEBY, SOLVER_PPCG = 1, 2, 3, 4
 COEF_CONDUCTIVITY, COEF_RECIP_CONDUCTIVITY = 1, 2
 MODE_DROPIN, MODE_NATIVE = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
 G_RECT, G_CIRC, G_POINT = 1, 2, 3
 
 FIELDS = ["density", "energy0", "energy1", "volume", "xarea", "yarea", "u", "u0", "p", "r", "w",
@@ -47,6 +49,12 @@ class BenchResult(ctypes.Structure):
                 ("inner_iterations", ctypes.c_long), ("k1_ms", ctypes.c_double),
                 ("k2_ms", ctypes.c_double), ("k_samples", ctypes.c_long),
                 ("final_summary", ctypes.c_double * 5)]
+
+
+class RefineResult(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("initial_residual", ctypes.c_double),
+                ("rel_residual", ctypes.c_double), ("iterations", ctypes.c_int), ("updates", ctypes.c_int),
+                ("converged", ctypes.c_int), ("pad_", ctypes.c_int)]
 
 
 _LIB = None
@@ -92,6 +100,7 @@ _SIGS = {
     "tl_print_profiling": (_I, [_P]),
     "tl_run_deck": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
     "tl_bench_deck": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(BenchResult)]),
+    "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
 }
 
 
@@ -297,10 +306,23 @@ class Chunk:
     def synchronize(self):
         _check(self.lib.tl_synchronize(self.h), "synchronize")
 
+    def solve_refined(self, precision=PRECISION_MIXED, tol=1e-10, max_iters=10000, delta=None):
+        """Solve A u = u0 after common_init to a true relative residual `tol`
+        (FP32 or FP64 inner CG with FP64 residual replacement; see
+        tl_solve_refined in include/tealeaf_mi355x.h)."""
+        if delta is None:
+            delta = 0.1 if precision == PRECISION_MIXED else 0.0
+        r = RefineResult()
+        _check(self.lib.tl_solve_refined(self.h, int(precision), float(tol), int(max_iters), float(delta),
+                                         ctypes.byref(r)), "solve_refined")
+        return {"seconds": r.seconds, "initial_residual": r.initial_residual, "rel_residual": r.rel_residual,
+                "iterations": r.iterations, "updates": r.updates, "converged": bool(r.converged)}
+
 
 def parse_log(path):
     """Parse the driver log written by tl_run_deck (and by the oracle)."""
-    out = {"summary": [], "solve": [], "switch": [], "estitc": [], "resid": [], "qa": None, "initres": []}
+    out = {"summary": [], "solve": [], "switch": [], "estitc": [], "resid": [], "qa": None, "initres": [],
+           "refine": []}
     with open(path) as f:
         for line in f:
             t = line.split()
@@ -318,6 +340,8 @@ def parse_log(path):
                 out["resid"].append([int(t[1]), int(t[2]), float(t[3])])
             elif t[0] == "INITRES":
                 out["initres"].append([int(t[1]), float(t[2])])
+            elif t[0] == "REFINE":
+                out["refine"].append([int(t[1]), int(t[2]), int(t[3]), int(t[4]), float(t[5]), float(t[6])])
             elif t[0] == "QA":
                 out["qa"] = (int(t[1]), float(t[2]), t[3])
     return out

@@ -53,11 +53,24 @@ struct tl_chunk {
     GraphSlot g_cg, g_cheby, g_ppcg, g_jacobi;
     // events around the eagerly launched first CG iteration of each batch
     // (one set per pipeline slot)
-    hipEvent_t ev_t[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+    hipEvent_t ev_t[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     bool timing = false;
     double t_k1_ms = 0, t_k2_ms = 0;
     long t_samples = 0;
+
+    // refined solve (tl_refine.hip): [0] FP64 inner fields, [1] FP32 inner
+    // fields, allocated on first use
+    struct RefineWork {
+        int bytes = 0;                    // 8 or 4: storage of the inner fields
+        tl::Geom g{};                     // layout of the inner fields
+        long off = 0, elems = 0;
+        void *base[7] = {};               // owned allocations
+        void *f[7] = {};                  // Kx, Ky, r, z, p, p2, d
+        tl::RefineState *st = nullptr;
+        tl::RefineState *h_st = nullptr;  // pinned, 2 slots
+        GraphSlot graph;
+    } rf[2];
 };
 
 namespace tl {
@@ -75,5 +88,7 @@ void prof_end(tl_chunk *c, const char *name);
 int ensure_coef_capacity(tl_chunk *c, int n);
 int ensure_hist_capacity(tl_chunk *c, int n);
 int read_result(tl_chunk *c, double *out, int nq);
+// frees the refined-solve workspaces (tl_refine.hip)
+void refine_release(tl_chunk *c);
 
 }  // namespace tl

@@ -177,6 +177,7 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
     if (c->s) hipStreamSynchronize(c->s);
     for (auto *gs : {&c->g_cg, &c->g_cheby, &c->g_ppcg, &c->g_jacobi})
         if (gs->exec) hipGraphExecDestroy(gs->exec);
+    refine_release(c);
     for (int i = 0; i < NUM_FIELDS_2D; ++i)
         if (c->base2[i]) hipFree(c->base2[i]);
     for (int i = 0; i < NUM_FIELDS_1D; ++i)
@@ -187,8 +188,9 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
         if (p) hipFree(p);
     if (c->h_result) hipHostFree(c->h_result);
     if (c->h_st) hipHostFree(c->h_st);
-    for (hipEvent_t ev : {c->pe0, c->pe1, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[1][0],
-                          c->ev_t[1][1], c->ev_t[1][2], c->ev_batch[0], c->ev_batch[1]})
+    for (hipEvent_t ev : {c->pe0, c->pe1, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[0][3],
+                          c->ev_t[1][0], c->ev_t[1][1], c->ev_t[1][2], c->ev_t[1][3], c->ev_batch[0],
+                          c->ev_batch[1]})
         if (ev) hipEventDestroy(ev);
     if (c->s) hipStreamDestroy(c->s);
     if (g_default == c) g_default = nullptr;
@@ -255,8 +257,8 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
         return hip_fail("hipMalloc scratch", e);
     }
     hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
-    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[1][0],
-                           &c->ev_t[1][1], &c->ev_t[1][2]})
+    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[0][3],
+                           &c->ev_t[1][0], &c->ev_t[1][1], &c->ev_t[1][2], &c->ev_t[1][3]})
         if ((e = hipEventCreate(ev)) != hipSuccess) {
             tl_chunk_destroy(c);
             return hip_fail("hipEventCreate", e);

@@ -104,4 +104,36 @@ struct RedArgs {
     int mode;
 };
 
+// Device-resident scalars of the refined solve (tl_refine.hip): an inner
+// preconditioned CG in the storage precision of the inner fields (FP32 or
+// FP64) with FP64 scalars, whose recursively updated residual is replaced by
+// the true FP64 residual u0 - A u at every reliable update.
+struct RefineState {
+    // inputs
+    double tol;           // target ||u0 - A u||_2 / ||u0 - A u_init||_2
+    double delta;         // update when ||r|| < delta * max ||r|| since the last one
+    int max_iters;
+    int pad0_;
+    // running scalars
+    double rr0;           // ||r||^2 of the first (initial) true residual
+    double rr_true;       // ||r||^2 of the latest true residual
+    double rr;            // ||r||^2 of the recursive residual
+    double rmax;          // max recursive ||r|| since the last update
+    double rro;           // r.z of the current residual
+    double rz_prev;       // r.z before the latest inner step
+    double pw, alpha, beta;
+    int n;                // inner iterations done
+    int halt;             // 1: inner iteration kernels must not run
+    int want_update;      // 1: a reliable update is due
+    int converged;
+    int updates;          // true residuals formed (the initial one included)
+    int pad1_;
+};
+
+enum RefineMode {
+    RF_UPDATE = 0,        // true residual r.z, r.r after u += d
+    RF_PW,                // p.w -> alpha
+    RF_RRN                // recursive r.z, r.r -> beta, update / exit tests
+};
+
 }  // namespace tl

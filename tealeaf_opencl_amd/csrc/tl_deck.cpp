@@ -252,6 +252,10 @@ int parse_deck(const std::string &path, Config *cfg, std::string *err)
             else if (k == "tl_eps") ok = getr(&cfg->eps);
             else if (k == "tl_coefficient_density") cfg->coefficient = 1;
             else if (k == "tl_coefficient_inverse_density") cfg->coefficient = 2;
+            // refined-solve keywords of this library (not in read_input.f90)
+            else if (k == "tl_mixed_precision") cfg->refine = 2;
+            else if (k == "tl_fp64_refined") cfg->refine = 1;
+            else if (k == "tl_refine_delta") ok = getr(&cfg->refine_delta);
             else if (k == "test_problem") ok = geti(&cfg->test_problem);
             else if (k == "state") {
                 int sn = 0;

@@ -36,6 +36,11 @@ struct Config {
     int verbose_on = 0, profiler_on = 0;
     int halo_depth = 1;           // Fortran keyword
     int test_problem = 0;
+    // refined solve instead of the reference solver (library keywords
+    // tl_fp64_refined = 1, tl_mixed_precision = 2; tl_refine_delta, < 0 for
+    // the default of the precision)
+    int refine = 0;
+    double refine_delta = -1.0;
     int nstates = 0;
     std::vector<double> density, energy, sxmin, sxmax, symin, symax, radius;
     std::vector<int> geometry;

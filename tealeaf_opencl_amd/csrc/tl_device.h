@@ -1,0 +1,288 @@
+// Device-side building blocks shared by the FP64 solver kernels
+// (tl_kernels.hip) and the mixed-precision refinement kernels (tl_refine.hip).
+//
+// Work decomposition (every kernel): a 256-thread block is four wavefronts;
+// a tile is 64 columns x 16 rows; wavefront w of the block owns the 4-row
+// strip w of the tile and lane l owns column l. A thread therefore handles one
+// column of one 4-row strip, which is exactly the y-strip of the strip-4
+// block-Jacobi preconditioner (kernel_files/tea_block_jacobi.cl), so the
+// tridiagonal solve of a strip stays in one thread's registers. Each row of a
+// strip is one fully coalesced load per wavefront (512 bytes in FP64, 256 in
+// FP32). Blocks stride over tiles; the grid is capped so reductions produce
+// few partials.
+//
+// Field loads are templated on the storage type (double or float); all
+// arithmetic on loaded values is FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "tl_common.h"
+
+namespace tl {
+
+// ---------------------------------------------------------------------------
+// tiling
+// ---------------------------------------------------------------------------
+#define TL_TILE_LOOP(e)                                                          \
+    const int tx_ = threadIdx.x & (TILE_X - 1);                                  \
+    const int ty_ = threadIdx.x / TILE_X;                                        \
+    const int nt_ = (e).ntx * (e).nty;                                           \
+    for (int t_ = blockIdx.x; t_ < nt_; t_ += gridDim.x) {                       \
+        const int j = (e).j0 + (t_ % (e).ntx) * TILE_X + tx_;                    \
+        const int ks = (e).k0 + ((t_ / (e).ntx) * STRIPS_PER_TILE + ty_) * STRIP; \
+        if (j > (e).j1 || ks > (e).k1) continue;
+#define TL_TILE_LOOP_END }
+
+__device__ __forceinline__ int IX(const Geom &g, int j, int k) { return k * g.pitch + j; }
+
+// 5-point operator of tea_leaf_common_cl.cl:28-32 (same association)
+__device__ __forceinline__ double opA(double kx0, double kx1, double ky0, double ky1, double vc,
+                                      double vw, double ve, double vs, double vn)
+{
+    return (1.0 + (ky1 + ky0) + (kx1 + kx0)) * vc - (ky1 * vn + ky0 * vs) - (kx1 * ve + kx0 * vw);
+}
+
+__device__ __forceinline__ double diag_of(double kx0, double kx1, double ky0, double ky1)
+{
+    return (1.0 + (ky1 + ky0) + (kx1 + kx0));
+}
+
+// BLOCK_TOP of macros_cl.cl:6-10 (BLOCK_TOP_CHECK only when y_max % 4 != 0)
+__device__ __forceinline__ int block_top(const Geom &g, int row)
+{
+    if (g.y_max % STRIP) {
+        const int t = g.y_max + 2 - row;
+        return t < STRIP ? t : STRIP;
+    }
+    return STRIP;
+}
+
+// block_solve_func (tea_block_jacobi.cl:6-42) on one strip held in registers.
+// b[k] = COEF_B, a[k] = COEF_A of strip row k; cp/bfp as produced by
+// tea_leaf_block_init; rows >= upper are left at zero.
+__device__ __forceinline__ void strip_solve(int upper, const double *r, const double *b,
+                                            const double *a, const double *cp,
+                                            const double *bfp, double *z)
+{
+    double dp[STRIP];
+#pragma unroll
+    for (int k = 0; k < STRIP; ++k) z[k] = 0.0;
+    if (upper <= 0) return;
+    dp[0] = r[0] / b[0];
+#pragma unroll
+    for (int k = 1; k < STRIP; ++k)
+        if (k < upper) dp[k] = (r[k] - a[k] * dp[k - 1]) * bfp[k];
+    // back substitution from row upper-1
+#pragma unroll
+    for (int k = STRIP - 1; k >= 0; --k) {
+        if (k > upper - 1) continue;
+        if (k == upper - 1) z[k] = dp[k];
+        else z[k] = dp[k] - cp[k] * z[k + 1];
+    }
+}
+
+// Thomas factors of tea_leaf_block_init (tea_block_jacobi.cl:78-105)
+__device__ __forceinline__ void strip_factors(int upper, const double *b, const double *a,
+                                              const double *c, double *cp, double *bfp)
+{
+    cp[0] = c[0] / b[0];
+    bfp[0] = 0.0;
+#pragma unroll
+    for (int k = 1; k < STRIP; ++k) {
+        if (k < upper) {
+            bfp[k] = 1.0 / (b[k] - a[k] * cp[k - 1]);
+            cp[k] = c[k] * bfp[k];
+        } else {
+            bfp[k] = 0.0;
+            cp[k] = 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reductions: wavefront shuffle sum, then the four wavefronts of the block in
+// fixed order, one partial per block (plain stores); a one-block finisher
+// kernel launched after the reduction kernel sums the partials in block order.
+// Deterministic for a given grid.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block reduction of NR sums into partials[blockIdx.x * MAX_RED + q]. Must be
+// reached by every thread of the block.
+template <int NR>
+__device__ __forceinline__ void block_partial(const double (&v)[NR], double *partials)
+{
+    __shared__ double lds[STRIPS_PER_TILE][MAX_RED];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) lds[w][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NR) {
+        const int q = threadIdx.x;
+        partials[blockIdx.x * MAX_RED + q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+    }
+}
+
+constexpr int FINISH_THREADS = 1024;
+
+// Sum of the first nq partials of nblocks blocks, in a fixed order, by one
+// block of FINISH_THREADS threads; every thread gets the totals. The loads of
+// two blocks' partials are issued before either is added so that a grid of
+// up to 2 * FINISH_THREADS blocks costs one memory round trip.
+__device__ __forceinline__ void sum_partials(const double *partials, int nblocks, int nq, double *tot)
+{
+    __shared__ double lds[FINISH_THREADS / 64][MAX_RED];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double acc[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += 2 * FINISH_THREADS) {
+        const int b1 = b0 + FINISH_THREADS;
+        double v0[MAX_RED], v1[MAX_RED];
+#pragma unroll
+        for (int q = 0; q < MAX_RED; ++q) {
+            v0[q] = (q < nq) ? partials[b0 * MAX_RED + q] : 0.0;
+            v1[q] = (q < nq && b1 < nblocks) ? partials[b1 * MAX_RED + q] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < MAX_RED; ++q) acc[q] += v0[q] + v1[q];
+    }
+#pragma unroll
+    for (int q = 0; q < MAX_RED; ++q) {
+        const double s = wave_sum(acc[q]);
+        if (lane == 0) lds[w][q] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAX_RED; ++q) {
+        double t = 0.0;
+        for (int i = 0; i < FINISH_THREADS / 64; ++i) t += lds[i][q];
+        tot[q] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strip helpers. Every load of a strip is issued before any of it is used;
+// rows past the extent are read too (the allocation margins hold them) and
+// only stores and sums are predicated on the row being inside the extent.
+// ---------------------------------------------------------------------------
+
+// rows ks .. ks+3 of column j (v[q] = row ks+q)
+template <typename T>
+__device__ __forceinline__ void col4(const T *f, int i0, int P, double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) v[q] = (double)f[i0 + q * P];
+}
+
+// rows ks-1 .. ks+4 of column j (v[q] = row ks+q-1)
+template <typename T>
+__device__ __forceinline__ void col6(const T *f, int i0, int P, double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) v[q] = (double)f[i0 + (q - 1) * P];
+}
+
+// face coefficients of a strip: Kx at the cell and its east neighbour, Ky at
+// rows ks .. ks+4
+template <typename T>
+__device__ __forceinline__ void strip_k(const T *Kx, const T *Ky, int i0, int P, double *kx0, double *kx1,
+                                        double *ky)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        kx0[q] = (double)Kx[i0 + q * P];
+        kx1[q] = (double)Kx[i0 + q * P + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = (double)Ky[i0 + q * P];
+}
+
+// w = A v on the strip: c6 = centre column rows ks-1..ks+4, w4/e4 = west/east
+__device__ __forceinline__ void strip_apply(const double *c6, const double *w4, const double *e4,
+                                            const double *kx0, const double *kx1, const double *ky, double *out)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        out[q] = opA(kx0[q], kx1[q], ky[q], ky[q + 1], c6[q + 1], w4[q], e4[q], c6[q], c6[q + 2]);
+}
+
+// A v for a field read straight from memory
+template <typename T>
+__device__ __forceinline__ void strip_matvec(const T *v, const double *kx0, const double *kx1,
+                                             const double *ky, int i0, int P, double *out, double *centre)
+{
+    double c6[STRIP + 2], w4[STRIP], e4[STRIP];
+    col6(v, i0, P, c6);
+    col4(v, i0 - 1, P, w4);
+    col4(v, i0 + 1, P, e4);
+    strip_apply(c6, w4, e4, kx0, kx1, ky, out);
+    if (centre)
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) centre[q] = c6[q + 1];
+}
+
+// tridiagonal coefficients of the strip from its face coefficients
+__device__ __forceinline__ void strip_abc(const double *kx0, const double *kx1, const double *ky, double *a,
+                                          double *b, double *c)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        a[q] = -ky[q];
+        c[q] = -ky[q + 1];
+        b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
+    }
+}
+
+// z = M_blk^-1 r on the strip, with factors either read (cp/bfp non-null)
+// or recomputed from the coefficients
+__device__ __forceinline__ void strip_block(const Geom &g, int ks, int i0, const double *kx0,
+                                            const double *kx1, const double *ky, const double *cp,
+                                            const double *bfp, const double *rv, double *zv)
+{
+    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
+    strip_abc(kx0, kx1, ky, a, b, c);
+    const int upper = block_top(g, ks);
+    if (cp) {
+        col4(cp, i0, g.pitch, cpv);
+        col4(bfp, i0, g.pitch, bfv);
+    } else {
+        strip_factors(upper, b, a, c, cpv, bfv);
+    }
+    strip_solve(upper, rv, b, a, cpv, bfv, zv);
+}
+
+// the strip's right-hand side with rows past k1 zeroed (the reference's
+// work-group copy only holds in-bounds rows)
+__device__ __forceinline__ void mask_rows(int ks, int k1, double *v)
+{
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) v[q] = (ks + q <= k1) ? v[q] : 0.0;
+}
+
+// z = M^-1 r for one strip with the preconditioner recomputed from Kx, Ky
+// (rv already masked past the extent)
+template <int PREC>
+__device__ __forceinline__ void strip_precondition(const Geom &g, int ks, int i0, const double *kx0,
+                                                   const double *kx1, const double *ky, const double *rv,
+                                                   double *zv)
+{
+    if (PREC == PREC_BLOCK) {
+        strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+    } else if (PREC == PREC_DIAG) {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) zv[q] = rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+    } else {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) zv[q] = rv[q];
+    }
+}
+
+}  // namespace tl

@@ -71,12 +71,14 @@ void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
                    const double *u0, double *u1, double *un, const RedArgs &ra);
 // p is double-buffered (pA, pB); the iteration counter in the state picks
-// which buffer is read and which is written
+// which buffer is read and which is written. ev_main, when given, is recorded
+// after the iteration kernel and before its finisher.
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
-                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra);
+                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra,
+                       hipEvent_t ev_main = nullptr);
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, double *r, double *z, double *sd,
-                        const double *Kx, const double *Ky, const RedArgs &ra);
+                        const double *Kx, const double *Ky, const RedArgs &ra, hipEvent_t ev_main = nullptr);
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
                          const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra);
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,

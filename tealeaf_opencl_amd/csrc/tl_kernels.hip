@@ -1,18 +1,5 @@
-// HIP kernels for gfx950 (MI355X): the TeaLeaf implicit diffusion solve.
-//
-// Work decomposition (every kernel): a 256-thread block is four wavefronts;
-// a tile is 64 columns x 16 rows; wavefront w of the block owns the 4-row
-// strip w of the tile and lane l owns column l. A thread therefore handles one
-// column of one 4-row strip, which is exactly the y-strip of the strip-4
-// block-Jacobi preconditioner (kernel_files/tea_block_jacobi.cl), so the
-// tridiagonal solve of a strip stays in one thread's registers. Each row of a
-// strip is one fully coalesced 512-byte load per wavefront. Blocks stride over
-// tiles; the grid is capped so reductions produce few partials.
-//
-// Reductions: wavefront shuffle sum, then the four wavefronts in fixed order,
-// then one partial per block; the last block to finish (atomic ticket) sums
-// the partials in block order and runs the scalar algebra of the iteration.
-// The result is deterministic for a given grid.
+// HIP kernels for gfx950 (MI355X): the TeaLeaf implicit diffusion solve in
+// FP64. Tiling, reductions and the strip helpers are in tl_device.h.
 //
 // Two families:
 //  * reference-granularity kernels (one per kernel_files/*.cl kernel, same
@@ -23,101 +10,17 @@
 //    instead of stored, and the diagonal / block-Jacobi factors are
 //    recomputed in registers from Kx, Ky with the same expressions as
 //    tea_leaf_init_jac_diag and tea_leaf_block_init.
+//
+// A reduction kernel stores one partial per block; the one-block finisher
+// k_finish that follows it sums them and runs the solver's scalar algebra on
+// the device-resident SolveState.
 #include <hip/hip_runtime.h>
 
 #include "tl_common.h"
+#include "tl_device.h"
 #include "tl_kernels.h"
 
 namespace tl {
-
-// ---------------------------------------------------------------------------
-// tiling helpers
-// ---------------------------------------------------------------------------
-#define TL_TILE_LOOP(e)                                                          \
-    const int tx_ = threadIdx.x & (TILE_X - 1);                                  \
-    const int ty_ = threadIdx.x / TILE_X;                                        \
-    const int nt_ = (e).ntx * (e).nty;                                           \
-    for (int t_ = blockIdx.x; t_ < nt_; t_ += gridDim.x) {                       \
-        const int j = (e).j0 + (t_ % (e).ntx) * TILE_X + tx_;                    \
-        const int ks = (e).k0 + ((t_ / (e).ntx) * STRIPS_PER_TILE + ty_) * STRIP; \
-        if (j > (e).j1 || ks > (e).k1) continue;
-#define TL_TILE_LOOP_END }
-
-__device__ __forceinline__ int IX(const Geom &g, int j, int k) { return k * g.pitch + j; }
-
-// 5-point operator of tea_leaf_common_cl.cl:28-32 (same association)
-__device__ __forceinline__ double opA(double kx0, double kx1, double ky0, double ky1, double vc,
-                                      double vw, double ve, double vs, double vn)
-{
-    return (1.0 + (ky1 + ky0) + (kx1 + kx0)) * vc - (ky1 * vn + ky0 * vs) - (kx1 * ve + kx0 * vw);
-}
-
-__device__ __forceinline__ double diag_of(double kx0, double kx1, double ky0, double ky1)
-{
-    return (1.0 + (ky1 + ky0) + (kx1 + kx0));
-}
-
-// BLOCK_TOP of macros_cl.cl:6-10 (BLOCK_TOP_CHECK only when y_max % 4 != 0)
-__device__ __forceinline__ int block_top(const Geom &g, int row)
-{
-    if (g.y_max % STRIP) {
-        const int t = g.y_max + 2 - row;
-        return t < STRIP ? t : STRIP;
-    }
-    return STRIP;
-}
-
-// block_solve_func (tea_block_jacobi.cl:6-42) on one strip held in registers.
-// b[k] = COEF_B, a[k] = COEF_A of strip row k; cp/bfp as produced by
-// tea_leaf_block_init; rows >= upper are left at zero.
-__device__ __forceinline__ void strip_solve(int upper, const double *r, const double *b,
-                                            const double *a, const double *cp,
-                                            const double *bfp, double *z)
-{
-    double dp[STRIP];
-#pragma unroll
-    for (int k = 0; k < STRIP; ++k) z[k] = 0.0;
-    if (upper <= 0) return;
-    dp[0] = r[0] / b[0];
-#pragma unroll
-    for (int k = 1; k < STRIP; ++k)
-        if (k < upper) dp[k] = (r[k] - a[k] * dp[k - 1]) * bfp[k];
-    // back substitution from row upper-1
-#pragma unroll
-    for (int k = STRIP - 1; k >= 0; --k) {
-        if (k > upper - 1) continue;
-        if (k == upper - 1) z[k] = dp[k];
-        else z[k] = dp[k] - cp[k] * z[k + 1];
-    }
-}
-
-// Thomas factors of tea_leaf_block_init (tea_block_jacobi.cl:78-105)
-__device__ __forceinline__ void strip_factors(int upper, const double *b, const double *a,
-                                              const double *c, double *cp, double *bfp)
-{
-    cp[0] = c[0] / b[0];
-    bfp[0] = 0.0;
-#pragma unroll
-    for (int k = 1; k < STRIP; ++k) {
-        if (k < upper) {
-            bfp[k] = 1.0 / (b[k] - a[k] * cp[k - 1]);
-            cp[k] = c[k] * bfp[k];
-        } else {
-            bfp[k] = 0.0;
-            cp[k] = 0.0;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// reductions
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
 
 // tea_solve.f90:322-331: error = SQRT(error), exit test, old_error, then the
 // solver-switch test at the top of the next iteration (:151-165)
@@ -222,26 +125,6 @@ __device__ void finish_scalars(const RedArgs &ra, const double *tot)
     }
 }
 
-// Block reduction of NR sums into this block's partials (plain stores; the
-// finisher kernel that follows reads them after the kernel boundary). Must be
-// reached by every thread of the block.
-template <int NR>
-__device__ __forceinline__ void block_partial(const double (&v)[NR], const RedArgs &ra)
-{
-    __shared__ double lds[STRIPS_PER_TILE][MAX_RED];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < NR; ++q) {
-        const double s = wave_sum(v[q]);
-        if (lane == 0) lds[w][q] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < NR) {
-        const int q = threadIdx.x;
-        ra.partials[blockIdx.x * MAX_RED + q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
-    }
-}
-
 __device__ __forceinline__ bool iteration_mode(int mode)
 {
     return mode == RED_CG_PW || mode == RED_CG_RRN || mode == RED_PPCG_PW || mode == RED_PPCG_RRN ||
@@ -249,129 +132,17 @@ __device__ __forceinline__ bool iteration_mode(int mode)
 }
 
 // Sums the partials of the preceding reduction kernel in block order and runs
-// the scalar algebra of its mode. One block of 256 threads. An iteration-mode
-// finisher of a halted solver does nothing, like the kernel before it.
-__global__ void __launch_bounds__(BLOCK_THREADS) k_finish(RedArgs ra, int nblocks, int nq)
+// the scalar algebra of its mode. An iteration-mode finisher of a halted
+// solver does nothing, like the kernel before it.
+__global__ void __launch_bounds__(FINISH_THREADS) k_finish(RedArgs ra, int nblocks, int nq)
 {
     if (ra.st && iteration_mode(ra.mode) && ra.st->halt) return;
-    __shared__ double lds[STRIPS_PER_TILE][MAX_RED];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double acc[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
-#pragma unroll
-        for (int q = 0; q < MAX_RED; ++q)
-            if (q < nq) acc[q] += ra.partials[b * MAX_RED + q];
-#pragma unroll
-    for (int q = 0; q < MAX_RED; ++q) {
-        const double s = wave_sum(acc[q]);
-        if (lane == 0) lds[w][q] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double tot[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int q = 0; q < MAX_RED; ++q)
-            if (q < nq) tot[q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
-        finish_scalars(ra, tot);
-    }
+    double tot[MAX_RED];
+    sum_partials(ra.partials, nblocks, nq, tot);
+    if (threadIdx.x == 0) finish_scalars(ra, tot);
 }
 
 __device__ __forceinline__ bool halted(const SolveState *st) { return st && st->halt; }
-
-// ---------------------------------------------------------------------------
-// strip helpers. Every load of a strip is issued before any of it is used;
-// rows past the extent are read too (the allocation margins hold them) and
-// only stores and sums are predicated on the row being inside the extent.
-// ---------------------------------------------------------------------------
-
-// rows ks .. ks+3 of column j (v[q] = row ks+q)
-__device__ __forceinline__ void col4(const double *f, int i0, int P, double *v)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) v[q] = f[i0 + q * P];
-}
-
-// rows ks-1 .. ks+4 of column j (v[q] = row ks+q-1)
-__device__ __forceinline__ void col6(const double *f, int i0, int P, double *v)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP + 2; ++q) v[q] = f[i0 + (q - 1) * P];
-}
-
-// face coefficients of a strip: Kx at the cell and its east neighbour, Ky at
-// rows ks .. ks+4
-__device__ __forceinline__ void strip_k(const double *Kx, const double *Ky, int i0, int P, double *kx0,
-                                        double *kx1, double *ky)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        kx0[q] = Kx[i0 + q * P];
-        kx1[q] = Kx[i0 + q * P + 1];
-    }
-#pragma unroll
-    for (int q = 0; q < STRIP + 1; ++q) ky[q] = Ky[i0 + q * P];
-}
-
-// w = A v on the strip: c6 = centre column rows ks-1..ks+4, w4/e4 = west/east
-__device__ __forceinline__ void strip_apply(const double *c6, const double *w4, const double *e4,
-                                            const double *kx0, const double *kx1, const double *ky, double *out)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q)
-        out[q] = opA(kx0[q], kx1[q], ky[q], ky[q + 1], c6[q + 1], w4[q], e4[q], c6[q], c6[q + 2]);
-}
-
-// A v for a field read straight from memory
-__device__ __forceinline__ void strip_matvec(const double *v, const double *kx0, const double *kx1,
-                                             const double *ky, int i0, int P, double *out, double *centre)
-{
-    double c6[STRIP + 2], w4[STRIP], e4[STRIP];
-    col6(v, i0, P, c6);
-    col4(v, i0 - 1, P, w4);
-    col4(v, i0 + 1, P, e4);
-    strip_apply(c6, w4, e4, kx0, kx1, ky, out);
-    if (centre)
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) centre[q] = c6[q + 1];
-}
-
-// tridiagonal coefficients of the strip from its face coefficients
-__device__ __forceinline__ void strip_abc(const double *kx0, const double *kx1, const double *ky, double *a,
-                                          double *b, double *c)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        a[q] = -ky[q];
-        c[q] = -ky[q + 1];
-        b[q] = (1.0 + (ky[q + 1] + ky[q]) + (kx1[q] + kx0[q]));
-    }
-}
-
-// z = M_blk^-1 r on the strip, with factors either read (cp/bfp non-null)
-// or recomputed from the coefficients
-__device__ __forceinline__ void strip_block(const Geom &g, int ks, int i0, const double *kx0,
-                                            const double *kx1, const double *ky, const double *cp,
-                                            const double *bfp, const double *rv, double *zv)
-{
-    double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
-    strip_abc(kx0, kx1, ky, a, b, c);
-    const int upper = block_top(g, ks);
-    if (cp) {
-        col4(cp, i0, g.pitch, cpv);
-        col4(bfp, i0, g.pitch, bfv);
-    } else {
-        strip_factors(upper, b, a, c, cpv, bfv);
-    }
-    strip_solve(upper, rv, b, a, cpv, bfv, zv);
-}
-
-// the strip's right-hand side with rows past k1 zeroed (the reference's
-// work-group copy only holds in-bounds rows)
-__device__ __forceinline__ void mask_rows(int ks, int k1, double *v)
-{
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) v[q] = (ks + q <= k1) ? v[q] : 0.0;
-}
 
 // ===========================================================================
 // reference-granularity kernels
@@ -492,7 +263,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_field_summary(Geom g, Ext e, 
         acc[3] += cm * y[q];
     }
     TL_TILE_LOOP_END
-    block_partial<4>(acc, ra);
+    block_partial<4>(acc, ra.partials);
 }
 
 // update_halo_{bottom,top} (update_halo_cl.cl:43-69): one thread per column
@@ -645,7 +416,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_dot(Geom g, Ext e, const doub
     for (int q = 0; q < STRIP; ++q)
         if (ks + q <= e.k1) acc[0] += x[q] * y[q];
     TL_TILE_LOOP_END
-    block_partial<1>(acc, ra);
+    block_partial<1>(acc, ra.partials);
 }
 
 // tea_leaf_finalise (tea_leaf_common_cl.cl:4-15)
@@ -732,7 +503,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_init_p(Geom g, Ext e, doub
         acc[0] += rv[q] * pv;
     }
     TL_TILE_LOOP_END
-    block_partial<1>(acc, ra);
+    block_partial<1>(acc, ra.partials);
 }
 
 // tea_leaf_cg_solve_calc_w (tea_leaf_cg_cl.cl:45-69)
@@ -753,7 +524,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_w(Geom g, Ext e, cons
         acc[0] += pc[q] * wv[q];
     }
     TL_TILE_LOOP_END
-    block_partial<1>(acc, ra);
+    block_partial<1>(acc, ra.partials);
 }
 
 // tea_leaf_cg_solve_calc_ur (tea_leaf_cg_cl.cl:73-137)
@@ -801,7 +572,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, dou
         }
     }
     TL_TILE_LOOP_END
-    block_partial<1>(acc, ra);
+    block_partial<1>(acc, ra.partials);
 }
 
 // tea_leaf_cg_solve_calc_p (tea_leaf_cg_cl.cl:141-160)
@@ -887,7 +658,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_u(Geom g, Ext e, double
         if (want) acc[0] += rv[q] * rv[q];
     }
     TL_TILE_LOOP_END
-    if (ra.st) block_partial<1>(acc, ra);
+    if (ra.st) block_partial<1>(acc, ra.partials);
 }
 
 // tea_leaf_ppcg_solve_init_sd (tea_leaf_ppcg_cl.cl:4-60)
@@ -1005,7 +776,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const d
         acc[0] += fabs(v - c6[q + 1]);
     }
     TL_TILE_LOOP_END
-    block_partial<1>(acc, ra);
+    block_partial<1>(acc, ra.partials);
 }
 
 // ===========================================================================
@@ -1057,7 +828,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
         acc[0] += c6[q + 1] * wv[q];
     }
     TL_TILE_LOOP_END
-    block_partial<1>(acc, ra);
+    block_partial<1>(acc, ra.partials);
 }
 
 // CG / PPCG outer step, second half (tea_leaf_cg_cl.cl:73-137): w = A p
@@ -1106,7 +877,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
         else acc[0] += rv[q] * zz;
     }
     TL_TILE_LOOP_END
-    if (!PPCG) block_partial<1>(acc, ra);
+    if (!PPCG) block_partial<1>(acc, ra.partials);
 }
 
 // PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
@@ -1147,7 +918,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
         if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
     }
     TL_TILE_LOOP_END
-    if (LAST) block_partial<1>(acc, ra);
+    if (LAST) block_partial<1>(acc, ra.partials);
 }
 
 // ===========================================================================
@@ -1163,11 +934,16 @@ static inline int grid_for(const Ext &e, int cap)
     hipLaunchKernelGGL(kern, dim3(grid_for(e, cap)), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__)
 
 // reduction kernel + the finisher that sums its per-block partials
-#define TL_LAUNCH_RED(kern, e, cap, stream, nq, ra, ...)                                           \
+#define TL_LAUNCH_RED(kern, e, cap, stream, nq, ra, ...) TL_LAUNCH_RED_EV(kern, e, cap, stream, nq, ra, nullptr, __VA_ARGS__)
+
+// ... with an optional event recorded between the reduction kernel and its
+// finisher (per-kernel timing of the iteration kernels)
+#define TL_LAUNCH_RED_EV(kern, e, cap, stream, nq, ra, ev, ...)                                    \
     do {                                                                                           \
         const int grid_ = grid_for(e, cap);                                                        \
         hipLaunchKernelGGL(kern, dim3(grid_), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__);         \
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK_THREADS), 0, stream, ra, grid_, nq);      \
+        if (ev) hipEventRecord(ev, stream);                                                        \
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, stream, ra, grid_, nq);      \
     } while (0)
 
 void launch_init_chunk_1d(hipStream_t s, const Geom &g, double xmin, double ymin, double dx, double dy,
@@ -1370,9 +1146,10 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
 }
 
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
-                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra)
+                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra,
+                       hipEvent_t ev_main)
 {
-    TL_LAUNCH_RED(k_fused_cg_w, e, cap, s, 1, ra, g, e, src, pA, pB, Kx, Ky, ra);
+    TL_LAUNCH_RED_EV(k_fused_cg_w, e, cap, s, 1, ra, ev_main, g, e, src, pA, pB, Kx, Ky, ra);
 }
 
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
@@ -1383,12 +1160,13 @@ void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, co
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, double *r, double *z, double *sd,
-                        const double *Kx, const double *Ky, const RedArgs &ra)
+                        const double *Kx, const double *Ky, const RedArgs &ra, hipEvent_t ev_main)
 {
-#define TL_FUR(P, M)                                                                               \
-    do {                                                                                           \
-        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra);  \
-        else TL_LAUNCH_RED((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra); \
+#define TL_FUR(P, M)                                                                                          \
+    do {                                                                                                      \
+        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra);             \
+        else TL_LAUNCH_RED_EV((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, ev_main, g, e, u, pA, pB, r, z, sd, Kx, \
+                              Ky, ra);                                                                        \
     } while (0)
     if (ppcg) {
         if (prec == PREC_BLOCK) TL_FUR(PREC_BLOCK, 1);

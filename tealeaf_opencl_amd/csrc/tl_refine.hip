@@ -1,0 +1,535 @@
+// Refined solve of A u = u0 for one implicit timestep (after common_init has
+// formed Kx, Ky and u = u0): a preconditioned CG whose vectors (r, z, p, the
+// solution increment d) and stencil coefficients are stored in FP32 or FP64,
+// with every arithmetic operation, every dot product and every scalar in FP64,
+// wrapped in FP64 reliable updates: whenever the recursive residual has
+// dropped by `delta` since the last update, or has reached the tolerance, the
+// increment is added into the FP64 solution (u += d, d = 0) and the residual
+// is replaced by the true FP64 residual u0 - A u (residual replacement,
+// van der Vorst & Ye, SIAM J. Sci. Comput. 22(5), 2000). The search direction
+// is kept across updates, so the Krylov recurrence is not restarted.
+// Convergence is declared on the true FP64 residual only:
+// ||u0 - A u||_2 <= tol * ||u0 - A u_init||_2.
+//
+// The CG recurrence is the one of tea_leaf_cg_cl.cl:45-160 with the same
+// operator (tea_leaf_common_cl.cl:17-36) and the same preconditioners
+// (tea_leaf_common_cl.cl:127-142, tea_block_jacobi.cl:1-105); the reference
+// has no mixed-precision path, so the refinement loop itself has no
+// counterpart there.
+//
+// Kernels per inner iteration (same fusion as the FP64 driver): k_rf_w
+// (p = z + beta p_old, w = A p, p.w) and k_rf_ur (d += alpha p,
+// r -= alpha A p, z = M^-1 r, r.z and r.r), each followed by a one-block
+// finisher that runs the scalar algebra on the device-resident RefineState.
+// Batches of iterations are captured in a hipGraph; the host only acts
+// between batches (reliable updates and the final test).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include "../../include/tealeaf_mi355x.h"
+#include "tl_chunk.h"
+#include "tl_device.h"
+#include "tl_kernels.h"
+
+namespace tl {
+
+namespace {
+
+enum { RW_KX = 0, RW_KY, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_N };
+
+struct RfArgs {
+    double *partials;
+    RefineState *st;
+    int mode;
+};
+
+__device__ void rf_scalars(RefineState *s, int mode, const double *tot)
+{
+    const double rz = tot[0], rr = tot[1];
+    switch (mode) {
+    case RF_UPDATE: {
+        if (s->updates == 0) {
+            s->rr0 = rr;
+            s->beta = 0.0;
+        } else {
+            s->beta = rz / s->rz_prev;
+        }
+        s->rro = rz;
+        s->rr_true = rr;
+        s->rr = rr;
+        s->rmax = sqrt(rr);
+        s->updates += 1;
+        s->want_update = 0;
+        if (sqrt(rr) <= s->tol * sqrt(s->rr0)) {
+            s->converged = 1;
+            s->halt = 1;
+        } else if (s->n >= s->max_iters || !isfinite(rr) || !isfinite(rz)) {
+            s->halt = 1;
+        } else {
+            s->halt = 0;
+        }
+        break;
+    }
+    case RF_PW:
+        s->pw = tot[0];
+        s->alpha = s->rro / s->pw;
+        break;
+    case RF_RRN: {
+        s->rz_prev = s->rro;
+        s->beta = rz / s->rro;
+        s->rro = rz;
+        s->rr = rr;
+        s->n += 1;
+        const double rn = sqrt(rr);
+        if (rn > s->rmax) s->rmax = rn;
+        if (rn <= s->tol * sqrt(s->rr0) || rn < s->delta * s->rmax || s->n >= s->max_iters || !isfinite(rz)) {
+            s->want_update = 1;
+            s->halt = 1;
+        }
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+__global__ void __launch_bounds__(FINISH_THREADS) k_rf_finish(RfArgs ra, int nblocks)
+{
+    if (ra.mode != RF_UPDATE && ra.st->halt) return;
+    double tot[MAX_RED];
+    sum_partials(ra.partials, nblocks, 2, tot);
+    if (threadIdx.x == 0) rf_scalars(ra.st, ra.mode, tot);
+}
+
+template <typename T>
+__device__ __forceinline__ double rnd(double v)
+{
+    return (double)(T)v;
+}
+
+// Kx, Ky over the whole padded index space, rounded to the inner storage
+__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *Kx, const double *Ky,
+                                                   float *kx, float *ky)
+{
+    const int w = g8.x_max + 2 * g8.hd, h = g8.y_max + 2 * g8.hd;
+    const long n = (long)w * h;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % w), k = (int)(t / w);
+        kx[IX(g4, j, k)] = (float)Kx[IX(g8, j, k)];
+        ky[IX(g4, j, k)] = (float)Ky[IX(g8, j, k)];
+    }
+}
+
+// Reliable update: u_out = u_in + d (FP64), r = u0 - A u_out with the FP64
+// coefficients, then r and z = M^-1 r rounded to the inner storage (M built
+// from the coefficients rounded the same way, i.e. the inner operator's
+// preconditioner). Sums: r.z of the stored values, and the FP64 r.r. d is
+// read at stencil neighbours, so it is cleared after the kernel, not in it.
+template <typename T, int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, Ext e, const double *u_in,
+                                                             double *u_out, const T *d, const double *u0,
+                                                             const double *Kx, const double *Ky, T *r, T *z,
+                                                             RfArgs ra)
+{
+    double acc[2] = {0.0, 0.0};
+    TL_TILE_LOOP(e)
+    const int i8 = IX(g8, j, ks), P8 = g8.pitch;
+    const int i4 = IX(g4, j, ks), P4 = g4.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    double uc[STRIP + 2], uw[STRIP], ue[STRIP], dc[STRIP + 2], dw[STRIP], de[STRIP], b[STRIP];
+    strip_k(Kx, Ky, i8, P8, kx0, kx1, ky);
+    col6(u_in, i8, P8, uc);
+    col4(u_in, i8 - 1, P8, uw);
+    col4(u_in, i8 + 1, P8, ue);
+    col6(d, i4, P4, dc);
+    col4(d, i4 - 1, P4, dw);
+    col4(d, i4 + 1, P4, de);
+    col4(u0, i8, P8, b);
+    double c6[STRIP + 2], w4[STRIP], e4[STRIP], av[STRIP], rv[STRIP], rt[STRIP], zv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) c6[q] = uc[q] + dc[q];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        w4[q] = uw[q] + dw[q];
+        e4[q] = ue[q] + de[q];
+    }
+    strip_apply(c6, w4, e4, kx0, kx1, ky, av);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = b[q] - av[q];
+    mask_rows(ks, e.k1, rv);
+    double kx0t[STRIP], kx1t[STRIP], kyt[STRIP + 1];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        rt[q] = rnd<T>(rv[q]);
+        kx0t[q] = rnd<T>(kx0[q]);
+        kx1t[q] = rnd<T>(kx1[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) kyt[q] = rnd<T>(ky[q]);
+    strip_precondition<PREC>(g8, ks, i8, kx0t, kx1t, kyt, rt, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const T zt = (T)zv[q];
+        u_out[i8 + q * P8] = c6[q + 1];
+        r[i4 + q * P4] = (T)rt[q];
+        z[i4 + q * P4] = zt;
+        acc[0] += rt[q] * (double)zt;
+        acc[1] += rv[q] * rv[q];
+    }
+    TL_TILE_LOOP_END
+    block_partial<2>(acc, ra.partials);
+}
+
+// p = z + beta p_old (rounded to the storage), w = A p, p.w. p is
+// double-buffered by the parity of the iteration counter.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *src, T *pA, T *pB,
+                                                        const T *Kx, const T *Ky, RfArgs ra)
+{
+    const RefineState *st = ra.st;
+    if (st->halt) return;
+    const double beta = st->beta;
+    const bool odd = (st->n & 1) != 0;
+    const T *p_old = odd ? pA : pB;
+    T *p_new = odd ? pB : pA;
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
+    double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col6(src, i0, P, sc);
+    col6(p_old, i0, P, pc);
+    col4(src, i0 - 1, P, sw);
+    col4(p_old, i0 - 1, P, pw_);
+    col4(src, i0 + 1, P, se);
+    col4(p_old, i0 + 1, P, pe);
+    double c6[STRIP + 2], w4[STRIP], e4[STRIP], wv[STRIP];
+    // beta == 0 (first iteration) ignores p_old entirely
+#pragma unroll
+    for (int q = 0; q < STRIP + 2; ++q) c6[q] = rnd<T>(beta != 0.0 ? sc[q] + beta * pc[q] : sc[q]);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        w4[q] = rnd<T>(beta != 0.0 ? sw[q] + beta * pw_[q] : sw[q]);
+        e4[q] = rnd<T>(beta != 0.0 ? se[q] + beta * pe[q] : se[q]);
+    }
+    strip_apply(c6, w4, e4, kx0, kx1, ky, wv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        p_new[i0 + q * P] = (T)c6[q + 1];
+        acc[0] += c6[q + 1] * wv[q];
+    }
+    TL_TILE_LOOP_END
+    block_partial<1>(acc, ra.partials);
+}
+
+// d += alpha p, r -= alpha A p (A p recomputed from the stored p),
+// z = M^-1 r with the preconditioner recomputed from the stored
+// coefficients; r.z and r.r of the stored values.
+template <typename T, int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
+                                                         T *z, const T *Kx, const T *Ky, RfArgs ra)
+{
+    const RefineState *st = ra.st;
+    if (st->halt) return;
+    const T *p = (st->n & 1) ? pB : pA;
+    const double alpha = st->alpha;
+    double acc[2] = {0.0, 0.0};
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
+    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4(d, i0, P, dv);
+    col4(r, i0, P, rv);
+    strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        dv[q] += alpha * pc[q];
+        rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
+    }
+    mask_rows(ks, e.k1, rv);
+    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        const T zt = (T)zv[q];
+        d[i] = (T)dv[q];
+        r[i] = (T)rv[q];
+        if (PREC != PREC_NONE) z[i] = zt;
+        acc[0] += rv[q] * (double)zt;
+        acc[1] += rv[q] * rv[q];
+    }
+    TL_TILE_LOOP_END
+    block_partial<2>(acc, ra.partials);
+}
+
+inline int grid_of(const Ext &e, int cap)
+{
+    const int tiles = e.ntx * e.nty;
+    return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
+}
+
+template <typename T>
+void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, double *u_out)
+{
+    const Ext e = ext_interior(c);
+    const int grid = grid_of(e, c->cap);
+    const RfArgs ra{c->partials, w.st, RF_UPDATE};
+    T *const *f = reinterpret_cast<T *const *>(w.f);
+#define TL_RFU(P)                                                                                           \
+    hipLaunchKernelGGL((k_rf_update<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, u_out, \
+                       f[RW_D], c->f2[F_U0], c->f2[F_KX], c->f2[F_KY], f[RW_R], f[RW_Z], ra)
+    if (c->prec == PREC_BLOCK) TL_RFU(PREC_BLOCK);
+    else if (c->prec == PREC_DIAG) TL_RFU(PREC_DIAG);
+    else TL_RFU(PREC_NONE);
+#undef TL_RFU
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra, grid);
+    // d = 0 (never inside a captured graph: updates are launched eagerly)
+    hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
+}
+
+template <typename T>
+void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w)
+{
+    const Ext e = ext_interior(c);
+    const int grid = grid_of(e, c->cap);
+    T *const *f = reinterpret_cast<T *const *>(w.f);
+    const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
+    const RfArgs ra_w{c->partials, w.st, RF_PW}, ra_u{c->partials, w.st, RF_RRN};
+    hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
+                       f[RW_KX], f[RW_KY], ra_w);
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
+#define TL_RFI(P)                                                                                       \
+    hipLaunchKernelGGL((k_rf_ur<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
+                       f[RW_P2], f[RW_R], f[RW_Z], f[RW_KX], f[RW_KY], ra_u)
+    if (c->prec == PREC_BLOCK) TL_RFI(PREC_BLOCK);
+    else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
+    else TL_RFI(PREC_NONE);
+#undef TL_RFI
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
+}
+
+constexpr int RF_BATCH = 16;
+
+void release_work(tl_chunk::RefineWork &w)
+{
+    if (w.graph.exec) hipGraphExecDestroy(w.graph.exec);
+    for (void *p : w.base)
+        if (p) hipFree(p);
+    if (w.st) hipFree(w.st);
+    if (w.h_st) hipHostFree(w.h_st);
+    w = tl_chunk::RefineWork();
+}
+
+int rf_alloc_impl(tl_chunk *c, int which)
+{
+    tl_chunk::RefineWork &w = c->rf[which];
+    hipError_t e;
+    w.bytes = which ? 4 : 8;
+    w.g = c->g;
+    if (which == 1) {
+        // FP32 layout: pitch a multiple of 32 floats and column hd on a
+        // 128-byte boundary, margins as for the FP64 fields
+        const int hd = c->g.hd;
+        w.g.pitch = (c->g.x_max + 2 * hd + 2 + 31) / 32 * 32;
+        const long colpad = (32 - (hd % 32)) % 32;
+        const long rows = (long)c->g.y_max + 2 * hd + 16;
+        w.off = 8L * w.g.pitch + colpad;
+        w.elems = rows * w.g.pitch + colpad + 64;
+        for (int i = 0; i < RW_N; ++i) {
+            if ((e = hipMalloc(&w.base[i], sizeof(float) * w.elems)) != hipSuccess) return hip_fail("hipMalloc refine", e);
+            hipMemsetAsync(w.base[i], 0, sizeof(float) * w.elems, c->s);
+            w.f[i] = static_cast<float *>(w.base[i]) + w.off;
+        }
+    } else {
+        // FP64: the chunk's own r, z, p fields and coefficients, plus d
+        w.off = c->off;
+        w.elems = c->elems2d;
+        if ((e = hipMalloc(&w.base[RW_D], sizeof(double) * w.elems)) != hipSuccess) return hip_fail("hipMalloc refine", e);
+        hipMemsetAsync(w.base[RW_D], 0, sizeof(double) * w.elems, c->s);
+        w.f[RW_D] = static_cast<double *>(w.base[RW_D]) + w.off;
+        w.f[RW_KX] = c->f2[F_KX];
+        w.f[RW_KY] = c->f2[F_KY];
+        w.f[RW_R] = c->f2[F_R];
+        w.f[RW_Z] = c->f2[F_Z];
+        w.f[RW_P] = c->f2[F_P];
+        w.f[RW_P2] = c->f2[F_P2];
+    }
+    if ((e = hipMalloc(&w.st, sizeof(RefineState))) != hipSuccess) return hip_fail("hipMalloc refine state", e);
+    if ((e = hipHostMalloc(&w.h_st, sizeof(RefineState) * 2, hipHostMallocDefault)) != hipSuccess)
+        return hip_fail("hipHostMalloc refine state", e);
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("refine alloc", e);
+    return TL_OK;
+}
+
+int rf_alloc(tl_chunk *c, int which)
+{
+    if (c->rf[which].st) return TL_OK;
+    const int rc = rf_alloc_impl(c, which);
+    if (rc != TL_OK) release_work(c->rf[which]);
+    return rc;
+}
+
+int rf_state_get(tl_chunk *c, tl_chunk::RefineWork &w, RefineState *h)
+{
+    hipError_t e = hipMemcpyAsync(&w.h_st[0], w.st, sizeof(RefineState), hipMemcpyDeviceToHost, c->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    if (e != hipSuccess) return hip_fail("refine state download", e);
+    *h = w.h_st[0];
+    return TL_OK;
+}
+
+template <typename T>
+int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
+{
+    const int key = c->prec;
+    if (w.graph.exec && w.graph.key == key) return TL_OK;
+    if (w.graph.exec) {
+        hipGraphExecDestroy(w.graph.exec);
+        w.graph.exec = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
+    for (int b = 0; b < RF_BATCH; ++b) launch_iteration<T>(c, w);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(c->s, &graph);
+    if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
+    e = hipGraphInstantiate(&w.graph.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        w.graph.exec = nullptr;
+        return hip_fail("hipGraphInstantiate", e);
+    }
+    w.graph.key = key;
+    return TL_OK;
+}
+
+// Replays the iteration graph until the device state halts; batch i+1 is
+// queued before the state after batch i is examined.
+int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches)
+{
+    hipError_t e;
+    auto enqueue = [&](int sl) -> hipError_t {
+        hipError_t ee = hipGraphLaunch(w.graph.exec, c->s);
+        if (ee != hipSuccess) return ee;
+        ee = hipMemcpyAsync(&w.h_st[sl], w.st, sizeof(RefineState), hipMemcpyDeviceToHost, c->s);
+        if (ee != hipSuccess) return ee;
+        return hipEventRecord(c->ev_batch[sl], c->s);
+    };
+    int slot = 0;
+    if ((e = enqueue(0)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
+    for (long guard = 0;; ++guard) {
+        if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
+        if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
+        if (w.h_st[slot].halt) break;
+        if (guard > max_batches) {
+            hipStreamSynchronize(c->s);
+            set_error("refined solve did not halt after %ld batches", guard);
+            return TL_ERR_SOLVER;
+        }
+        slot = 1 - slot;
+    }
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("batch drain", e);
+    return TL_OK;
+}
+
+template <typename T>
+int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl_refine_result *out)
+{
+    int rc = rf_alloc(c, which);
+    if (rc != TL_OK) return rc;
+    tl_chunk::RefineWork &w = c->rf[which];
+    hipError_t e;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if ((e = hipEventCreate(&t0)) != hipSuccess || (e = hipEventCreate(&t1)) != hipSuccess) {
+        if (t0) hipEventDestroy(t0);
+        return hip_fail("hipEventCreate", e);
+    }
+    struct EvGuard {
+        hipEvent_t a, b;
+        ~EvGuard() { hipEventDestroy(a); hipEventDestroy(b); }
+    } guard{t0, t1};
+    if ((rc = rf_capture<T>(c, w)) != TL_OK) return rc;
+
+    RefineState h;
+    memset(&h, 0, sizeof h);
+    h.tol = tol;
+    h.delta = delta;
+    h.max_iters = max_iters;
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("refine start", e);
+    w.h_st[0] = h;
+    hipEventRecord(t0, c->s);
+    hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
+    hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
+    if (which == 1)
+        hipLaunchKernelGGL(k_rf_narrow, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_KX], c->f2[F_KY],
+                           static_cast<float *>(w.f[RW_KX]), static_cast<float *>(w.f[RW_KY]));
+    double *ubuf[2] = {c->f2[F_U], c->f2[F_U2]};
+    int cur = 0;
+    launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);
+    cur = 1 - cur;
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine update", e);
+    if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
+    const long max_batches = max_iters / RF_BATCH + 4;
+    while (!h.converged && h.n < max_iters && isfinite(h.rr)) {
+        if ((rc = rf_batches(c, w, max_batches)) != TL_OK) return rc;
+        if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
+        if (!h.want_update) break;
+        launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);
+        cur = 1 - cur;
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine update", e);
+        if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
+    }
+    if (cur == 1) launch_copy(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U2], c->f2[F_U]);
+    hipEventRecord(t1, c->s);
+    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("refine finish", e);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t0, t1);
+    if (out) {
+        memset(out, 0, sizeof *out);
+        out->seconds = 1e-3 * ms;
+        out->initial_residual = sqrt(h.rr0);
+        out->rel_residual = h.rr0 > 0 ? sqrt(h.rr_true / h.rr0) : 0.0;
+        out->iterations = h.n;
+        out->updates = h.updates;
+        out->converged = h.converged;
+    }
+    if (!h.converged && !(h.n >= max_iters)) {
+        set_error("refined solve broke down after %d iterations (residual %g)", h.n, sqrt(h.rr));
+        return TL_ERR_SOLVER;
+    }
+    return TL_OK;
+}
+
+}  // namespace
+
+void refine_release(tl_chunk *c)
+{
+    for (auto &w : c->rf) release_work(w);
+}
+
+}  // namespace tl
+
+using namespace tl;
+
+extern "C" int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_iters, double delta,
+                                tl_refine_result *out)
+{
+    if (!c) {
+        set_error("tl_solve_refined: null chunk");
+        return TL_ERR_ARG;
+    }
+    if ((precision != TL_PRECISION_FP64 && precision != TL_PRECISION_MIXED) || !(tol > 0.0) || max_iters < 1 ||
+        delta < 0.0 || delta >= 1.0) {
+        set_error("tl_solve_refined: invalid precision %d, tol %g, max_iters %d or delta %g", precision, tol,
+                  max_iters, delta);
+        return TL_ERR_ARG;
+    }
+    if (precision == TL_PRECISION_MIXED) return rf_solve<float>(c, 1, tol, max_iters, delta, out);
+    return rf_solve<double>(c, 0, tol, max_iters, delta, out);
+}

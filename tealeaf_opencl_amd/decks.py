@@ -11,6 +11,8 @@ The random source is splitmix64 -> U[0,1) with 53 bits.
 """
 import math
 
+import numpy as np
+
 MASK = (1 << 64) - 1
 
 
@@ -59,6 +61,30 @@ KAT_STATES = [
     dict(density=0.1, energy=0.1, geometry="rectangle", xmin=5.0, xmax=6.0, ymin=1.0, ymax=8.0),
     dict(density=0.1, energy=0.1, geometry="rectangle", xmin=5.0, xmax=10.0, ymin=7.0, ymax=8.0),
 ]
+
+
+GEOM = {"rectangle": 1, "circle": 2, "point": 3}
+
+
+def state_arrays(states, x_cells, y_cells, xmax=10.0, ymax=10.0):
+    """Deck states -> the arrays generate_chunk takes, with the read_input.f90:320-327 shrink."""
+    n = len(states)
+    dens = np.array([s["density"] for s in states], dtype=np.float64)
+    ener = np.array([s["energy"] for s in states], dtype=np.float64)
+    xmin = np.array([s.get("xmin", 0.0) for s in states], dtype=np.float64)
+    xmx = np.array([s.get("xmax", 0.0) for s in states], dtype=np.float64)
+    ymin = np.array([s.get("ymin", 0.0) for s in states], dtype=np.float64)
+    ymx = np.array([s.get("ymax", 0.0) for s in states], dtype=np.float64)
+    rad = np.array([s.get("radius", 0.0) for s in states], dtype=np.float64)
+    geom = np.array([GEOM.get(s.get("geometry", ""), 0) for s in states], dtype=np.int32)
+    dx = xmax / float(np.float32(x_cells))
+    dy = ymax / float(np.float32(y_cells))
+    for i in range(1, n):
+        xmin[i] += dx / 100.0
+        ymin[i] += dy / 100.0
+        xmx[i] -= dx / 100.0
+        ymx[i] -= dy / 100.0
+    return dens, ener, xmin, xmx, ymin, ymx, rad, geom
 
 
 def _fmt(v):
@@ -123,3 +149,21 @@ def config2_deck(path, solver="tl_use_cg", preconditioner="jac_block", seed=1, n
         ex += ["tl_ppcg_inner_steps=10"]
     return write_deck(path, n, n, seeded_states(seed, 2, True), solver=solver,
                       preconditioner=preconditioner, end_step=steps, extra=ex, **kw)
+
+
+def config3_deck(path, precision="mixed", preconditioner="jac_block", seed=1, n=4096, steps=1, eps=1e-10,
+                 extra=(), **kw):
+    """4096^2, the two seeded rectangles, conductivity 1/density (contrast
+    1e3), CG + block Jacobi. precision: "mixed" (FP32 inner CG with FP64
+    residual replacement), "fp64" (the same refined solve with FP64 inner
+    vectors) or "reference" (the reference CG and its exit test)."""
+    ex = list(extra)
+    if precision == "mixed":
+        ex.append("tl_mixed_precision")
+    elif precision == "fp64":
+        ex.append("tl_fp64_refined")
+    elif precision != "reference":
+        raise ValueError("precision must be mixed, fp64 or reference")
+    return write_deck(path, n, n, seeded_states(seed, 2, False), solver="tl_use_cg",
+                      preconditioner=preconditioner, end_step=steps, eps=eps,
+                      coefficient="tl_coefficient_inverse_density", extra=ex, **kw)

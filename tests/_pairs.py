@@ -1,31 +1,7 @@
 """Helpers that set up the same chunk on the GPU library and on the CPU oracle."""
 import numpy as np
 
-from tealeaf_opencl_amd.decks import seeded_states
-
-GEOM = {"rectangle": 1, "circle": 2, "point": 3}
-
-
-def state_arrays(states, x_cells, y_cells, xmax=10.0, ymax=10.0):
-    """Deck states -> the arrays generate_chunk takes, with the read_input.f90:320-327 shrink."""
-    n = len(states)
-    dens = np.array([s["density"] for s in states], dtype=np.float64)
-    ener = np.array([s["energy"] for s in states], dtype=np.float64)
-    xmin = np.array([s.get("xmin", 0.0) for s in states], dtype=np.float64)
-    xmx = np.array([s.get("xmax", 0.0) for s in states], dtype=np.float64)
-    ymin = np.array([s.get("ymin", 0.0) for s in states], dtype=np.float64)
-    ymx = np.array([s.get("ymax", 0.0) for s in states], dtype=np.float64)
-    rad = np.array([s.get("radius", 0.0) for s in states], dtype=np.float64)
-    geom = np.array([GEOM.get(s.get("geometry", ""), 0) for s in states], dtype=np.int32)
-    dx = xmax / float(np.float32(x_cells))
-    dy = ymax / float(np.float32(y_cells))
-    for i in range(1, n):
-        xmin[i] += dx / 100.0
-        ymin[i] += dy / 100.0
-        xmx[i] -= dx / 100.0
-        ymx[i] -= dy / 100.0
-    return dens, ener, xmin, xmx, ymin, ymx, rad, geom
-
+from tealeaf_opencl_amd.decks import seeded_states, state_arrays
 
 def setup(chunk, seed=1, n_rect=2, circle=True, dt=0.004, coef=1, reflective=False, zero=(1, 1, 1, 1)):
     """initialise_chunk + generate_chunk + set_field + common_init on one chunk."""

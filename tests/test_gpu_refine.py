@@ -1,0 +1,147 @@
+"""Refined solve (tl_solve_refined): preconditioned CG with FP64 or FP32 inner
+vectors and FP64 residual replacement.
+
+The reference has no mixed-precision path, so parity here is the property the
+solve promises, checked with the oracle's FP64 operator (oracle/tl_oracle.c,
+tea_leaf_common_cl.cl:17-36): the GPU's solution u satisfies
+||u0 - A u||_2 <= tol * ||u0 - A u0||_2 when the residual is formed by the
+oracle. The solution is also compared with the oracle's own FP64 CG run to a
+much tighter tolerance, and deck runs are compared with the oracle's FP64
+field summaries.
+"""
+import numpy as np
+import pytest
+
+from _pairs import setup
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(67, 13, 1), (130, 70, 2), (200, 41, 1), (256, 256, 1)]
+
+
+def oracle_residual(o, u):
+    o.put_field("u", u)
+    o.calc_residual()
+    return float(np.linalg.norm(o.interior(o.get_field("r"))))
+
+
+def oracle_cg(o, tol=1e-14, max_iters=20000):
+    """Plain FP64 PCG with the oracle's reference-granularity kernels, from
+    the oracle's current u (tea_solve.f90:103-125, 270-300)."""
+    o.calc_residual()
+    rro = o.cg_init()
+    r0 = np.sqrt(o.calc_2norm(1))
+    for _ in range(max_iters):
+        pw = o.cg_calc_w()
+        alpha = rro / pw
+        rrn = o.cg_calc_ur(alpha)
+        if np.sqrt(o.calc_2norm(1)) <= tol * r0:
+            break
+        o.cg_calc_p(rrn / rro)
+        rro = rrn
+    return o.get_field("u")
+
+
+def pair(hiplib, oracle, x, y, hd, prec, coef, seed=1):
+    g, o = hiplib.Chunk(x, y, hd, prec), oracle.OracleChunk(x, y, hd, prec)
+    for c in (g, o):
+        setup(c, seed=seed, coef=coef)
+    return g, o
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("prec", [1, 2, 3])
+@pytest.mark.parametrize("coef", [1, 2])
+@pytest.mark.parametrize("x,y,hd", SIZES[:3])
+def test_true_residual(hiplib, oracle, x, y, hd, prec, coef, precision):
+    tol = 1e-10
+    g, o = pair(hiplib, oracle, x, y, hd, prec, coef)
+    r0 = oracle_residual(o, o.get_field("u0"))
+    u_before = g.get_field("u")
+    res = g.solve_refined(precision, tol=tol, max_iters=5000)
+    assert res["converged"], res
+    u = g.get_field("u")
+    r = oracle_residual(o, u)
+    # the oracle's FP64 residual of the GPU solution meets the tolerance
+    assert r <= 1.01 * tol * r0, (r / r0, res)
+    # and agrees with the GPU's own FP64 residual
+    assert abs(res["rel_residual"] - r / r0) <= 1e-2 * tol
+    assert abs(res["initial_residual"] - r0) <= 1e-12 * r0
+    # halos untouched by the solve
+    halo = np.ones(u.shape, dtype=bool)
+    g.interior(halo)[:] = False
+    assert np.array_equal(u[halo], u_before[halo])
+    # close to the oracle's FP64 CG solution
+    o.put_field("u", o.get_field("u0"))
+    u_ref = oracle_cg(o)
+    inner = o.interior(u_ref)
+    err = np.max(np.abs(g.interior(u) - inner)) / np.max(np.abs(inner))
+    assert err < 1e-7, err
+
+
+@pytest.mark.parametrize("prec", [2, 3])
+def test_mixed_iterations_close_to_fp64(hiplib, oracle, prec):
+    """Residual replacement keeps the Krylov recurrence: the FP32-inner solve
+    needs about as many iterations as the FP64 one on a 1e3-contrast problem."""
+    res = []
+    for precision in (0, 1):
+        g, _ = pair(hiplib, oracle, 256, 256, 1, prec, 2)
+        res.append(g.solve_refined(precision, tol=1e-10, max_iters=5000))
+    fp64, mixed = res
+    assert fp64["converged"] and mixed["converged"]
+    assert mixed["iterations"] <= 1.25 * fp64["iterations"] + 5, (fp64, mixed)
+    assert fp64["updates"] <= 3
+    assert mixed["rel_residual"] <= 1e-10
+
+
+def test_deterministic_and_reentrant(hiplib, oracle):
+    out = []
+    for _ in range(2):
+        g, _ = pair(hiplib, oracle, 130, 70, 2, 3, 2)
+        res = g.solve_refined(1, tol=1e-10)
+        out.append((res["iterations"], res["updates"], g.get_field("u")))
+        # a second solve on the same chunk (new timestep state) reuses the
+        # captured graph and workspace
+        g.common_init(2, 0.004)
+        res2 = g.solve_refined(1, tol=1e-10)
+        assert res2["iterations"] == res["iterations"]
+        assert np.array_equal(g.get_field("u"), out[-1][2])
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_iteration_cap(hiplib, oracle):
+    g, _ = pair(hiplib, oracle, 200, 41, 1, 2, 2)
+    res = g.solve_refined(1, tol=1e-12, max_iters=7)
+    assert not res["converged"]
+    assert res["iterations"] == 7
+    assert 0.0 < res["rel_residual"] < 1.0
+
+
+def test_bad_arguments(hiplib, oracle):
+    g, _ = pair(hiplib, oracle, 67, 13, 1, 1, 1)
+    for kw in ({"precision": 2}, {"tol": 0.0}, {"max_iters": 0}, {"delta": 1.0}):
+        args = {"precision": 1, "tol": 1e-10, "max_iters": 100, "delta": 0.1}
+        args.update(kw)
+        with pytest.raises(hiplib.TealeafError):
+            g.solve_refined(**args)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_deck_summary_vs_oracle(hiplib, oracle, tmp_path, precision):
+    """Three timesteps of the inverse-density deck: the refined solve's field
+    summary against the oracle's FP64 reference CG at a tight tolerance."""
+    from tealeaf_opencl_amd import decks
+    deck_ref = decks.config3_deck(str(tmp_path / "ref.in"), precision="reference", n=192, steps=3, eps=1e-13,
+                                  summary_frequency=1)
+    deck = decks.config3_deck(str(tmp_path / "mix.in"), precision=precision, n=192, steps=3, eps=1e-10,
+                              summary_frequency=1)
+    lo = oracle.run_deck(deck_ref, str(tmp_path / "o.log"))
+    lg = hiplib.run_deck(deck, str(tmp_path / "g.log"))
+    assert len(lg["refine"]) == 3 and all(r[3] == 1 for r in lg["refine"])
+    so, sg = np.array(lo["summary"]), np.array(lg["summary"])
+    assert so.shape == sg.shape
+    # vol, mass, density do not depend on the solve; ie and temp do
+    assert np.max(np.abs(sg[:, 1:4] - so[:, 1:4]) / np.abs(so[:, 1:4])) < 1e-12
+    rel = np.abs(sg[:, 4:6] - so[:, 4:6]) / np.abs(so[:, 4:6])
+    assert np.max(rel) < 1e-8, rel
