@@ -24,12 +24,13 @@ sys.path.insert(0, ROOT)
 PEAK_HBM = 8.0e12
 METRIC = "solve s to 1e-10 rel resid @4096² (FP64 vs mixed); HBM GB/s % peak; iters/s"
 
-# algorithmic FP64 bytes per cell (each distinct array read and/or written once)
+# algorithmic FP64 bytes per cell (each distinct array read and/or written
+# once); the face coefficients Kx, Ky are recomputed from the conductivity w
 ALG_BYTES = {
-    # p_new = z + beta p_old; w = A p_new; p.w : R z, p_old, Kx, Ky  W p_new
-    "k_fused_cg_w": 5 * 8,
-    # u += a p; r -= a A p; z = M_blk^-1 r; r.z : R u, p, r, Kx, Ky  W u, r, z
-    "k_fused_cg_ur": 8 * 8,
+    # p_new = z + beta p_old; w = A p_new; p.w : R z, p_old, w  W p_new
+    "k_fused_cg_w": 4 * 8,
+    # u += a p; r -= a A p; z = M_blk^-1 r; r.z : R u, p, r, w  W u, r, z
+    "k_fused_cg_ur": 7 * 8,
 }
 REF_BYTES_PER_ITER = 144  # reference CG + strip-4 block Jacobi (calc_w + calc_ur + calc_p)
 

@@ -198,8 +198,9 @@ int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, 
  * chunk's preconditioner. No reference counterpart: the reference solves in
  * FP64 only (tea_solve.f90:37-420).
  *   precision TL_PRECISION_FP64: inner CG with FP64 vectors;
- *             TL_PRECISION_MIXED: inner CG with FP32 vectors and FP32
- *             stencil coefficients, FP64 arithmetic, dot products and scalars.
+ *             TL_PRECISION_MIXED: inner CG with FP32 vectors and an FP32
+ *             copy of the conductivity (face coefficients formed from it in
+ *             FP64), FP64 arithmetic, dot products and scalars.
  * Either way the increment is folded into the FP64 solution and the residual
  * is replaced by the true FP64 residual u0 - A u whenever the recursive
  * residual norm has fallen below delta times its largest value since the last

@@ -13,11 +13,11 @@ LIBNAME = "libtealeaf_mi355x.so"
 ARCH = os.environ.get("TL_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["tl_kernels.hip", "tl_refine.hip", "tl_chunk.hip", "tl_driver.hip", "tl_deck.cpp"]
-HEADERS = ["tl_common.h", "tl_kernels.h", "tl_chunk.h", "tl_deck.h"]
+HEADERS = ["tl_common.h", "tl_device.h", "tl_kernels.h", "tl_chunk.h", "tl_deck.h"]
 
 # packed FP32 instructions are disabled on every device compile
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Werror=shadow",
           "-I", os.path.join(HERE, "..", "include")]
 
 

@@ -40,6 +40,8 @@ struct tl_chunk {
 
     // host copies of scalars the reference keeps in kernel arguments
     double rx = 0, ry = 0, theta = 0, dx = 0, dy = 0;
+    // face parameters of the latest common_init (F_COND holds its conductivity)
+    tl::Faces faces{};
 
     // profiler_on: per-kernel device time (ms) and calls (ocl_errors.cpp:64-121)
     std::map<std::string, std::pair<double, long>> prof;
@@ -66,7 +68,7 @@ struct tl_chunk {
         tl::Geom g{};                     // layout of the inner fields
         long off = 0, elems = 0;
         void *base[7] = {};               // owned allocations
-        void *f[7] = {};                  // Kx, Ky, r, z, p, p2, d
+        void *f[7] = {};                  // conductivity w, r, z, p, p2, d
         tl::RefineState *st = nullptr;
         tl::RefineState *h_st = nullptr;  // pinned, 2 slots
         GraphSlot graph;

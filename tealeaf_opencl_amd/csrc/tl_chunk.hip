@@ -471,6 +471,15 @@ extern "C" int tl_common_init(tl_chunk *c, int coefficient, double dt, double *r
                        c->f2[F_KY], *rx, *ry, coefficient);
     TL_CHECK_LAUNCH(c, "tea_leaf_init_common");
     prof_end(c, "tea_leaf_init_common");
+    // the same coefficients in recomputable form for the fused kernels
+    launch_conductivity(c->s, c->g, c->f2[F_DENSITY], c->f2[F_COND], coefficient);
+    TL_CHECK_LAUNCH(c, "conductivity");
+    c->faces.rx = *rx;
+    c->faces.ry = *ry;
+    c->faces.zl = !reflective_boundary && zero_boundary[0];
+    c->faces.zr = !reflective_boundary && zero_boundary[1];
+    c->faces.zb = !reflective_boundary && zero_boundary[2];
+    c->faces.zt = !reflective_boundary && zero_boundary[3];
     if (!reflective_boundary) {
         prof_begin(c);
         launch_zero_boundary(c->s, c->g, ext_range(c, -hd, hd, -hd, hd), c->cap, c->f2[F_KX], c->f2[F_KY],

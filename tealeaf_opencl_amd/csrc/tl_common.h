@@ -27,6 +27,9 @@ enum Field2D {
     NUM_PUBLIC_2D,
     // second buffers of the double-buffered iteration kernels
     F_P2 = NUM_PUBLIC_2D, F_SD2, F_U2,
+    // conductivity w (density or 1/density) of the current timestep, from
+    // which the fused iteration kernels recompute Kx, Ky
+    F_COND,
     NUM_FIELDS_2D
 };
 enum Field1D {
@@ -50,6 +53,15 @@ struct Ext {
 struct Geom {
     int x_max, y_max, hd;
     int pitch;
+};
+
+// What tea_leaf_init_common + tea_leaf_zero_boundary (tea_leaf_common_cl.cl:
+// 56-125) leave in Kx, Ky, as parameters for recomputing any face
+// coefficient from the conductivity field: the scale factors and which walls
+// are zeroed (none when the boundary is reflective).
+struct Faces {
+    double rx, ry;
+    int zl, zr, zb, zt;
 };
 
 // Device-resident solver scalars: the per-iteration algebra of

@@ -205,6 +205,61 @@ __device__ __forceinline__ void strip_k(const T *Kx, const T *Ky, int i0, int P,
     for (int q = 0; q < STRIP + 1; ++q) ky[q] = (double)Ky[i0 + q * P];
 }
 
+// Kx at face (j,k) (between cells j-1 and j of row k) and Ky at face (j,k)
+// (between rows k-1 and k of column j) as tea_leaf_init_common computes them
+// (same expression and association, so the values are bitwise those it
+// stores) over its launch extent [1, n+2hd-1] (ocl_kernels.cpp:112), zero
+// outside it, then zeroed like tea_leaf_zero_boundary.
+__device__ __forceinline__ double face_coef(double wa, double wb, double scale)
+{
+    double v = (wa + wb) / (2.0 * wa * wb);
+    v *= scale;
+    return v;
+}
+
+__device__ __forceinline__ bool outside_init(const Geom &g, int j, int k)
+{
+    return j < 1 || j > g.x_max + 2 * g.hd - 1 || k < 1 || k > g.y_max + 2 * g.hd - 1;
+}
+
+__device__ __forceinline__ double kx_at(const Geom &g, const Faces &f, int j, int k, double wl, double wc)
+{
+    const int hd = g.hd;
+    const bool zero = outside_init(g, j, k) || (f.zl && j <= hd) || (f.zr && j > g.x_max - 1 + hd) ||
+                      (f.zb && k < hd) || (f.zt && k > g.y_max - 1 + hd);
+    const double v = face_coef(wl, wc, f.rx);
+    return zero ? 0.0 : v;
+}
+
+__device__ __forceinline__ double ky_at(const Geom &g, const Faces &f, int j, int k, double wd, double wc)
+{
+    const int hd = g.hd;
+    const bool zero = outside_init(g, j, k) || (f.zl && j < hd) || (f.zr && j > g.x_max - 1 + hd) ||
+                      (f.zb && k <= hd) || (f.zt && k > g.y_max - 1 + hd);
+    const double v = face_coef(wd, wc, f.ry);
+    return zero ? 0.0 : v;
+}
+
+// face coefficients of the strip (same outputs as strip_k) recomputed from
+// the conductivity field w (stored as T; the faces are formed in FP64 from
+// the stored values)
+template <typename T>
+__device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, const T *w, int i0, int P, int j, int ks,
+                                             double *kx0, double *kx1, double *ky)
+{
+    double wc[STRIP + 2], ww[STRIP], we[STRIP];
+    col6(w, i0, P, wc);
+    col4(w, i0 - 1, P, ww);
+    col4(w, i0 + 1, P, we);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        kx0[q] = kx_at(g, f, j, ks + q, ww[q], wc[q + 1]);
+        kx1[q] = kx_at(g, f, j + 1, ks + q, wc[q + 1], we[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = ky_at(g, f, j, ks + q, wc[q], wc[q + 1]);
+}
+
 // w = A v on the strip: c6 = centre column rows ks-1..ks+4, w4/e4 = west/east
 __device__ __forceinline__ void strip_apply(const double *c6, const double *w4, const double *e4,
                                             const double *kx0, const double *kx1, const double *ky, double *out)

@@ -356,11 +356,11 @@ void cg_iteration(tl_chunk *c, hipEvent_t *ev)
     const Ext in = ext_interior(c);
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
     if (ev) hipEventRecord(ev[0], c->s);
-    launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_KX], c->f2[F_KY],
+    launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
                       red_args(c, RED_CG_PW, true), ev ? ev[1] : nullptr);
     if (ev) hipEventRecord(ev[2], c->s);
     launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                       c->f2[F_Z], c->f2[F_SD], c->f2[F_KX], c->f2[F_KY], red_args(c, RED_CG_RRN, true),
+                       c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_CG_RRN, true),
                        ev ? ev[3] : nullptr);
 }
 
@@ -440,15 +440,15 @@ int ppcg_graph(tl_chunk *c, int inner)
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
     return capture(c, c->g_ppcg, c->prec * 100000 + inner, [&]() {
         for (int b = 0; b < B; ++b) {
-            launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_KX], c->f2[F_KY],
+            launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
                               red_args(c, RED_PPCG_PW, true));
             launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 1, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                               c->f2[F_Z], c->f2[F_SD], c->f2[F_KX], c->f2[F_KY], red_args(c, RED_STORE, true));
+                               c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_STORE, true));
             for (int i = 1; i <= inner; ++i) {
                 const double *sin = (i & 1) ? c->f2[F_SD] : c->f2[F_SD2];
                 double *sout = (i & 1) ? c->f2[F_SD2] : c->f2[F_SD];
                 launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner, c->f2[F_U], c->f2[F_R], sin, sout,
-                                        c->f2[F_KX], c->f2[F_KY], c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
+                                        c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
                                         red_args(c, RED_PPCG_RRN, true));
             }
         }

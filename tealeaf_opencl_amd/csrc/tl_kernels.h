@@ -24,6 +24,8 @@ void launch_field_summary(hipStream_t s, const Geom &g, const Ext &e, int cap, c
 void launch_halo(hipStream_t s, const Geom &g, double *a, int depth, int face);
 void launch_init_common(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, double *Kx,
                         double *Ky, double rx, double ry, int coef);
+// conductivity (density or 1/density) over the whole padded index space
+void launch_conductivity(hipStream_t s, const Geom &g, const double *rho, double *w, int coef);
 void launch_zero_boundary(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Kx, double *Ky,
                           const int *zb);
 void launch_init_u(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, const double *en,
@@ -72,18 +74,19 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
                    const double *u0, double *u1, double *un, const RedArgs &ra);
 // p is double-buffered (pA, pB); the iteration counter in the state picks
 // which buffer is read and which is written. ev_main, when given, is recorded
-// after the iteration kernel and before its finisher.
+// after the iteration kernel and before its finisher. The fused kernels
+// recompute Kx, Ky from the conductivity W and the face parameters.
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
-                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra,
+                       double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
                        hipEvent_t ev_main = nullptr);
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, double *r, double *z, double *sd,
-                        const double *Kx, const double *Ky, const RedArgs &ra, hipEvent_t ev_main = nullptr);
+                        const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main = nullptr);
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
                          const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra);
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
-                             const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                             const double *W, const Faces &fc, const double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra);
 
 }  // namespace tl

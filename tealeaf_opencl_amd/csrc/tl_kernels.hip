@@ -323,6 +323,20 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_common(Geom g, Ext e, co
     TL_TILE_LOOP_END
 }
 
+// conductivity of tea_leaf_init_common (tea_leaf_common_cl.cl:56-90: density,
+// or its reciprocal) over the whole padded index space, kept for the fused
+// iteration kernels, which recompute Kx, Ky from it
+__global__ void __launch_bounds__(256) k_conductivity(Geom g, const double *rho, double *w, int coef)
+{
+    const int nx = g.x_max + 2 * g.hd, ny = g.y_max + 2 * g.hd;
+    const long n = (long)nx * ny;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % nx), k = (int)(t / nx);
+        const double d = rho[IX(g, j, k)];
+        w[IX(g, j, k)] = (coef == COEF_CONDUCTIVITY) ? d : 1.0 / d;
+    }
+}
+
 // tea_leaf_zero_boundary (tea_leaf_common_cl.cl:92-125), launch bounds (-hd, hd)
 __global__ void __launch_bounds__(BLOCK_THREADS) k_zero_boundary(Geom g, Ext e, double *Kx, double *Ky,
                                                                  int zl, int zr, int zb, int zt)
@@ -791,8 +805,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const d
 // Stores p_new only; w is recomputed by the second half.
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, const double *src,
                                                               double *pA, double *pB,
-                                                              const double *Kx, const double *Ky,
-                                                              RedArgs ra)
+                                                              const double *W, Faces fc, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
@@ -805,7 +818,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col6(src, i0, P, sc);
     col6(p_old, i0, P, pc);
     col4(src, i0 - 1, P, sw);
@@ -840,8 +853,8 @@ template <int PREC, int PPCG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, double *u,
                                                                const double *pA, const double *pB,
                                                                double *r, double *z,
-                                                               double *sd, const double *Kx,
-                                                               const double *Ky, RedArgs ra)
+                                                               double *sd, const double *W, Faces fc,
+                                                               RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
@@ -852,7 +865,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4(u, i0, P, uv);
     col4(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -885,8 +898,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
 // (tea_leaf_ppcg.f90:41-60 via calc_2norm(2)); its finisher runs the tail.
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
-    Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *Kx,
-    const double *Ky, const double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
+    Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *W, Faces fc,
+    const double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
@@ -896,7 +909,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP],
         zo[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4(u, i0, P, uv);
     col4(r, i0, P, rv);
     if (LAST && PREC != PREC_NONE) col4(z, i0, P, zo);
@@ -996,6 +1009,11 @@ void launch_init_common(hipStream_t s, const Geom &g, const Ext &e, int cap, con
                         double *Ky, double rx, double ry, int coef)
 {
     TL_LAUNCH(k_init_common, e, cap, s, g, e, rho, Kx, Ky, rx, ry, coef);
+}
+
+void launch_conductivity(hipStream_t s, const Geom &g, const double *rho, double *w, int coef)
+{
+    hipLaunchKernelGGL(k_conductivity, dim3(1024), dim3(256), 0, s, g, rho, w, coef);
 }
 
 void launch_zero_boundary(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Kx, double *Ky,
@@ -1146,10 +1164,10 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
 }
 
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
-                       double *pA, double *pB, const double *Kx, const double *Ky, const RedArgs &ra,
+                       double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
                        hipEvent_t ev_main)
 {
-    TL_LAUNCH_RED_EV(k_fused_cg_w, e, cap, s, 1, ra, ev_main, g, e, src, pA, pB, Kx, Ky, ra);
+    TL_LAUNCH_RED_EV(k_fused_cg_w, e, cap, s, 1, ra, ev_main, g, e, src, pA, pB, W, fc, ra);
 }
 
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
@@ -1160,13 +1178,13 @@ void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, co
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, double *r, double *z, double *sd,
-                        const double *Kx, const double *Ky, const RedArgs &ra, hipEvent_t ev_main)
+                        const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main)
 {
 #define TL_FUR(P, M)                                                                                          \
     do {                                                                                                      \
-        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, Kx, Ky, ra);             \
-        else TL_LAUNCH_RED_EV((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, ev_main, g, e, u, pA, pB, r, z, sd, Kx, \
-                              Ky, ra);                                                                        \
+        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, W, fc, ra);              \
+        else TL_LAUNCH_RED_EV((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, ev_main, g, e, u, pA, pB, r, z, sd, W,  \
+                              fc, ra);                                                                        \
     } while (0)
     if (ppcg) {
         if (prec == PREC_BLOCK) TL_FUR(PREC_BLOCK, 1);
@@ -1182,14 +1200,14 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
 
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
-                             const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                             const double *W, const Faces &fc, const double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra)
 {
 #define TL_FPI(P, L)                                                                                         \
     do {                                                                                                     \
-        if (L) TL_LAUNCH_RED((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, g, e, u, r, sd_in, sd_out, Kx, Ky, z, \
+        if (L) TL_LAUNCH_RED((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, g, e, u, r, sd_in, sd_out, W, fc, z, \
                              ch_a, ch_b, step, ra);                                                          \
-        else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, Kx, Ky, z, ch_a, ch_b, \
+        else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, W, fc, z, ch_a, ch_b,  \
                        step, ra);                                                                            \
     } while (0)
     if (last) {

@@ -1,6 +1,7 @@
 // Refined solve of A u = u0 for one implicit timestep (after common_init has
-// formed Kx, Ky and u = u0): a preconditioned CG whose vectors (r, z, p, the
-// solution increment d) and stencil coefficients are stored in FP32 or FP64,
+// formed Kx, Ky, the conductivity and u = u0): a preconditioned CG whose
+// vectors (r, z, p, the solution increment d) and conductivity (from which
+// every kernel forms the face coefficients) are stored in FP32 or FP64,
 // with every arithmetic operation, every dot product and every scalar in FP64,
 // wrapped in FP64 reliable updates: whenever the recursive residual has
 // dropped by `delta` since the last update, or has reached the tolerance, the
@@ -37,7 +38,7 @@ namespace tl {
 
 namespace {
 
-enum { RW_KX = 0, RW_KY, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_N };
+enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_N };
 
 struct RfArgs {
     double *partials;
@@ -109,29 +110,29 @@ __device__ __forceinline__ double rnd(double v)
     return (double)(T)v;
 }
 
-// Kx, Ky over the whole padded index space, rounded to the inner storage
-__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *Kx, const double *Ky,
-                                                   float *kx, float *ky)
+// the conductivity over the whole padded index space, rounded to the inner
+// storage (the inner operator's face coefficients are formed from it)
+__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *w8, float *w4)
 {
     const int w = g8.x_max + 2 * g8.hd, h = g8.y_max + 2 * g8.hd;
     const long n = (long)w * h;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const int j = (int)(t % w), k = (int)(t / w);
-        kx[IX(g4, j, k)] = (float)Kx[IX(g8, j, k)];
-        ky[IX(g4, j, k)] = (float)Ky[IX(g8, j, k)];
+        w4[IX(g4, j, k)] = (float)w8[IX(g8, j, k)];
     }
 }
 
 // Reliable update: u_out = u_in + d (FP64), r = u0 - A u_out with the FP64
-// coefficients, then r and z = M^-1 r rounded to the inner storage (M built
-// from the coefficients rounded the same way, i.e. the inner operator's
-// preconditioner). Sums: r.z of the stored values, and the FP64 r.r. d is
-// read at stencil neighbours, so it is cleared after the kernel, not in it.
+// face coefficients (bitwise those of tea_leaf_init_common), then r rounded
+// to the inner storage and z = M^-1 r with the inner operator's
+// preconditioner (faces formed from the stored inner conductivity). Sums:
+// r.z of the stored values, and the FP64 r.r. d is read at stencil
+// neighbours, so it is cleared after the kernel, not in it.
 template <typename T, int PREC>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, Ext e, const double *u_in,
                                                              double *u_out, const T *d, const double *u0,
-                                                             const double *Kx, const double *Ky, T *r, T *z,
-                                                             RfArgs ra)
+                                                             const double *cond8, const T *cond_in, Faces fc,
+                                                             T *r, T *z, RfArgs ra)
 {
     double acc[2] = {0.0, 0.0};
     TL_TILE_LOOP(e)
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
     const int i4 = IX(g4, j, ks), P4 = g4.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     double uc[STRIP + 2], uw[STRIP], ue[STRIP], dc[STRIP + 2], dw[STRIP], de[STRIP], b[STRIP];
-    strip_k(Kx, Ky, i8, P8, kx0, kx1, ky);
+    strip_k_cond(g8, fc, cond8, i8, P8, j, ks, kx0, kx1, ky);
     col6(u_in, i8, P8, uc);
     col4(u_in, i8 - 1, P8, uw);
     col4(u_in, i8 + 1, P8, ue);
@@ -161,14 +162,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
     mask_rows(ks, e.k1, rv);
     double kx0t[STRIP], kx1t[STRIP], kyt[STRIP + 1];
 #pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        rt[q] = rnd<T>(rv[q]);
-        kx0t[q] = rnd<T>(kx0[q]);
-        kx1t[q] = rnd<T>(kx1[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < STRIP + 1; ++q) kyt[q] = rnd<T>(ky[q]);
-    strip_precondition<PREC>(g8, ks, i8, kx0t, kx1t, kyt, rt, zv);
+    for (int q = 0; q < STRIP; ++q) rt[q] = rnd<T>(rv[q]);
+    strip_k_cond(g4, fc, cond_in, i4, P4, j, ks, kx0t, kx1t, kyt);
+    strip_precondition<PREC>(g4, ks, i4, kx0t, kx1t, kyt, rt, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
@@ -187,7 +183,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
 // double-buffered by the parity of the iteration counter.
 template <typename T>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *src, T *pA, T *pB,
-                                                        const T *Kx, const T *Ky, RfArgs ra)
+                                                        const T *W, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -200,7 +196,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col6(src, i0, P, sc);
     col6(p_old, i0, P, pc);
     col4(src, i0 - 1, P, sw);
@@ -232,7 +228,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
 // coefficients; r.z and r.r of the stored values.
 template <typename T, int PREC>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
-                                                         T *z, const T *Kx, const T *Ky, RfArgs ra)
+                                                         T *z, const T *W, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -242,7 +238,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4(d, i0, P, dv);
     col4(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -283,7 +279,7 @@ void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, dou
     T *const *f = reinterpret_cast<T *const *>(w.f);
 #define TL_RFU(P)                                                                                           \
     hipLaunchKernelGGL((k_rf_update<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, u_out, \
-                       f[RW_D], c->f2[F_U0], c->f2[F_KX], c->f2[F_KY], f[RW_R], f[RW_Z], ra)
+                       f[RW_D], c->f2[F_U0], c->f2[F_COND], f[RW_W], c->faces, f[RW_R], f[RW_Z], ra)
     if (c->prec == PREC_BLOCK) TL_RFU(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFU(PREC_DIAG);
     else TL_RFU(PREC_NONE);
@@ -302,11 +298,11 @@ void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w)
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW}, ra_u{c->partials, w.st, RF_RRN};
     hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
-                       f[RW_KX], f[RW_KY], ra_w);
+                       f[RW_W], c->faces, ra_w);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
 #define TL_RFI(P)                                                                                       \
     hipLaunchKernelGGL((k_rf_ur<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
-                       f[RW_P2], f[RW_R], f[RW_Z], f[RW_KX], f[RW_KY], ra_u)
+                       f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], c->faces, ra_u)
     if (c->prec == PREC_BLOCK) TL_RFI(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
     else TL_RFI(PREC_NONE);
@@ -347,14 +343,13 @@ int rf_alloc_impl(tl_chunk *c, int which)
             w.f[i] = static_cast<float *>(w.base[i]) + w.off;
         }
     } else {
-        // FP64: the chunk's own r, z, p fields and coefficients, plus d
+        // FP64: the chunk's own r, z, p fields and conductivity, plus d
         w.off = c->off;
         w.elems = c->elems2d;
         if ((e = hipMalloc(&w.base[RW_D], sizeof(double) * w.elems)) != hipSuccess) return hip_fail("hipMalloc refine", e);
         hipMemsetAsync(w.base[RW_D], 0, sizeof(double) * w.elems, c->s);
         w.f[RW_D] = static_cast<double *>(w.base[RW_D]) + w.off;
-        w.f[RW_KX] = c->f2[F_KX];
-        w.f[RW_KY] = c->f2[F_KY];
+        w.f[RW_W] = c->f2[F_COND];
         w.f[RW_R] = c->f2[F_R];
         w.f[RW_Z] = c->f2[F_Z];
         w.f[RW_P] = c->f2[F_P];
@@ -467,8 +462,8 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
     hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
     hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
     if (which == 1)
-        hipLaunchKernelGGL(k_rf_narrow, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_KX], c->f2[F_KY],
-                           static_cast<float *>(w.f[RW_KX]), static_cast<float *>(w.f[RW_KY]));
+        hipLaunchKernelGGL(k_rf_narrow, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_COND],
+                           static_cast<float *>(w.f[RW_W]));
     double *ubuf[2] = {c->f2[F_U], c->f2[F_U2]};
     int cur = 0;
     launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);
