@@ -32,6 +32,12 @@ ALG_BYTES = {
     # u += a p; r -= a A p; z = M_blk^-1 r; r.z : R u, p, r, w  W u, r, z
     "k_fused_cg_ur": 7 * 8,
 }
+# the refined solve's inner kernels (tl_refine.hip): same streams as the fused
+# FP64 kernels, with r, z, p, d and the face field stored in FP64 or FP32
+ALG_BYTES_RF = {
+    "fp64": {"k_rf_w": 4 * 8, "k_rf_ur": 7 * 8},
+    "mixed": {"k_rf_w": 4 * 4, "k_rf_ur": 7 * 4},
+}
 REF_BYTES_PER_ITER = 144  # reference CG + strip-4 block Jacobi (calc_w + calc_ur + calc_p)
 
 
@@ -92,6 +98,19 @@ def refine_leg(n, tol=1e-10):
         finally:
             c.close()
     rel = np.abs(summaries["mixed"] - summaries["fp64"]) / np.abs(summaries["fp64"])
+    cells = n * n
+    kernels = {}
+    for name in ("fp64", "mixed"):
+        r = out[name]
+        kb = ALG_BYTES_RF[name]
+        kernels[name] = {
+            "k_rf_w": {"avg_ms": r["k1_ms"], "alg_bytes_per_launch": kb["k_rf_w"] * cells,
+                       "achieved_GBs": kb["k_rf_w"] * cells / (r["k1_ms"] * 1e-3) / 1e9 if r["k1_ms"] > 0 else None},
+            "k_rf_ur": {"avg_ms": r["k2_ms"], "alg_bytes_per_launch": kb["k_rf_ur"] * cells,
+                        "achieved_GBs": kb["k_rf_ur"] * cells / (r["k2_ms"] * 1e-3) / 1e9 if r["k2_ms"] > 0 else None},
+            "samples": r["k_samples"],
+            "s_per_iteration": r["seconds"] / max(r["iterations"], 1),
+        }
     return {
         "workload": "%dx%d, 2 rectangles (seed 1), conductivity 1/density, CG + strip-4 block Jacobi, "
                     "true ||u0-Au||/||u0-Au_init|| <= %g" % (n, n, tol),
@@ -102,6 +121,7 @@ def refine_leg(n, tol=1e-10):
         "speedup": out["fp64"]["seconds"] / out["mixed"]["seconds"],
         "summary_rel_diff_mixed_vs_fp64": dict(zip(["volume", "mass", "internal_energy", "temperature"],
                                                   rel.tolist())),
+        "kernels": kernels,
     }
 
 

@@ -198,9 +198,9 @@ int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, 
  * chunk's preconditioner. No reference counterpart: the reference solves in
  * FP64 only (tea_solve.f90:37-420).
  *   precision TL_PRECISION_FP64: inner CG with FP64 vectors;
- *             TL_PRECISION_MIXED: inner CG with FP32 vectors and an FP32
- *             copy of the conductivity (face coefficients formed from it in
- *             FP64), FP64 arithmetic, dot products and scalars.
+ *             TL_PRECISION_MIXED: inner CG with FP32 vectors and the
+ *             conductivity held as 1/(2w) in FP32 (face coefficients formed
+ *             from it in FP64), FP64 arithmetic, dot products and scalars.
  * Either way the increment is folded into the FP64 solution and the residual
  * is replaced by the true FP64 residual u0 - A u whenever the recursive
  * residual norm has fallen below delta times its largest value since the last
@@ -216,10 +216,13 @@ typedef struct {
     double seconds;           /* device time of the whole solve, setup included */
     double initial_residual;  /* ||u0 - A u_init||_2 */
     double rel_residual;      /* final true ||u0 - A u||_2 / initial_residual */
+    double k1_ms, k2_ms;      /* average durations of the two inner iteration
+                                 kernels (HIP events, first iteration of each
+                                 batch), -1 when none was sampled */
     int iterations;           /* inner iterations */
     int updates;              /* true residuals formed, the initial one included */
     int converged;
-    int pad_;
+    int k_samples;
 } tl_refine_result;
 int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_iters, double delta,
                      tl_refine_result *out);

@@ -53,8 +53,9 @@ class BenchResult(ctypes.Structure):
 
 class RefineResult(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("initial_residual", ctypes.c_double),
-                ("rel_residual", ctypes.c_double), ("iterations", ctypes.c_int), ("updates", ctypes.c_int),
-                ("converged", ctypes.c_int), ("pad_", ctypes.c_int)]
+                ("rel_residual", ctypes.c_double), ("k1_ms", ctypes.c_double), ("k2_ms", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("updates", ctypes.c_int), ("converged", ctypes.c_int),
+                ("k_samples", ctypes.c_int)]
 
 
 _LIB = None
@@ -316,7 +317,8 @@ class Chunk:
         _check(self.lib.tl_solve_refined(self.h, int(precision), float(tol), int(max_iters), float(delta),
                                          ctypes.byref(r)), "solve_refined")
         return {"seconds": r.seconds, "initial_residual": r.initial_residual, "rel_residual": r.rel_residual,
-                "iterations": r.iterations, "updates": r.updates, "converged": bool(r.converged)}
+                "iterations": r.iterations, "updates": r.updates, "converged": bool(r.converged),
+                "k1_ms": r.k1_ms, "k2_ms": r.k2_ms, "k_samples": r.k_samples}
 
 
 def parse_log(path):

@@ -1,8 +1,9 @@
 // Refined solve of A u = u0 for one implicit timestep (after common_init has
 // formed Kx, Ky, the conductivity and u = u0): a preconditioned CG whose
-// vectors (r, z, p, the solution increment d) and conductivity (from which
-// every kernel forms the face coefficients) are stored in FP32 or FP64,
-// with every arithmetic operation, every dot product and every scalar in FP64,
+// vectors (r, z, p, the solution increment d) and the per-cell field the
+// kernels form face coefficients from (the conductivity, or 1/(2w) in FP32)
+// are stored in FP32 or FP64, with every arithmetic operation, every dot
+// product and every scalar in FP64,
 // wrapped in FP64 reliable updates: whenever the recursive residual has
 // dropped by `delta` since the last update, or has reached the tolerance, the
 // increment is added into the FP64 solution (u += d, d = 0) and the residual
@@ -36,7 +37,9 @@
 
 namespace tl {
 
-namespace {
+// kernels and helpers of the refined solve (a named namespace: kernels with
+// internal linkage are not visible to the profiling tools)
+namespace rf {
 
 enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_N };
 
@@ -110,22 +113,31 @@ __device__ __forceinline__ double rnd(double v)
     return (double)(T)v;
 }
 
-// the conductivity over the whole padded index space, rounded to the inner
-// storage (the inner operator's face coefficients are formed from it)
-__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *w8, float *w4)
+// Field from which the inner kernels form their face coefficients: the FP64
+// conductivity itself for FP64 inner fields (faces bitwise those of
+// tea_leaf_init_common), 1/(2w) rounded to FP32 for FP32 inner fields (faces
+// without a division).
+template <typename T>
+constexpr bool inner_half_inv()
+{
+    return sizeof(T) == 4;
+}
+
+// 1/(2w) over the whole padded index space, rounded to FP32
+__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *w8, float *h4)
 {
     const int w = g8.x_max + 2 * g8.hd, h = g8.y_max + 2 * g8.hd;
     const long n = (long)w * h;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const int j = (int)(t % w), k = (int)(t / w);
-        w4[IX(g4, j, k)] = (float)w8[IX(g8, j, k)];
+        h4[IX(g4, j, k)] = (float)(0.5 / w8[IX(g8, j, k)]);
     }
 }
 
 // Reliable update: u_out = u_in + d (FP64), r = u0 - A u_out with the FP64
 // face coefficients (bitwise those of tea_leaf_init_common), then r rounded
 // to the inner storage and z = M^-1 r with the inner operator's
-// preconditioner (faces formed from the stored inner conductivity). Sums:
+// preconditioner (faces formed like the inner kernels form them). Sums:
 // r.z of the stored values, and the FP64 r.r. d is read at stencil
 // neighbours, so it is cleared after the kernel, not in it.
 template <typename T, int PREC>
@@ -163,7 +175,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
     double kx0t[STRIP], kx1t[STRIP], kyt[STRIP + 1];
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) rt[q] = rnd<T>(rv[q]);
-    strip_k_cond(g4, fc, cond_in, i4, P4, j, ks, kx0t, kx1t, kyt);
+    strip_k_cond<inner_half_inv<T>()>(g4, fc, cond_in, i4, P4, j, ks, kx0t, kx1t, kyt);
     strip_precondition<PREC>(g4, ks, i4, kx0t, kx1t, kyt, rt, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
@@ -196,7 +208,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
-    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col6(src, i0, P, sc);
     col6(p_old, i0, P, pc);
     col4(src, i0 - 1, P, sw);
@@ -238,7 +250,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
-    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4(d, i0, P, dv);
     col4(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -289,17 +301,22 @@ void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, dou
     hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
 }
 
+// One inner iteration; with `ev`, ev[0..1] bracket k_rf_w and ev[2..3]
+// k_rf_ur (their finishers outside the brackets).
 template <typename T>
-void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w)
+void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nullptr)
 {
     const Ext e = ext_interior(c);
     const int grid = grid_of(e, c->cap);
     T *const *f = reinterpret_cast<T *const *>(w.f);
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW}, ra_u{c->partials, w.st, RF_RRN};
+    if (ev) hipEventRecord(ev[0], c->s);
     hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
                        f[RW_W], c->faces, ra_w);
+    if (ev) hipEventRecord(ev[1], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
+    if (ev) hipEventRecord(ev[2], c->s);
 #define TL_RFI(P)                                                                                       \
     hipLaunchKernelGGL((k_rf_ur<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
                        f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], c->faces, ra_u)
@@ -307,6 +324,7 @@ void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w)
     else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
     else TL_RFI(PREC_NONE);
 #undef TL_RFI
+    if (ev) hipEventRecord(ev[3], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
 }
 
@@ -390,7 +408,8 @@ int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
     }
     hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
-    for (int b = 0; b < RF_BATCH; ++b) launch_iteration<T>(c, w);
+    // the first iteration of each batch is launched eagerly (timed)
+    for (int b = 0; b < RF_BATCH - 1; ++b) launch_iteration<T>(c, w);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(c->s, &graph);
     if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
@@ -404,12 +423,22 @@ int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
     return TL_OK;
 }
 
+struct KernelTimes {
+    double k1_ms = 0, k2_ms = 0;
+    int samples = 0;
+};
+
 // Replays the iteration graph until the device state halts; batch i+1 is
-// queued before the state after batch i is examined.
-int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches)
+// queued before the state after batch i is examined. Each batch starts with
+// one eagerly launched iteration bracketed by events; the events of a batch
+// are read once it has finished (every examined batch follows one that did
+// not halt, so its eager iteration ran).
+template <typename T>
+int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches, KernelTimes *kt)
 {
     hipError_t e;
     auto enqueue = [&](int sl) -> hipError_t {
+        launch_iteration<T>(c, w, c->ev_t[sl]);
         hipError_t ee = hipGraphLaunch(w.graph.exec, c->s);
         if (ee != hipSuccess) return ee;
         ee = hipMemcpyAsync(&w.h_st[sl], w.st, sizeof(RefineState), hipMemcpyDeviceToHost, c->s);
@@ -421,6 +450,13 @@ int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches)
     for (long guard = 0;; ++guard) {
         if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
         if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
+        float a = 0.f, b = 0.f;
+        if (hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
+            hipEventElapsedTime(&b, c->ev_t[slot][2], c->ev_t[slot][3]) == hipSuccess) {
+            kt->k1_ms += a;
+            kt->k2_ms += b;
+            kt->samples += 1;
+        }
         if (w.h_st[slot].halt) break;
         if (guard > max_batches) {
             hipStreamSynchronize(c->s);
@@ -430,6 +466,7 @@ int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches)
         slot = 1 - slot;
     }
     if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("batch drain", e);
+    (void)hipGetLastError();
     return TL_OK;
 }
 
@@ -471,8 +508,9 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine update", e);
     if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
     const long max_batches = max_iters / RF_BATCH + 4;
+    KernelTimes kt;
     while (!h.converged && h.n < max_iters && isfinite(h.rr)) {
-        if ((rc = rf_batches(c, w, max_batches)) != TL_OK) return rc;
+        if ((rc = rf_batches<T>(c, w, max_batches, &kt)) != TL_OK) return rc;
         if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
         if (!h.want_update) break;
         launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);
@@ -493,6 +531,9 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
         out->iterations = h.n;
         out->updates = h.updates;
         out->converged = h.converged;
+        out->k_samples = kt.samples;
+        out->k1_ms = kt.samples ? kt.k1_ms / kt.samples : -1.0;
+        out->k2_ms = kt.samples ? kt.k2_ms / kt.samples : -1.0;
     }
     if (!h.converged && !(h.n >= max_iters)) {
         set_error("refined solve broke down after %d iterations (residual %g)", h.n, sqrt(h.rr));
@@ -501,11 +542,11 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
     return TL_OK;
 }
 
-}  // namespace
+}  // namespace rf
 
 void refine_release(tl_chunk *c)
 {
-    for (auto &w : c->rf) release_work(w);
+    for (auto &w : c->rf) rf::release_work(w);
 }
 
 }  // namespace tl
@@ -525,6 +566,6 @@ extern "C" int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_
                   max_iters, delta);
         return TL_ERR_ARG;
     }
-    if (precision == TL_PRECISION_MIXED) return rf_solve<float>(c, 1, tol, max_iters, delta, out);
-    return rf_solve<double>(c, 0, tol, max_iters, delta, out);
+    if (precision == TL_PRECISION_MIXED) return rf::rf_solve<float>(c, 1, tol, max_iters, delta, out);
+    return rf::rf_solve<double>(c, 0, tol, max_iters, delta, out);
 }
