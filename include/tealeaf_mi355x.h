@@ -206,8 +206,10 @@ int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, 
  * residual norm has fallen below delta times its largest value since the last
  * replacement (0 < delta < 1; 0 replaces it only at the tolerance), and the
  * solve ends when ||u0 - A u||_2 <= tol * ||u0 - A u_init||_2 holds for the
- * true residual, or after max_iters inner iterations (converged = 0, TL_OK).
- * The result is left in field TL_F_U. */
+ * true residual, after max_iters inner iterations, or when three replacements
+ * in a row have not halved the true residual (it has reached its FP64
+ * rounding floor above tol: stalled = 1); the last two return TL_OK with
+ * converged = 0. The result is left in field TL_F_U. */
 enum {
     TL_PRECISION_FP64 = 0,
     TL_PRECISION_MIXED = 1
@@ -222,10 +224,50 @@ typedef struct {
     int iterations;           /* inner iterations */
     int updates;              /* true residuals formed, the initial one included */
     int converged;
+    int stalled;              /* stopped at the rounding floor above tol */
     int k_samples;
+    int pad_;
 } tl_refine_result;
 int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_iters, double delta,
                      tl_refine_result *out);
+
+/* Adjoint of the implicit timesteps with respect to the per-cell density. No
+ * reference counterpart (the reference has no derivatives); the forward model
+ * is its timestep with zero-flux walls: from the chunk's energy1 e^0 and
+ * density rho, nsteps times { tl_common_init(coefficient, dt), solve
+ * A(rho) u = u0 (tl_solve_refined, FP64, true relative residual tol),
+ * tl_finalise }, and the loss is L = sum over the interior of (u - u_obs)^2
+ * after the last step.
+ *
+ * tl_forward_loss runs that forward model and returns L.
+ * tl_adjoint_density runs it keeping u every checkpoint_every steps, then the
+ * backward pass: each segment is recomputed from its checkpoint, the adjoint
+ * system A lambda = dL/du is solved with the same solver, dL/dK at the faces
+ * is formed from -(lambda_a - lambda_b)(u_a - u_b) and chained through the
+ * harmonic-mean face coefficient to rho, plus the term of u^0 = rho e^0.
+ * A solve that stalls at its rounding floor above tol is accepted (counted in
+ * stalled_solves); one that exhausts max_iters is TL_ERR_SOLVER.
+ * u_obs and grad are host arrays in the layout of tl_field_get (halo cells
+ * of grad are zero). On return energy1 holds the final forward energy;
+ * u, u0, p, r, z are overwritten. */
+typedef struct {
+    double loss;
+    double forward_s;           /* device time of the forward timesteps */
+    double adjoint_s;           /* loss, recompute, adjoint solves, gradient kernels */
+    double work_bytes;          /* device memory of checkpoints, recompute buffers
+                                   and adjoint vectors */
+    double chunk_bytes;         /* device memory of the chunk's own fields */
+    long forward_iterations;
+    long recompute_iterations;
+    long adjoint_iterations;
+    int checkpoints;
+    int stalled_solves;         /* solves stopped at their FP64 rounding floor
+                                   above tol (accepted) */
+} tl_adjoint_result;
+int tl_forward_loss(tl_chunk *c, int coefficient, double dt, int nsteps, double tol, int max_iters,
+                    const double *u_obs, double *loss);
+int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int nsteps, double tol, int max_iters,
+                       int checkpoint_every, const double *u_obs, double *grad, tl_adjoint_result *out);
 
 #ifdef __cplusplus
 }

@@ -55,7 +55,15 @@ class RefineResult(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("initial_residual", ctypes.c_double),
                 ("rel_residual", ctypes.c_double), ("k1_ms", ctypes.c_double), ("k2_ms", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("updates", ctypes.c_int), ("converged", ctypes.c_int),
-                ("k_samples", ctypes.c_int)]
+                ("stalled", ctypes.c_int), ("k_samples", ctypes.c_int), ("pad_", ctypes.c_int)]
+
+
+class AdjointResult(ctypes.Structure):
+    _fields_ = [("loss", ctypes.c_double), ("forward_s", ctypes.c_double), ("adjoint_s", ctypes.c_double),
+                ("work_bytes", ctypes.c_double), ("chunk_bytes", ctypes.c_double),
+                ("forward_iterations", ctypes.c_long), ("recompute_iterations", ctypes.c_long),
+                ("adjoint_iterations", ctypes.c_long), ("checkpoints", ctypes.c_int),
+                ("stalled_solves", ctypes.c_int)]
 
 
 _LIB = None
@@ -102,6 +110,8 @@ _SIGS = {
     "tl_run_deck": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
     "tl_bench_deck": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(BenchResult)]),
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
+    "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
+    "tl_adjoint_density": (_I, [_P, _I, _D, _I, _D, _I, _I, _DP, _DP, ctypes.POINTER(AdjointResult)]),
 }
 
 
@@ -307,6 +317,31 @@ class Chunk:
     def synchronize(self):
         _check(self.lib.tl_synchronize(self.h), "synchronize")
 
+    def forward_loss(self, coefficient, dt, nsteps, u_obs, tol=1e-12, max_iters=20000):
+        """L = sum over the interior of (u - u_obs)^2 after `nsteps` timesteps
+        from the current energy1 (tl_forward_loss)."""
+        uo = np.ascontiguousarray(u_obs, dtype=np.float64)
+        if uo.shape != self.shape:
+            raise ValueError("u_obs must have shape %s" % (self.shape,))
+        v = _D()
+        _check(self.lib.tl_forward_loss(self.h, int(coefficient), float(dt), int(nsteps), float(tol),
+                                        int(max_iters), _dptr(uo), ctypes.byref(v)), "forward_loss")
+        return v.value
+
+    def adjoint_density(self, coefficient, dt, nsteps, u_obs, tol=1e-12, max_iters=20000, checkpoint_every=5):
+        """(L, dL/d density, info) by the adjoint of `nsteps` timesteps
+        (tl_adjoint_density); the gradient has the layout of get_field."""
+        uo = np.ascontiguousarray(u_obs, dtype=np.float64)
+        if uo.shape != self.shape:
+            raise ValueError("u_obs must have shape %s" % (self.shape,))
+        grad = np.zeros(self.shape)
+        r = AdjointResult()
+        _check(self.lib.tl_adjoint_density(self.h, int(coefficient), float(dt), int(nsteps), float(tol),
+                                           int(max_iters), int(checkpoint_every), _dptr(uo), _dptr(grad),
+                                           ctypes.byref(r)), "adjoint_density")
+        info = {k: getattr(r, k) for k, _ in AdjointResult._fields_}
+        return r.loss, grad, info
+
     def solve_refined(self, precision=PRECISION_MIXED, tol=1e-10, max_iters=10000, delta=None):
         """Solve A u = u0 after common_init to a true relative residual `tol`
         (FP32 or FP64 inner CG with FP64 residual replacement; see
@@ -318,6 +353,7 @@ class Chunk:
                                          ctypes.byref(r)), "solve_refined")
         return {"seconds": r.seconds, "initial_residual": r.initial_residual, "rel_residual": r.rel_residual,
                 "iterations": r.iterations, "updates": r.updates, "converged": bool(r.converged),
+                "stalled": bool(r.stalled),
                 "k1_ms": r.k1_ms, "k2_ms": r.k2_ms, "k_samples": r.k_samples}
 
 

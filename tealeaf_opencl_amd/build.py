@@ -12,7 +12,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libtealeaf_mi355x.so"
 ARCH = os.environ.get("TL_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["tl_kernels.hip", "tl_refine.hip", "tl_chunk.hip", "tl_driver.hip", "tl_deck.cpp"]
+SOURCES = ["tl_kernels.hip", "tl_refine.hip", "tl_adjoint.hip", "tl_chunk.hip", "tl_driver.hip", "tl_deck.cpp"]
 HEADERS = ["tl_common.h", "tl_device.h", "tl_kernels.h", "tl_chunk.h", "tl_deck.h"]
 
 # packed FP32 instructions are disabled on every device compile

@@ -139,7 +139,8 @@ struct RefineState {
     int want_update;      // 1: a reliable update is due
     int converged;
     int updates;          // true residuals formed (the initial one included)
-    int pad1_;
+    int stalls;           // consecutive updates whose true residual fell by less than half
+    double rr_prev;       // ||r||^2 of the previous true residual
 };
 
 enum RefineMode {

@@ -9,7 +9,9 @@
 // increment is added into the FP64 solution (u += d, d = 0) and the residual
 // is replaced by the true FP64 residual u0 - A u (residual replacement,
 // van der Vorst & Ye, SIAM J. Sci. Comput. 22(5), 2000). The search direction
-// is kept across updates, so the Krylov recurrence is not restarted.
+// is kept across updates while the recursive residual still tracks the true
+// one, so the Krylov recurrence is not restarted; it is restarted from the
+// true residual when the two have drifted apart by more than a factor 2.
 // Convergence is declared on the true FP64 residual only:
 // ||u0 - A u||_2 <= tol * ||u0 - A u_init||_2.
 //
@@ -49,6 +51,9 @@ struct RfArgs {
     int mode;
 };
 
+// replacements in a row without progress after which the solve stops
+constexpr int STALL_LIMIT = 3;
+
 __device__ void rf_scalars(RefineState *s, int mode, const double *tot)
 {
     const double rz = tot[0], rr = tot[1];
@@ -57,9 +62,17 @@ __device__ void rf_scalars(RefineState *s, int mode, const double *tot)
         if (s->updates == 0) {
             s->rr0 = rr;
             s->beta = 0.0;
+            s->stalls = 0;
         } else {
-            s->beta = rz / s->rz_prev;
+            // keep the search direction while the recursive residual tracked
+            // the true one (within a factor 2 in norm); otherwise restart the
+            // recurrence from the true residual
+            s->beta = (rr > 4.0 * s->rr) ? 0.0 : rz / s->rz_prev;
+            // the true residual has reached the rounding floor of the FP64
+            // residual when replacement after replacement no longer halves it
+            s->stalls = (rr > 0.25 * s->rr_prev) ? s->stalls + 1 : 0;
         }
+        s->rr_prev = rr;
         s->rro = rz;
         s->rr_true = rr;
         s->rr = rr;
@@ -69,7 +82,7 @@ __device__ void rf_scalars(RefineState *s, int mode, const double *tot)
         if (sqrt(rr) <= s->tol * sqrt(s->rr0)) {
             s->converged = 1;
             s->halt = 1;
-        } else if (s->n >= s->max_iters || !isfinite(rr) || !isfinite(rz)) {
+        } else if (s->n >= s->max_iters || !isfinite(rr) || !isfinite(rz) || s->stalls >= STALL_LIMIT) {
             s->halt = 1;
         } else {
             s->halt = 0;
@@ -509,7 +522,7 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
     if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
     const long max_batches = max_iters / RF_BATCH + 4;
     KernelTimes kt;
-    while (!h.converged && h.n < max_iters && isfinite(h.rr)) {
+    while (!h.converged && h.n < max_iters && isfinite(h.rr) && h.stalls < STALL_LIMIT) {
         if ((rc = rf_batches<T>(c, w, max_batches, &kt)) != TL_OK) return rc;
         if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
         if (!h.want_update) break;
@@ -531,11 +544,12 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
         out->iterations = h.n;
         out->updates = h.updates;
         out->converged = h.converged;
+        out->stalled = !h.converged && h.stalls >= STALL_LIMIT;
         out->k_samples = kt.samples;
         out->k1_ms = kt.samples ? kt.k1_ms / kt.samples : -1.0;
         out->k2_ms = kt.samples ? kt.k2_ms / kt.samples : -1.0;
     }
-    if (!h.converged && !(h.n >= max_iters)) {
+    if (!h.converged && !(h.n >= max_iters) && h.stalls < STALL_LIMIT) {
         set_error("refined solve broke down after %d iterations (residual %g)", h.n, sqrt(h.rr));
         return TL_ERR_SOLVER;
     }

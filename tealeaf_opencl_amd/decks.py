@@ -167,3 +167,32 @@ def config3_deck(path, precision="mixed", preconditioner="jac_block", seed=1, n=
     return write_deck(path, n, n, seeded_states(seed, 2, False), solver="tl_use_cg",
                       preconditioner=preconditioner, end_step=steps, eps=eps,
                       coefficient="tl_coefficient_inverse_density", extra=ex, **kw)
+
+
+def _splitmix_uniform(seed, n):
+    """n draws of the splitmix64 -> U[0,1) stream of SplitMix64(seed), vectorised."""
+    k = np.arange(1, n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & MASK) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def config4_density(n, seed=1, hd=1, sigma=1.0):
+    """Per-cell density of the adjoint workload on [0,10]^2 (layout of
+    Chunk.get_field, halo cells zero): rho = 10^U[-1,2] * (1 + 0.5 exp(-|x - x0|^2
+    / (2 sigma^2))) at the cell centres, x0 uniform in [2,8]^2, all draws from
+    splitmix64(seed) (x0 first, then the cells row by row)."""
+    rng = SplitMix64(seed)
+    x0 = 2.0 + 6.0 * rng.uniform()
+    y0 = 2.0 + 6.0 * rng.uniform()
+    u = _splitmix_uniform(rng.state, n * n).reshape(n, n)
+    dx = 10.0 / n
+    c = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(c, c)
+    bump = 1.0 + 0.5 * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / (2.0 * sigma * sigma))
+    rho = np.zeros((n + 2 * hd, n + 2 * hd))
+    rho[hd:hd + n, hd:hd + n] = 10.0 ** (-1.0 + 3.0 * u) * bump
+    return rho

@@ -31,3 +31,30 @@ def rel_err(a, b):
 
 def scal_err(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
+
+
+def oracle_residual(o, u):
+    """||u0 - A u||_2 over the interior, formed by the oracle."""
+    o.put_field("u", u)
+    o.calc_residual()
+    return float(np.linalg.norm(o.interior(o.get_field("r"))))
+
+
+def oracle_cg(o, tol=1e-14, max_iters=20000):
+    """Plain FP64 PCG with the oracle's reference-granularity kernels from the
+    oracle's current u to ||r|| <= tol ||r_0|| (tea_solve.f90:103-125, 270-300);
+    returns u."""
+    o.calc_residual()
+    rro = o.cg_init()
+    r0 = np.sqrt(o.calc_2norm(1))
+    if r0 == 0.0:
+        return o.get_field("u")
+    for _ in range(max_iters):
+        pw = o.cg_calc_w()
+        alpha = rro / pw
+        rrn = o.cg_calc_ur(alpha)
+        if np.sqrt(o.calc_2norm(1)) <= tol * r0:
+            break
+        o.cg_calc_p(rrn / rro)
+        rro = rrn
+    return o.get_field("u")

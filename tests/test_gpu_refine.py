@@ -12,34 +12,11 @@ field summaries.
 import numpy as np
 import pytest
 
-from _pairs import setup
+from _pairs import oracle_cg, oracle_residual, setup
 
 pytestmark = pytest.mark.gpu
 
 SIZES = [(67, 13, 1), (130, 70, 2), (200, 41, 1), (256, 256, 1)]
-
-
-def oracle_residual(o, u):
-    o.put_field("u", u)
-    o.calc_residual()
-    return float(np.linalg.norm(o.interior(o.get_field("r"))))
-
-
-def oracle_cg(o, tol=1e-14, max_iters=20000):
-    """Plain FP64 PCG with the oracle's reference-granularity kernels, from
-    the oracle's current u (tea_solve.f90:103-125, 270-300)."""
-    o.calc_residual()
-    rro = o.cg_init()
-    r0 = np.sqrt(o.calc_2norm(1))
-    for _ in range(max_iters):
-        pw = o.cg_calc_w()
-        alpha = rro / pw
-        rrn = o.cg_calc_ur(alpha)
-        if np.sqrt(o.calc_2norm(1)) <= tol * r0:
-            break
-        o.cg_calc_p(rrn / rro)
-        rro = rrn
-    return o.get_field("u")
 
 
 def pair(hiplib, oracle, x, y, hd, prec, coef, seed=1):
@@ -145,3 +122,14 @@ def test_deck_summary_vs_oracle(hiplib, oracle, tmp_path, precision):
     assert np.max(np.abs(sg[:, 1:4] - so[:, 1:4]) / np.abs(so[:, 1:4])) < 1e-12
     rel = np.abs(sg[:, 4:6] - so[:, 4:6]) / np.abs(so[:, 4:6])
     assert np.max(rel) < 1e-8, rel
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_stalls_at_rounding_floor(hiplib, oracle, precision):
+    """A tolerance below what the FP64 true residual can reach ends the solve
+    once replacements stop making progress, long before max_iters."""
+    g, o = pair(hiplib, oracle, 130, 70, 2, 3, 2)
+    res = g.solve_refined(precision, tol=1e-19, max_iters=20000)
+    assert not res["converged"] and res["stalled"], res
+    assert res["iterations"] < 5000
+    assert res["rel_residual"] < 1e-12
