@@ -144,6 +144,10 @@ def main():
 
     tl.load_library()
     n = args.n
+    # attainable streaming bandwidth on the same layout (copy, and the stream
+    # mix of the dominant kernel without its stencil)
+    with tl.Chunk(n, n, 1, tl.PREC_JAC_BLOCK) as sc:
+        stream = {"copy_GBs": sc.stream_bandwidth(0, 20) / 1e9, "mix_4r3w_GBs": sc.stream_bandwidth(1, 20) / 1e9}
     with tempfile.TemporaryDirectory() as d:
         deck = decks.config2_deck(os.path.join(d, "bench.in"), n=n, steps=args.warmup + args.steps,
                                   summary_frequency=0, max_iters=args.max_iters)
@@ -163,6 +167,7 @@ def main():
         ach = alg / (k2 * 1e-3)
         roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": ach / PEAK_HBM, "traffic": load_traffic(n), "kernel": "k_fused_cg_ur<block>",
+                "measured_stream": stream, "frac_of_stream_mix": ach / 1e9 / stream["mix_4r3w_GBs"],
                 "alg_bytes_per_launch": alg, "avg_ms": k2, "samples": r["k_samples"],
                 "k_fused_cg_w": {"avg_ms": k1, "alg_bytes_per_launch": ALG_BYTES["k_fused_cg_w"] * cells,
                                  "achieved_GBs": ALG_BYTES["k_fused_cg_w"] * cells / (k1 * 1e-3) / 1e9}}

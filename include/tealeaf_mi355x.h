@@ -170,6 +170,11 @@ int tl_block_solve(tl_chunk *c);
 int tl_init_jac_diag(tl_chunk *c);
 int tl_synchronize(tl_chunk *c);
 int tl_print_profiling(tl_chunk *c);
+/* Measurement only (no reference counterpart): attainable streaming bandwidth
+ * in bytes/s on this chunk's layout, from `reps` event-timed launches of a
+ * copy (kind 0, 16 B/cell) or of a 4-read/3-write stream with the mix of the
+ * CG update kernel (kind 1, 56 B/cell), over temporary fields. */
+int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
 
 /* Whole-deck driver (start.f90 + diffuse.f90 + tea_solve.f90 restated).
  * Writes a line-oriented log: SUMMARY step vol mass density ie temp /

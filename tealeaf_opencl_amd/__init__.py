@@ -111,6 +111,7 @@ _SIGS = {
     "tl_bench_deck": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(BenchResult)]),
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
+    "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
     "tl_adjoint_density": (_I, [_P, _I, _D, _I, _D, _I, _I, _DP, _DP, ctypes.POINTER(AdjointResult)]),
 }
 
@@ -316,6 +317,13 @@ class Chunk:
 
     def synchronize(self):
         _check(self.lib.tl_synchronize(self.h), "synchronize")
+
+    def stream_bandwidth(self, kind=1, reps=20):
+        """Attainable streaming bytes/s on this chunk's layout (kind 0 copy,
+        kind 1 the 4-read/3-write mix of the CG update kernel)."""
+        v = _D()
+        _check(self.lib.tl_stream_bandwidth(self.h, int(kind), int(reps), ctypes.byref(v)), "stream_bandwidth")
+        return v.value
 
     def forward_loss(self, coefficient, dt, nsteps, u_obs, tol=1e-12, max_iters=20000):
         """L = sum over the interior of (u - u_obs)^2 after `nsteps` timesteps

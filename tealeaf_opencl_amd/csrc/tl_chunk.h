@@ -25,7 +25,7 @@ struct tl_chunk {
     double *f1[tl::NUM_FIELDS_1D] = {};
     int len1d = 0;
 
-    int cap = 2048;                  // max blocks per launch
+    int cap = 8192;                  // max blocks per launch
     double *partials = nullptr;
     double *result = nullptr;
     tl::SolveState *st = nullptr;

@@ -772,6 +772,53 @@ extern "C" int tl_jacobi_solve(tl_chunk *c, double *error)
 }
 
 // ocl_errors.cpp:203-232
+// Attainable streaming bandwidth on the chunk's layout: `reps` launches of a
+// copy (kind 0: 8 B/cell read + 8 B/cell written) or of the 4-read/3-write
+// mix of the CG update kernel (kind 1: 56 B/cell) over the interior, on
+// temporary fields, timed with events on the chunk's stream.
+extern "C" int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s)
+{
+    if (!c || (kind != 0 && kind != 1) || reps < 1 || !bytes_per_s) {
+        set_error("tl_stream_bandwidth: bad kind %d or reps %d", kind, reps);
+        return TL_ERR_ARG;
+    }
+    const int nf = kind == 0 ? 2 : 5;
+    double *base[5] = {}, *f[5] = {};
+    hipError_t e = hipSuccess;
+    int rc = TL_OK;
+    for (int i = 0; i < nf && e == hipSuccess; ++i) {
+        e = hipMalloc(&base[i], sizeof(double) * c->elems2d);
+        if (e == hipSuccess) e = hipMemsetAsync(base[i], 0, sizeof(double) * c->elems2d, c->s);
+        f[i] = base[i] + c->off;
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) {
+        const Ext in = ext_interior(c);
+        auto once = [&]() {
+            if (kind == 0) launch_copy(c->s, c->g, in, c->cap, f[0], f[1]);
+            else launch_stream_mix(c->s, c->g, in, c->cap, f[0], f[1], f[2], f[3], f[4]);
+        };
+        once();
+        hipEventRecord(t0, c->s);
+        for (int r = 0; r < reps; ++r) once();
+        hipEventRecord(t1, c->s);
+        e = hipEventSynchronize(t1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        const double cells = (double)c->g.x_max * c->g.y_max;
+        *bytes_per_s = cells * (kind == 0 ? 16.0 : 56.0) * reps / (1e-3 * ms);
+    }
+    if (e != hipSuccess) rc = hip_fail("tl_stream_bandwidth", e);
+    hipStreamSynchronize(c->s);
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
+    for (double *b : base)
+        if (b) hipFree(b);
+    return rc;
+}
+
 extern "C" int tl_print_profiling(tl_chunk *c)
 {
     if (!c || !c->profiler_on) return TL_OK;

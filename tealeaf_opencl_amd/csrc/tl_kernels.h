@@ -18,6 +18,9 @@ void launch_generate(hipStream_t s, const Geom &g, const Ext &e, int cap, double
                      double *e0, double *u, double *u0, int nstates, const double *const *sarr,
                      const int *sgeom, int g_rect, int g_circ, int g_point);
 void launch_copy(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src, double *dst);
+// 4 fields read, 3 written (the stream mix of the CG update kernel), no stencil
+void launch_stream_mix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *a, const double *b, double *c,
+                       const double *d, double *out);
 void launch_field_summary(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *vol,
                           const double *rho, const double *en, const double *u, const RedArgs &ra);
 // face: 0 left, 1 right, 2 bottom, 3 top

@@ -240,6 +240,30 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_copy(Geom g, Ext e, const dou
     TL_TILE_LOOP_END
 }
 
+// Streaming reference with the stream mix of k_fused_cg_ur (4 fields read,
+// 3 written, no stencil): a += s b, c -= s d, e = c d. Used only to measure
+// the attainable bandwidth of that mix.
+__global__ void __launch_bounds__(BLOCK_THREADS) k_stream_mix(Geom g, Ext e, double *a, const double *b, double *c,
+                                                              const double *d, double *out, double s)
+{
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double av[STRIP], bv[STRIP], cv[STRIP], dv[STRIP];
+    col4(a, i0, P, av);
+    col4(b, i0, P, bv);
+    col4(c, i0, P, cv);
+    col4(d, i0, P, dv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const double cn = cv[q] - s * dv[q];
+        a[i0 + q * P] = av[q] + s * bv[q];
+        c[i0 + q * P] = cn;
+        out[i0 + q * P] = cn * dv[q];
+    }
+    TL_TILE_LOOP_END
+}
+
 // field_summary (field_summary_cl.cl:3-43), energy1 bound to `energy`
 __global__ void __launch_bounds__(BLOCK_THREADS) k_field_summary(Geom g, Ext e, const double *vol,
                                                                  const double *rho, const double *en,
@@ -837,7 +861,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
-        p_new[i0 + q * P] = c6[q + 1];
+        store_nt(p_new + i0 + q * P, c6[q + 1]);
         acc[0] += c6[q + 1] * wv[q];
     }
     TL_TILE_LOOP_END
@@ -866,8 +890,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP];
     strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col4(u, i0, P, uv);
-    col4(r, i0, P, rv);
+    col4_nt(u, i0, P, uv);
+    col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
@@ -880,13 +904,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        u[i] = uv[q];
-        r[i] = rv[q];
+        store_nt(u + i, uv[q]);
+        store_nt(r + i, rv[q]);
         double zz = rv[q];
         if (PREC == PREC_BLOCK) zz = zv[q];
         if (PREC == PREC_DIAG) zz = rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
-        if (PREC != PREC_NONE) z[i] = zz;
-        if (PPCG) sd[i] = zz / theta;
+        if (PREC != PREC_NONE) store_nt(z + i, zz);
+        if (PPCG) store_nt(sd + i, zz / theta);
         else acc[0] += rv[q] * zz;
     }
     TL_TILE_LOOP_END
@@ -910,9 +934,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP],
         zo[STRIP];
     strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col4(u, i0, P, uv);
-    col4(r, i0, P, rv);
-    if (LAST && PREC != PREC_NONE) col4(z, i0, P, zo);
+    col4_nt(u, i0, P, uv);
+    col4_nt(r, i0, P, rv);
+    if (LAST && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
     strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) rv[q] -= res[q];
@@ -922,12 +946,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        u[i] = uv[q] + sc[q];
-        r[i] = rv[q];
-        if (PREC == PREC_BLOCK) sd_out[i] = ca * sc[q] + cb * zv[q];
+        store_nt(u + i, uv[q] + sc[q]);
+        store_nt(r + i, rv[q]);
+        if (PREC == PREC_BLOCK) store_nt(sd_out + i, ca * sc[q] + cb * zv[q]);
         else if (PREC == PREC_DIAG)
-            sd_out[i] = ca * sc[q] + cb * rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
-        else sd_out[i] = ca * sc[q] + cb * rv[q];
+            store_nt(sd_out + i, ca * sc[q] + cb * rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])));
+        else store_nt(sd_out + i, ca * sc[q] + cb * rv[q]);
         if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
     }
     TL_TILE_LOOP_END
@@ -986,6 +1010,12 @@ void launch_generate(hipStream_t s, const Geom &g, const Ext &e, int cap, double
 void launch_copy(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src, double *dst)
 {
     TL_LAUNCH(k_copy, e, cap, s, g, e, src, dst);
+}
+
+void launch_stream_mix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *a, const double *b, double *c,
+                       const double *d, double *out)
+{
+    TL_LAUNCH(k_stream_mix, e, cap, s, g, e, a, b, c, d, out, 1e-3);
 }
 
 void launch_field_summary(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *vol,

@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
-        p_new[i0 + q * P] = (T)c6[q + 1];
+        store_nt(p_new + i0 + q * P, c6[q + 1]);
         acc[0] += c6[q + 1] * wv[q];
     }
     TL_TILE_LOOP_END
@@ -264,8 +264,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
     strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col4(d, i0, P, dv);
-    col4(r, i0, P, rv);
+    col4_nt(d, i0, P, dv);
+    col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
@@ -279,9 +279,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
         const T zt = (T)zv[q];
-        d[i] = (T)dv[q];
-        r[i] = (T)rv[q];
-        if (PREC != PREC_NONE) z[i] = zt;
+        store_nt(d + i, dv[q]);
+        store_nt(r + i, rv[q]);
+        if (PREC != PREC_NONE) store_nt(z + i, (double)zt);
         acc[0] += rv[q] * (double)zt;
         acc[1] += rv[q] * rv[q];
     }
