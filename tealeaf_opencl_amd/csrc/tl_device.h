@@ -282,6 +282,22 @@ __device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, cons
     col6(w, i0, P, wc);
     col4(w, i0 - 1, P, ww);
     col4(w, i0 + 1, P, we);
+    const int hd = g.hd;
+    // every face of the strip lies strictly inside the chunk: no zeroing and
+    // no extent test can apply (the common case; only strips on a wall take
+    // the checked path)
+    const bool inner = j > hd && j + 1 < hd + g.x_max && ks > hd && ks + STRIP < hd + g.y_max;
+    if (inner) {
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx0[q] = HALF_INV ? face_coef_h(ww[q], wc[q + 1], f.rx) : face_coef(ww[q], wc[q + 1], f.rx);
+            kx1[q] = HALF_INV ? face_coef_h(wc[q + 1], we[q], f.rx) : face_coef(wc[q + 1], we[q], f.rx);
+        }
+#pragma unroll
+        for (int q = 0; q < STRIP + 1; ++q)
+            ky[q] = HALF_INV ? face_coef_h(wc[q], wc[q + 1], f.ry) : face_coef(wc[q], wc[q + 1], f.ry);
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         kx0[q] = kx_at<HALF_INV>(g, f, j, ks + q, ww[q], wc[q + 1]);
