@@ -231,10 +231,20 @@ typedef struct {
     int converged;
     int stalled;              /* stopped at the rounding floor above tol */
     int k_samples;
-    int pad_;
+    int inner_iterations;     /* PPCG inner steps (tl_solve_refined_ppcg) */
 } tl_refine_result;
 int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_iters, double delta,
                      tl_refine_result *out);
+/* The same refined solve with PPCG (tea_solve.f90:231-267 recurrence, the
+ * outer r.z formed with the z of the outer step as the reference does):
+ * `presteps` CG iterations (2 .. 1024) give the eigenvalue bounds of the
+ * Chebyshev smoother (tea_leaf_cheby.f90:96-172, widened by 0.95 / 1.05), then
+ * PPCG outer steps of `inner_steps` (1 .. 256) smoothing steps each, all in the
+ * storage precision of the inner fields, with the same residual replacement
+ * and stopping rules. iterations counts CG + PPCG outer steps;
+ * inner_iterations the smoothing steps. */
+int tl_solve_refined_ppcg(tl_chunk *c, int precision, double tol, int max_iters, double delta,
+                          int presteps, int inner_steps, tl_refine_result *out);
 
 /* Adjoint of the implicit timesteps with respect to the per-cell density. No
  * reference counterpart (the reference has no derivatives); the forward model

@@ -55,7 +55,7 @@ class RefineResult(ctypes.Structure):
     _fields_ = [("seconds", ctypes.c_double), ("initial_residual", ctypes.c_double),
                 ("rel_residual", ctypes.c_double), ("k1_ms", ctypes.c_double), ("k2_ms", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("updates", ctypes.c_int), ("converged", ctypes.c_int),
-                ("stalled", ctypes.c_int), ("k_samples", ctypes.c_int), ("pad_", ctypes.c_int)]
+                ("stalled", ctypes.c_int), ("k_samples", ctypes.c_int), ("inner_iterations", ctypes.c_int)]
 
 
 class AdjointResult(ctypes.Structure):
@@ -112,6 +112,7 @@ _SIGS = {
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
     "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
+    "tl_solve_refined_ppcg": (_I, [_P, _I, _D, _I, _D, _I, _I, ctypes.POINTER(RefineResult)]),
     "tl_adjoint_density": (_I, [_P, _I, _D, _I, _D, _I, _I, _DP, _DP, ctypes.POINTER(AdjointResult)]),
 }
 
@@ -359,10 +360,26 @@ class Chunk:
         r = RefineResult()
         _check(self.lib.tl_solve_refined(self.h, int(precision), float(tol), int(max_iters), float(delta),
                                          ctypes.byref(r)), "solve_refined")
-        return {"seconds": r.seconds, "initial_residual": r.initial_residual, "rel_residual": r.rel_residual,
-                "iterations": r.iterations, "updates": r.updates, "converged": bool(r.converged),
-                "stalled": bool(r.stalled),
-                "k1_ms": r.k1_ms, "k2_ms": r.k2_ms, "k_samples": r.k_samples}
+        return _refine_dict(r)
+
+    def solve_refined_ppcg(self, precision=PRECISION_MIXED, tol=1e-10, max_iters=10000, delta=None, presteps=20,
+                           inner_steps=10):
+        """The refined solve with PPCG outer steps after `presteps` CG
+        iterations (tl_solve_refined_ppcg)."""
+        if delta is None:
+            delta = 0.1 if precision == PRECISION_MIXED else 0.0
+        r = RefineResult()
+        _check(self.lib.tl_solve_refined_ppcg(self.h, int(precision), float(tol), int(max_iters), float(delta),
+                                              int(presteps), int(inner_steps), ctypes.byref(r)),
+               "solve_refined_ppcg")
+        return _refine_dict(r)
+
+
+def _refine_dict(r):
+    return {"seconds": r.seconds, "initial_residual": r.initial_residual, "rel_residual": r.rel_residual,
+            "iterations": r.iterations, "updates": r.updates, "converged": bool(r.converged),
+            "stalled": bool(r.stalled), "k1_ms": r.k1_ms, "k2_ms": r.k2_ms, "k_samples": r.k_samples,
+            "inner_iterations": r.inner_iterations}
 
 
 def parse_log(path):

@@ -12,8 +12,9 @@ LIBDIR = os.path.join(HERE, "lib")
 LIBNAME = "libtealeaf_mi355x.so"
 ARCH = os.environ.get("TL_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["tl_kernels.hip", "tl_refine.hip", "tl_adjoint.hip", "tl_chunk.hip", "tl_driver.hip", "tl_deck.cpp"]
-HEADERS = ["tl_common.h", "tl_device.h", "tl_kernels.h", "tl_chunk.h", "tl_deck.h"]
+SOURCES = ["tl_kernels.hip", "tl_refine.hip", "tl_adjoint.hip", "tl_chunk.hip", "tl_driver.hip", "tl_deck.cpp",
+           "tl_eigen.cpp"]
+HEADERS = ["tl_common.h", "tl_device.h", "tl_kernels.h", "tl_chunk.h", "tl_deck.h", "tl_eigen.h"]
 
 # packed FP32 instructions are disabled on every device compile
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

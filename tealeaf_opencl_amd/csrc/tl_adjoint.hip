@@ -361,8 +361,8 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
         out->work_bytes = (double)wk.bases.size() * sizeof(double) * c->elems2d;
         double chunk = (double)NUM_FIELDS_2D * sizeof(double) * c->elems2d;
         for (const auto &w : c->rf)
-            for (int i = 0; i < 7; ++i)
-                if (w.base[i]) chunk += (double)w.bytes * w.elems;
+            for (void *buf : w.base)
+                if (buf) chunk += (double)w.bytes * w.elems;
         out->chunk_bytes = chunk;
         out->forward_iterations = cnt.forward;
         out->recompute_iterations = cnt.recompute;

@@ -67,11 +67,14 @@ struct tl_chunk {
         int bytes = 0;                    // 8 or 4: storage of the inner fields
         tl::Geom g{};                     // layout of the inner fields
         long off = 0, elems = 0;
-        void *base[7] = {};               // owned allocations
-        void *f[7] = {};                  // conductivity w, r, z, p, p2, d
+        void *base[8] = {};               // owned allocations
+        void *f[8] = {};                  // conductivity w, r, z, p, p2, d, sd, sd2
         tl::RefineState *st = nullptr;
         tl::RefineState *h_st = nullptr;  // pinned, 2 slots
-        GraphSlot graph;
+        GraphSlot graph;                  // CG iterations
+        GraphSlot graph_ppcg;             // PPCG outer steps
+        double *hist = nullptr;           // CG alphas, betas of the first iterations
+        double *coef = nullptr;           // PPCG inner Chebyshev alphas, betas
     } rf[2];
 };
 

@@ -141,6 +141,9 @@ struct RefineState {
     int updates;          // true residuals formed (the initial one included)
     int stalls;           // consecutive updates whose true residual fell by less than half
     double rr_prev;       // ||r||^2 of the previous true residual
+    double theta;         // PPCG: centre of the eigenvalue interval (sd = z / theta)
+    int stop_n;           // > 0: halt (without an update) once n reaches it
+    int pad2_;
 };
 
 enum RefineMode {

@@ -32,9 +32,13 @@
 #include <math.h>
 #include <string.h>
 
+#include <functional>
+#include <vector>
+
 #include "../../include/tealeaf_mi355x.h"
 #include "tl_chunk.h"
 #include "tl_device.h"
+#include "tl_eigen.h"
 #include "tl_kernels.h"
 
 namespace tl {
@@ -43,21 +47,24 @@ namespace tl {
 // internal linkage are not visible to the profiling tools)
 namespace rf {
 
-enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_N };
+enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_SD, RW_SD2, RW_N };
 
 struct RfArgs {
     double *partials;
     RefineState *st;
     int mode;
+    double *hist_a;       // CG alphas / betas by iteration (0-based), or null
+    double *hist_b;
+    int hist_len;
 };
 
 // replacements in a row without progress after which the solve stops
 constexpr int STALL_LIMIT = 3;
 
-__device__ void rf_scalars(RefineState *s, int mode, const double *tot)
+__device__ void rf_scalars(RefineState *s, const RfArgs &ra, const double *tot)
 {
     const double rz = tot[0], rr = tot[1];
-    switch (mode) {
+    switch (ra.mode) {
     case RF_UPDATE: {
         if (s->updates == 0) {
             s->rr0 = rr;
@@ -92,10 +99,12 @@ __device__ void rf_scalars(RefineState *s, int mode, const double *tot)
     case RF_PW:
         s->pw = tot[0];
         s->alpha = s->rro / s->pw;
+        if (ra.hist_a && s->n < ra.hist_len) ra.hist_a[s->n] = s->alpha;
         break;
     case RF_RRN: {
         s->rz_prev = s->rro;
         s->beta = rz / s->rro;
+        if (ra.hist_b && s->n < ra.hist_len) ra.hist_b[s->n] = s->beta;
         s->rro = rz;
         s->rr = rr;
         s->n += 1;
@@ -103,6 +112,8 @@ __device__ void rf_scalars(RefineState *s, int mode, const double *tot)
         if (rn > s->rmax) s->rmax = rn;
         if (rn <= s->tol * sqrt(s->rr0) || rn < s->delta * s->rmax || s->n >= s->max_iters || !isfinite(rz)) {
             s->want_update = 1;
+            s->halt = 1;
+        } else if (s->stop_n > 0 && s->n >= s->stop_n) {
             s->halt = 1;
         }
         break;
@@ -117,7 +128,7 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_rf_finish(RfArgs ra, int nbl
     if (ra.mode != RF_UPDATE && ra.st->halt) return;
     double tot[MAX_RED];
     sum_partials(ra.partials, nblocks, 2, tot);
-    if (threadIdx.x == 0) rf_scalars(ra.st, ra.mode, tot);
+    if (threadIdx.x == 0) rf_scalars(ra.st, ra, tot);
 }
 
 template <typename T>
@@ -289,6 +300,91 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     block_partial<2>(acc, ra.partials);
 }
 
+// PPCG outer step, second half (the refined form of k_fused_cg_ur in PPCG
+// mode, tea_leaf_cg_cl.cl:73-137 + tea_leaf_ppcg_cl.cl:4-60): d += alpha p,
+// r -= alpha A p, z = M^-1 r, sd = z / theta; no reduction (the outer step's
+// r.z is formed after the inner steps with this z, as the reference does).
+template <typename T, int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
+                                                              T *z, T *sd, const T *W, Faces fc, RfArgs ra)
+{
+    const RefineState *st = ra.st;
+    if (st->halt) return;
+    const T *p = (st->n & 1) ? pB : pA;
+    const double alpha = st->alpha, theta = st->theta;
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
+    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    col4_nt(d, i0, P, dv);
+    col4_nt(r, i0, P, rv);
+    strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        dv[q] += alpha * pc[q];
+        rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
+    }
+    mask_rows(ks, e.k1, rv);
+    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        const double zt = rnd<T>(zv[q]);
+        store_nt(d + i, dv[q]);
+        store_nt(r + i, rv[q]);
+        store_nt(z + i, zt);
+        store_nt(sd + i, zt / theta);
+    }
+    TL_TILE_LOOP_END
+}
+
+// PPCG inner step (tea_leaf_ppcg_cl.cl:62-155, fused update_r + calc_sd):
+// d += sd, r -= A sd, sd' = a_i sd + b_i M^-1 r; sd double-buffered. LAST
+// also sums r.z with the z of the outer step and r.r.
+template <typename T, int PREC, int LAST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d, T *r, const T *sd_in, T *sd_out,
+                                                            const T *W, Faces fc, const T *z, const double *ch_a,
+                                                            const double *ch_b, int step, RfArgs ra)
+{
+    const RefineState *st = ra.st;
+    if (st->halt) return;
+    const double ca = ch_a[step], cb = ch_b[step];
+    double acc[2] = {0.0, 0.0};
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], dv[STRIP], rv[STRIP], mv[STRIP],
+        zo[STRIP];
+    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    col4_nt(d, i0, P, dv);
+    col4_nt(r, i0, P, rv);
+    if (LAST) col4_nt(z, i0, P, zo);
+    strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = rnd<T>(rv[q] - res[q]);
+    mask_rows(ks, e.k1, rv);
+    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, mv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        store_nt(d + i, dv[q] + sc[q]);
+        store_nt(r + i, rv[q]);
+        store_nt(sd_out + i, ca * sc[q] + cb * mv[q]);
+        if (LAST) {
+            acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+            acc[1] += rv[q] * rv[q];
+        }
+    }
+    TL_TILE_LOOP_END
+    if (LAST) block_partial<2>(acc, ra.partials);
+}
+
+// history of the first CG iterations kept for the eigenvalue estimate, and
+// the largest number of PPCG inner steps
+constexpr int RF_HIST = 1024;
+constexpr int RF_MAX_INNER = 256;
+
 inline int grid_of(const Ext &e, int cap)
 {
     const int tiles = e.ntx * e.nty;
@@ -300,7 +396,7 @@ void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, dou
 {
     const Ext e = ext_interior(c);
     const int grid = grid_of(e, c->cap);
-    const RfArgs ra{c->partials, w.st, RF_UPDATE};
+    const RfArgs ra{c->partials, w.st, RF_UPDATE, nullptr, nullptr, 0};
     T *const *f = reinterpret_cast<T *const *>(w.f);
 #define TL_RFU(P)                                                                                           \
     hipLaunchKernelGGL((k_rf_update<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, u_out, \
@@ -323,7 +419,8 @@ void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nul
     const int grid = grid_of(e, c->cap);
     T *const *f = reinterpret_cast<T *const *>(w.f);
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
-    const RfArgs ra_w{c->partials, w.st, RF_PW}, ra_u{c->partials, w.st, RF_RRN};
+    const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
+        ra_u{c->partials, w.st, RF_RRN, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) hipEventRecord(ev[0], c->s);
     hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
                        f[RW_W], c->faces, ra_w);
@@ -345,10 +442,12 @@ constexpr int RF_BATCH = 16;
 
 void release_work(tl_chunk::RefineWork &w)
 {
-    if (w.graph.exec) hipGraphExecDestroy(w.graph.exec);
+    for (auto *gs : {&w.graph, &w.graph_ppcg})
+        if (gs->exec) hipGraphExecDestroy(gs->exec);
     for (void *p : w.base)
         if (p) hipFree(p);
-    if (w.st) hipFree(w.st);
+    for (void *p : {(void *)w.st, (void *)w.hist, (void *)w.coef})
+        if (p) hipFree(p);
     if (w.h_st) hipHostFree(w.h_st);
     w = tl_chunk::RefineWork();
 }
@@ -385,7 +484,14 @@ int rf_alloc_impl(tl_chunk *c, int which)
         w.f[RW_Z] = c->f2[F_Z];
         w.f[RW_P] = c->f2[F_P];
         w.f[RW_P2] = c->f2[F_P2];
+        w.f[RW_SD] = c->f2[F_SD];
+        w.f[RW_SD2] = c->f2[F_SD2];
     }
+    if ((e = hipMalloc(&w.hist, sizeof(double) * 2 * RF_HIST)) != hipSuccess ||
+        (e = hipMalloc(&w.coef, sizeof(double) * 2 * RF_MAX_INNER)) != hipSuccess)
+        return hip_fail("hipMalloc refine history", e);
+    hipMemsetAsync(w.hist, 0, sizeof(double) * 2 * RF_HIST, c->s);
+    hipMemsetAsync(w.coef, 0, sizeof(double) * 2 * RF_MAX_INNER, c->s);
     if ((e = hipMalloc(&w.st, sizeof(RefineState))) != hipSuccess) return hip_fail("hipMalloc refine state", e);
     if ((e = hipHostMalloc(&w.h_st, sizeof(RefineState) * 2, hipHostMallocDefault)) != hipSuccess)
         return hip_fail("hipHostMalloc refine state", e);
@@ -410,30 +516,84 @@ int rf_state_get(tl_chunk *c, tl_chunk::RefineWork &w, RefineState *h)
     return TL_OK;
 }
 
+// One PPCG outer step: the CG half-step pair with the PPCG form of the
+// second kernel, then `inner` smoothing steps whose last one finishes the
+// step's scalars (tea_solve.f90:231-267 with tea_leaf_run_ppcg_inner_steps,
+// :422-475).
 template <typename T>
-int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
+void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
 {
-    const int key = c->prec;
-    if (w.graph.exec && w.graph.key == key) return TL_OK;
-    if (w.graph.exec) {
-        hipGraphExecDestroy(w.graph.exec);
-        w.graph.exec = nullptr;
+    const Ext e = ext_interior(c);
+    const int grid = grid_of(e, c->cap);
+    T *const *f = reinterpret_cast<T *const *>(w.f);
+    const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
+    const RfArgs ra_w{c->partials, w.st, RF_PW, nullptr, nullptr, 0},
+        ra_u{c->partials, w.st, RF_RRN, nullptr, nullptr, 0};
+    hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
+                       f[RW_W], c->faces, ra_w);
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
+    const double *ca = w.coef, *cb = w.coef + RF_MAX_INNER;
+#define TL_RFO(P)                                                                                              \
+    do {                                                                                                       \
+        hipLaunchKernelGGL((k_rf_ur_ppcg<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D],     \
+                           f[RW_P], f[RW_P2], f[RW_R], f[RW_Z], f[RW_SD], f[RW_W], c->faces, ra_u);             \
+        for (int i = 0; i < inner; ++i) {                                                                      \
+            T *sin = (i & 1) ? f[RW_SD2] : f[RW_SD], *sout = (i & 1) ? f[RW_SD] : f[RW_SD2];                   \
+            if (i == inner - 1)                                                                                \
+                hipLaunchKernelGGL((k_rf_inner<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
+                                   f[RW_D], f[RW_R], sin, sout, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u);   \
+            else                                                                                               \
+                hipLaunchKernelGGL((k_rf_inner<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
+                                   f[RW_D], f[RW_R], sin, sout, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u);   \
+        }                                                                                                      \
+    } while (0)
+    if (c->prec == PREC_BLOCK) TL_RFO(PREC_BLOCK);
+    else if (c->prec == PREC_DIAG) TL_RFO(PREC_DIAG);
+    else TL_RFO(PREC_NONE);
+#undef TL_RFO
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
+}
+
+int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body)
+{
+    if (gs.exec && gs.key == key) return TL_OK;
+    if (gs.exec) {
+        hipGraphExecDestroy(gs.exec);
+        gs.exec = nullptr;
     }
     hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
-    // the first iteration of each batch is launched eagerly (timed)
-    for (int b = 0; b < RF_BATCH - 1; ++b) launch_iteration<T>(c, w);
+    body();
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(c->s, &graph);
     if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
-    e = hipGraphInstantiate(&w.graph.exec, graph, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (e != hipSuccess) {
-        w.graph.exec = nullptr;
+        gs.exec = nullptr;
         return hip_fail("hipGraphInstantiate", e);
     }
-    w.graph.key = key;
+    gs.key = key;
     return TL_OK;
+}
+
+// CG iterations: the first iteration of each batch is launched eagerly (timed)
+template <typename T>
+int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
+{
+    return capture_into(c, w.graph, c->prec, [&]() {
+        for (int b = 0; b < RF_BATCH - 1; ++b) launch_iteration<T>(c, w);
+    });
+}
+
+int ppcg_batch(int inner) { return inner + 2 >= 32 ? 1 : 32 / (inner + 2); }
+
+template <typename T>
+int rf_capture_ppcg(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
+{
+    return capture_into(c, w.graph_ppcg, c->prec * 1000 + inner, [&]() {
+        for (int b = 0; b < ppcg_batch(inner); ++b) launch_ppcg_outer<T>(c, w, inner);
+    });
 }
 
 struct KernelTimes {
@@ -441,18 +601,19 @@ struct KernelTimes {
     int samples = 0;
 };
 
-// Replays the iteration graph until the device state halts; batch i+1 is
-// queued before the state after batch i is examined. Each batch starts with
-// one eagerly launched iteration bracketed by events; the events of a batch
+// Replays `exec` until the device state halts; batch i+1 is queued before the
+// state after batch i is examined. With `timed`, each batch starts with one
+// eagerly launched CG iteration bracketed by events; the events of a batch
 // are read once it has finished (every examined batch follows one that did
 // not halt, so its eager iteration ran).
 template <typename T>
-int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches, KernelTimes *kt)
+int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, hipGraphExec_t exec, bool timed, long max_batches,
+               KernelTimes *kt)
 {
     hipError_t e;
     auto enqueue = [&](int sl) -> hipError_t {
-        launch_iteration<T>(c, w, c->ev_t[sl]);
-        hipError_t ee = hipGraphLaunch(w.graph.exec, c->s);
+        if (timed) launch_iteration<T>(c, w, c->ev_t[sl]);
+        hipError_t ee = hipGraphLaunch(exec, c->s);
         if (ee != hipSuccess) return ee;
         ee = hipMemcpyAsync(&w.h_st[sl], w.st, sizeof(RefineState), hipMemcpyDeviceToHost, c->s);
         if (ee != hipSuccess) return ee;
@@ -464,7 +625,7 @@ int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches, KernelTim
         if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
         if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
         float a = 0.f, b = 0.f;
-        if (hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
+        if (timed && hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
             hipEventElapsedTime(&b, c->ev_t[slot][2], c->ev_t[slot][3]) == hipSuccess) {
             kt->k1_ms += a;
             kt->k2_ms += b;
@@ -483,8 +644,33 @@ int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, long max_batches, KernelTim
     return TL_OK;
 }
 
+// Runs batches of `exec` with reliable updates between them until the solve
+// has converged, stalled, run out of iterations, or halted at stop_n.
 template <typename T>
-int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl_refine_result *out)
+int rf_phase(tl_chunk *c, tl_chunk::RefineWork &w, hipGraphExec_t exec, bool timed, long max_batches,
+             KernelTimes *kt, double **ubuf, int *cur, RefineState *h)
+{
+    int rc;
+    hipError_t e;
+    while (!h->converged && h->n < h->max_iters && isfinite(h->rr) && h->stalls < STALL_LIMIT) {
+        if ((rc = rf_batches<T>(c, w, exec, timed, max_batches, kt)) != TL_OK) return rc;
+        if ((rc = rf_state_get(c, w, h)) != TL_OK) return rc;
+        if (!h->want_update) break;
+        launch_update<T>(c, w, ubuf[*cur], ubuf[1 - *cur]);
+        *cur = 1 - *cur;
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine update", e);
+        if ((rc = rf_state_get(c, w, h)) != TL_OK) return rc;
+    }
+    return TL_OK;
+}
+
+// The refined solve. inner == 0: CG. inner > 0: PPCG with `inner` smoothing
+// steps per outer step after `presteps` CG iterations, whose coefficients
+// give the eigenvalue bounds (tea_leaf_cheby.f90:96-172, widened by
+// 0.95 / 1.05 as tea_solve.f90:175-176 does).
+template <typename T>
+int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, int presteps, int inner,
+             tl_refine_result *out)
 {
     int rc = rf_alloc(c, which);
     if (rc != TL_OK) return rc;
@@ -500,12 +686,14 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
         ~EvGuard() { hipEventDestroy(a); hipEventDestroy(b); }
     } guard{t0, t1};
     if ((rc = rf_capture<T>(c, w)) != TL_OK) return rc;
+    if (inner > 0 && (rc = rf_capture_ppcg<T>(c, w, inner)) != TL_OK) return rc;
 
     RefineState h;
     memset(&h, 0, sizeof h);
     h.tol = tol;
     h.delta = delta;
     h.max_iters = max_iters;
+    h.stop_n = inner > 0 ? presteps : 0;
     if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("refine start", e);
     w.h_st[0] = h;
     hipEventRecord(t0, c->s);
@@ -520,16 +708,36 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
     cur = 1 - cur;
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine update", e);
     if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
-    const long max_batches = max_iters / RF_BATCH + 4;
     KernelTimes kt;
-    while (!h.converged && h.n < max_iters && isfinite(h.rr) && h.stalls < STALL_LIMIT) {
-        if ((rc = rf_batches<T>(c, w, max_batches, &kt)) != TL_OK) return rc;
-        if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
-        if (!h.want_update) break;
-        launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);
-        cur = 1 - cur;
-        if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine update", e);
-        if ((rc = rf_state_get(c, w, &h)) != TL_OK) return rc;
+    if ((rc = rf_phase<T>(c, w, w.graph.exec, true, max_iters / RF_BATCH + 4, &kt, ubuf, &cur, &h)) != TL_OK)
+        return rc;
+    int outer0 = h.n;
+    if (inner > 0 && !h.converged && h.halt && !h.want_update && h.n >= h.stop_n && h.n < max_iters &&
+        h.stalls < STALL_LIMIT) {
+        // eigenvalue bounds from the CG coefficients, Chebyshev coefficients
+        const int ns = h.n < RF_HIST ? h.n : RF_HIST;
+        std::vector<double> a(ns), b(ns);
+        hipMemcpy(a.data(), w.hist, sizeof(double) * ns, hipMemcpyDeviceToHost);
+        hipMemcpy(b.data(), w.hist + RF_HIST, sizeof(double) * ns, hipMemcpyDeviceToHost);
+        double eigmin = 0, eigmax = 0, theta = 0;
+        if (calc_eigenvalues(a.data(), b.data(), ns, &eigmin, &eigmax) != 0) {
+            set_error("refined PPCG: eigenvalue estimate from %d CG steps failed", ns);
+            return TL_ERR_SOLVER;
+        }
+        eigmin *= 0.95;
+        eigmax *= 1.05;
+        std::vector<double> ca, cb;
+        calc_ch_coefs(ca, cb, eigmin, eigmax, &theta, inner);
+        hipMemcpy(w.coef, ca.data(), sizeof(double) * inner, hipMemcpyHostToDevice);
+        hipMemcpy(w.coef + RF_MAX_INNER, cb.data(), sizeof(double) * inner, hipMemcpyHostToDevice);
+        h.theta = theta;
+        h.stop_n = 0;
+        h.halt = 0;
+        w.h_st[0] = h;
+        hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
+        outer0 = h.n;
+        const long mb = (max_iters - h.n) / ppcg_batch(inner) + 4;
+        if ((rc = rf_phase<T>(c, w, w.graph_ppcg.exec, false, mb, &kt, ubuf, &cur, &h)) != TL_OK) return rc;
     }
     if (cur == 1) launch_copy(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U2], c->f2[F_U]);
     hipEventRecord(t1, c->s);
@@ -548,6 +756,7 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, tl
         out->k_samples = kt.samples;
         out->k1_ms = kt.samples ? kt.k1_ms / kt.samples : -1.0;
         out->k2_ms = kt.samples ? kt.k2_ms / kt.samples : -1.0;
+        out->inner_iterations = inner > 0 ? (h.n - outer0) * inner : 0;
     }
     if (!h.converged && !(h.n >= max_iters) && h.stalls < STALL_LIMIT) {
         set_error("refined solve broke down after %d iterations (residual %g)", h.n, sqrt(h.rr));
@@ -580,6 +789,26 @@ extern "C" int tl_solve_refined(tl_chunk *c, int precision, double tol, int max_
                   max_iters, delta);
         return TL_ERR_ARG;
     }
-    if (precision == TL_PRECISION_MIXED) return rf::rf_solve<float>(c, 1, tol, max_iters, delta, out);
-    return rf::rf_solve<double>(c, 0, tol, max_iters, delta, out);
+    if (precision == TL_PRECISION_MIXED) return rf::rf_solve<float>(c, 1, tol, max_iters, delta, 0, 0, out);
+    return rf::rf_solve<double>(c, 0, tol, max_iters, delta, 0, 0, out);
+}
+
+extern "C" int tl_solve_refined_ppcg(tl_chunk *c, int precision, double tol, int max_iters, double delta,
+                                     int presteps, int inner_steps, tl_refine_result *out)
+{
+    if (!c) {
+        set_error("tl_solve_refined_ppcg: null chunk");
+        return TL_ERR_ARG;
+    }
+    if ((precision != TL_PRECISION_FP64 && precision != TL_PRECISION_MIXED) || !(tol > 0.0) || max_iters < 1 ||
+        delta < 0.0 || delta >= 1.0 || presteps < 2 || presteps > rf::RF_HIST || inner_steps < 1 ||
+        inner_steps > rf::RF_MAX_INNER) {
+        set_error("tl_solve_refined_ppcg: invalid precision %d, tol %g, max_iters %d, delta %g, presteps %d or "
+                  "inner steps %d",
+                  precision, tol, max_iters, delta, presteps, inner_steps);
+        return TL_ERR_ARG;
+    }
+    if (precision == TL_PRECISION_MIXED)
+        return rf::rf_solve<float>(c, 1, tol, max_iters, delta, presteps, inner_steps, out);
+    return rf::rf_solve<double>(c, 0, tol, max_iters, delta, presteps, inner_steps, out);
 }

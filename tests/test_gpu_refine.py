@@ -133,3 +133,46 @@ def test_stalls_at_rounding_floor(hiplib, oracle, precision):
     assert not res["converged"] and res["stalled"], res
     assert res["iterations"] < 5000
     assert res["rel_residual"] < 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("prec", [1, 2, 3])
+@pytest.mark.parametrize("x,y,hd", [(130, 70, 2), (200, 41, 1)])
+def test_ppcg_true_residual(hiplib, oracle, x, y, hd, prec, precision):
+    """Refined PPCG (CG presteps, eigenvalue bounds, Chebyshev smoothing inner
+    steps) reaches the true residual tolerance by the oracle's operator and
+    takes fewer outer steps than CG."""
+    tol = 1e-10
+    g, o = pair(hiplib, oracle, x, y, hd, prec, 2)
+    r0 = oracle_residual(o, o.get_field("u0"))
+    res = g.solve_refined_ppcg(precision, tol=tol, max_iters=5000, presteps=20, inner_steps=8)
+    assert res["converged"], res
+    r = oracle_residual(o, g.get_field("u"))
+    assert r <= 1.01 * tol * r0, (r / r0, res)
+    assert res["inner_iterations"] == 8 * (res["iterations"] - 20)
+    g2, _ = pair(hiplib, oracle, x, y, hd, prec, 2)
+    cg = g2.solve_refined(precision, tol=tol, max_iters=5000)
+    assert res["iterations"] < cg["iterations"], (res, cg)
+
+
+@pytest.mark.parametrize("eps,bound", [(1e-10, 2e-7), (1e-12, 2e-9)])
+def test_ppcg_deck_summary(hiplib, oracle, tmp_path, eps, bound):
+    """A PPCG deck with tl_mixed_precision: field summary against the
+    oracle's FP64 CG at a tight tolerance (the timestep's solution does not
+    depend on the solver). After the Chebyshev smoothing the remaining
+    residual is smooth, and A^-1 barely damps smooth components, so at the
+    same true residual the summary error is larger than after CG (here
+    ~7e-8 at 1e-10); it falls with the tolerance."""
+    from tealeaf_opencl_amd import decks
+    common = dict(n=192, steps=2, summary_frequency=1, extra=["tl_ppcg_inner_steps=6", "tl_ch_cg_presteps=15"])
+    ref = decks.config3_deck(str(tmp_path / "ref.in"), precision="reference", eps=1e-14, **common)
+    mix = decks.config3_deck(str(tmp_path / "mix.in"), precision="mixed", eps=eps, **common)
+    txt = open(mix).read().replace("tl_use_cg", "tl_use_ppcg")
+    open(mix, "w").write(txt)
+    lo = oracle.run_deck(ref, str(tmp_path / "o.log"))
+    lg = hiplib.run_deck(mix, str(tmp_path / "g.log"))
+    assert len(lg["refine"]) == 2 and all(r[3] == 1 for r in lg["refine"])
+    assert all(s[4] > 0 for s in lg["solve"])
+    so, sg = np.array(lo["summary"]), np.array(lg["summary"])
+    rel = np.abs(sg[:, 4:6] - so[:, 4:6]) / np.abs(so[:, 4:6])
+    assert np.max(rel) < bound, rel
