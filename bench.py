@@ -52,6 +52,12 @@ def parse():
     ap.add_argument("--mode", type=int, default=1, help="1 native (default), 0 drop-in")
     ap.add_argument("--max-iters", type=int, default=5000, help="tl_max_iters of the deck (short profiles)")
     ap.add_argument("--no-e10", action="store_true", help="skip the solve-to-1e-10 leg")
+    ap.add_argument("--large", action="store_true",
+                    help="run the large combined workload (8192^2 PPCG FP64 vs mixed, adjoint) instead")
+    ap.add_argument("--large-n", type=int, default=8192)
+    ap.add_argument("--large-steps", type=int, default=20)
+    ap.add_argument("--large-fd", type=int, default=2, help="finite-difference cells of the large adjoint")
+    ap.add_argument("--large-skip", default="", help="comma list of large legs to skip: fp64,mixed,adjoint")
     return ap.parse_args()
 
 
@@ -125,6 +131,88 @@ def refine_leg(n, tol=1e-10):
     }
 
 
+def large_leg(args):
+    """The large combined workload: n^2 cells (8192^2 by default, 67M cells,
+    537 MB per FP64 field), eight seeded rectangles with densities
+    10^U[-1,2], PPCG + block Jacobi with 10 inner steps, tl_eps 1e-10, for
+    `steps` timesteps, once with the reference FP64 PPCG (device-resident
+    driver) and once with the refined FP32-inner PPCG to a true relative
+    residual of 1e-10; then the density adjoint over the same timesteps
+    (conductivity = density, CG + block Jacobi solves to 1e-10, checkpoints
+    every 5 steps) with central differences (h = 1e-6 relative) on a few
+    sensitive cells."""
+    import numpy as np
+    import tealeaf_opencl_amd as tl
+    from tealeaf_opencl_amd import adjoint, decks
+
+    n, steps = args.large_n, args.large_steps
+    skip = set(x for x in args.large_skip.split(",") if x)
+    out = {"workload": "%dx%d, 8 seeded rectangles (density 10^U[-1,2], seed 1), PPCG + strip-4 block Jacobi, "
+                       "10 inner steps, dt 0.004, %d timesteps" % (n, n, steps), "cells": n * n}
+    legs = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name in ("fp64", "mixed"):
+            if name in skip:
+                continue
+            deck = decks.config5_deck(os.path.join(d, name + ".in"), precision="reference" if name == "fp64" else "mixed",
+                                      n=n, steps=steps, summary_frequency=0)
+            r = tl.bench_deck(deck, mode=tl.MODE_NATIVE, warmup=0, steps=steps)
+            legs[name] = {"seconds": r["seconds"], "outer_iterations": r["iterations"],
+                          "inner_iterations": r["inner_iterations"], "final_summary": r["final_summary"]}
+            print("large %s: %.3f s" % (name, r["seconds"]), file=sys.stderr, flush=True)
+    out.update(legs)
+    if "fp64" in legs and "mixed" in legs:
+        a, b = np.array(legs["fp64"]["final_summary"]), np.array(legs["mixed"]["final_summary"])
+        out["speedup"] = legs["fp64"]["seconds"] / legs["mixed"]["seconds"]
+        out["summary_rel_diff_mixed_vs_fp64"] = dict(zip(["volume", "mass", "density", "internal_energy",
+                                                          "temperature"], (np.abs(b - a) / np.abs(a)).tolist()))
+    if "adjoint" not in skip:
+        dt, coef, tol, every = 0.004, tl.COEF_CONDUCTIVITY, 1e-10, 5
+        c = tl.Chunk(n, n, 1, tl.PREC_JAC_BLOCK)
+        try:
+            dx = 10.0 / float(np.float32(n))
+            c.initialise_chunk(0.0, 0.0, dx, dx)
+            c.generate_chunk(*decks.state_arrays(decks.config5_states(2), n, n))
+            c.set_field()
+            e0 = c.get_field("energy1")
+            u_obs = adjoint.forward_u(c, coef, dt, steps, tol)
+            c.generate_chunk(*decks.state_arrays(decks.config5_states(1), n, n))
+            c.set_field()
+            rho, e0 = c.get_field("density"), c.get_field("energy1")
+            L, grad, info = c.adjoint_density(coef, dt, steps, u_obs, tol=tol, checkpoint_every=every)
+            print("large adjoint: forward %.3f s, adjoint %.3f s" % (info["forward_s"], info["adjoint_s"]),
+                  file=sys.stderr, flush=True)
+            fd = []
+            s_ = np.abs(rho * grad)
+            ks, js = np.nonzero(s_ >= 1e-2 * s_.max())
+            rng = np.random.default_rng(7)
+            for i in rng.choice(len(ks), size=min(args.large_fd, len(ks)), replace=False):
+                k, j = int(ks[i]), int(js[i])
+                h = 1e-6 * rho[k, j]
+                vals = []
+                for sg in (1.0, -1.0):
+                    r2 = rho.copy()
+                    r2[k, j] += sg * h
+                    c.put_field("density", r2)
+                    c.put_field("energy1", e0)
+                    vals.append(c.forward_loss(coef, dt, steps, u_obs, tol=tol))
+                fdv = (vals[0] - vals[1]) / (2 * h)
+                fd.append({"cell": [k, j], "fd": fdv, "adjoint": float(grad[k, j]),
+                           "rel_err": abs(fdv - grad[k, j]) / abs(grad[k, j])})
+            out["adjoint"] = {"loss": L, "forward_s": info["forward_s"], "adjoint_s": info["adjoint_s"],
+                              "adjoint_over_forward": info["adjoint_s"] / info["forward_s"],
+                              "checkpoint_every": every, "checkpoints": info["checkpoints"],
+                              "device_bytes": info["chunk_bytes"] + info["work_bytes"],
+                              "work_bytes": info["work_bytes"], "chunk_bytes": info["chunk_bytes"],
+                              "iterations": {"forward": info["forward_iterations"],
+                                             "recompute": info["recompute_iterations"],
+                                             "adjoint": info["adjoint_iterations"]},
+                              "stalled_solves": info["stalled_solves"], "finite_differences": fd}
+        finally:
+            c.close()
+    return out
+
+
 def load_traffic(n):
     path = os.path.join(ROOT, "profiles", "traffic_k_fused_cg_ur_%d.json" % n)
     if os.path.exists(path):
@@ -143,6 +231,9 @@ def main():
     from tealeaf_opencl_amd import decks
 
     tl.load_library()
+    if args.large:
+        print(json.dumps({"metric": METRIC, "large": large_leg(args)}))
+        return
     n = args.n
     # attainable streaming bandwidth on the same layout (copy, and the stream
     # mix of the dominant kernel without its stencil)

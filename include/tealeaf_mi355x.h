@@ -182,6 +182,14 @@ int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
  * switch_n est_itc / SWITCH / ESTITC / RESID / QA lines. max_steps > 0 caps
  * the number of timesteps. */
 int tl_run_deck(const char *deck_path, const char *log_path, int mode, int max_steps);
+/* tl_run_deck that also writes the reference's tea.out lines around the solve
+ * to tea_out_path (may be null): the step line (timestep.f90:58), the solve
+ * summary (tea_solve.f90:201-203, 356-371), the field-summary block and the
+ * known-answer verdict (field_summary.f90:41-104) with the reference's
+ * Fortran edit descriptors, and the per-step wall clock. The reference's
+ * input echo, decomposition and profiler tables are not written. */
+int tl_run_deck_out(const char *deck_path, const char *log_path, const char *tea_out_path, int mode,
+                    int max_steps);
 
 /* Timed run for benchmarks: builds the deck's chunk, runs `warmup` then
  * `steps` timesteps with the given mode and reports wall seconds of the timed

@@ -108,6 +108,7 @@ _SIGS = {
     "tl_synchronize": (_I, [_P]),
     "tl_print_profiling": (_I, [_P]),
     "tl_run_deck": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
+    "tl_run_deck_out": (_I, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, _I, _I]),
     "tl_bench_deck": (_I, [ctypes.c_char_p, _I, _I, _I, ctypes.POINTER(BenchResult)]),
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
@@ -410,10 +411,13 @@ def parse_log(path):
     return out
 
 
-def run_deck(deck, log_path, mode=MODE_NATIVE, max_steps=0):
-    """Run a whole tea.in deck on the GPU; returns the parsed log."""
+def run_deck(deck, log_path, mode=MODE_NATIVE, max_steps=0, tea_out=None):
+    """Run a whole tea.in deck on the GPU; returns the parsed log. With
+    `tea_out`, the reference's tea.out lines are written there too."""
     lib = load_library()
-    _check(lib.tl_run_deck(str(deck).encode(), str(log_path).encode(), int(mode), int(max_steps)), "tl_run_deck")
+    _check(lib.tl_run_deck_out(str(deck).encode(), str(log_path).encode(),
+                               str(tea_out).encode() if tea_out else None, int(mode), int(max_steps)),
+           "tl_run_deck")
     return parse_log(log_path)
 
 

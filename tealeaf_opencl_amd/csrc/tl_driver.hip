@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <cmath>
 #include <functional>
 #include <string>
 #include <vector>
@@ -53,7 +54,96 @@ const double kat_temp[7] = {0.0, 157.550841832793, 116.067951160930, 95.48651033
         if (rc_ != TL_OK) return rc_;  \
     } while (0)
 
-void emit_summary(FILE *lg, tl_chunk *c, int step, bool complete, const Config &cfg, double *out5)
+// ---------------------------------------------------------------------------
+// tea.out: the lines the reference writes to unit g_out around the solve, with
+// its Fortran edit descriptors (field_summary.f90:41-81, tea_solve.f90:
+// 201-203, 356-371, timestep.f90:58, diffuse.f90:69). The library is not
+// re-entrant (one in-order stream, like the reference), so the open file of
+// the current run is a file-scope handle.
+// ---------------------------------------------------------------------------
+FILE *g_tout = nullptr;
+
+std::string right(const std::string &s, int w)
+{
+    if ((int)s.size() >= w) return (int)s.size() == w ? s : std::string(w, '*');
+    return std::string(w - s.size(), ' ') + s;
+}
+
+// Ew.d (no scale factor): [-]0.d1..ddE+ee, or [-]0.d1..dd+eee for |exp| > 99
+std::string fort_E(double v, int w, int d)
+{
+    char buf[64];
+    std::string mant, sign = (v < 0 || (v == 0 && std::signbit(v))) ? "-" : "";
+    int ex = 0;
+    if (v == 0.0) {
+        mant = std::string(d, '0');
+    } else {
+        snprintf(buf, sizeof buf, "%.*e", d - 1, fabs(v));  // m.mmm e+xx
+        std::string t(buf);
+        const size_t epos = t.find('e');
+        mant = t.substr(0, 1) + t.substr(2, epos - 2);
+        ex = atoi(t.c_str() + epos + 1) + 1;
+    }
+    char eb[16];
+    if (ex >= -99 && ex <= 99) snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    else snprintf(eb, sizeof eb, "%c%03d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    return right(sign + "0." + mant + eb, w);
+}
+
+// 1PEw.d: [-]d.dd..E+ee
+std::string fort_1PE(double v, int w, int d)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.*E", d, v);
+    std::string t(buf);
+    const size_t epos = t.find('E');
+    const int ex = atoi(t.c_str() + epos + 1);
+    char eb[16];
+    snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    return right(t.substr(0, epos) + eb, w);
+}
+
+std::string fort_F(double v, int w, int d)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.*f", d, v);
+    return right(buf, w);
+}
+
+std::string fort_I(long v, int w) { return right(std::to_string(v), w); }
+
+// list-directed REAL(8) item (the leading blank of the record is written by
+// the caller)
+std::string fort_list(double v)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%25.16E", v);
+    return buf;
+}
+
+void tout(const std::string &line)
+{
+    if (g_tout) fprintf(g_tout, "%s\n", line.c_str());
+}
+
+// tea_solve.f90:356-371
+void tout_solve(double conduction, bool check, double exact, int iterations, bool ppcg, int inner)
+{
+    tout("Conduction error " + fort_E(conduction, 14, 7));
+    if (check) tout("EXACT error calculated as" + fort_E(exact, 14, 7));
+    tout("Iteration count " + fort_I(iterations, 8));
+    if (ppcg) tout("PPCG Iteration count" + fort_I(inner, 8) + " (Total " + fort_I(inner + iterations, 8) + ")");
+}
+
+// tea_solve.f90:201-203
+void tout_switch(int n, double rro, double eigmin, double eigmax, double cn, double old_error)
+{
+    tout("Switching after " + fort_I(n, 3) + " CG its, error " + fort_E(rro, 15, 7));
+    tout("Eigen min" + fort_E(eigmin, 14, 6) + " Eigen max" + fort_E(eigmax, 14, 6) + " Condition number" +
+         fort_F(cn, 14, 6) + " Error" + fort_E(old_error, 14, 6));
+}
+
+void emit_summary(FILE *lg, tl_chunk *c, int step, double time, bool complete, const Config &cfg, double *out5)
 {
     double vol, mass, ie, temp;
     if (tl_field_summary(c, &vol, &mass, &ie, &temp) != TL_OK) return;
@@ -61,9 +151,21 @@ void emit_summary(FILE *lg, tl_chunk *c, int step, bool complete, const Config &
         out5[0] = vol; out5[1] = mass; out5[2] = mass / vol; out5[3] = ie; out5[4] = temp;
     }
     if (lg) fprintf(lg, "SUMMARY %d %.17e %.17e %.17e %.17e %.17e\n", step, vol, mass, mass / vol, ie, temp);
-    if (complete && cfg.test_problem >= 1 && cfg.test_problem <= 6 && lg) {
+    // field_summary.f90:41-81
+    tout("");
+    tout(" Time " + fort_list(time));
+    tout(std::string(13, ' ') + right("Volume", 26) + right("Mass", 26) + right("Density", 26) + right("Energy", 26) +
+         right("U", 26));
+    tout(" step:" + fort_I(step, 7) + fort_E(vol, 26, 17) + fort_E(mass, 26, 17) + fort_E(mass / vol, 26, 17) +
+         fort_E(ie, 26, 17) + fort_E(temp, 26, 17));
+    tout("");
+    if (complete && cfg.test_problem >= 1 && cfg.test_problem <= 6) {
         const double qa = fabs((100.0 * (temp / kat_temp[cfg.test_problem])) - 100.0);
-        fprintf(lg, "QA %d %.7e %s\n", cfg.test_problem, qa, qa < 0.001 ? "PASSED" : "FAILED");
+        if (lg) fprintf(lg, "QA %d %.7e %s\n", cfg.test_problem, qa, qa < 0.001 ? "PASSED" : "FAILED");
+        // field_summary.f90:84-104 (the reference's wording, typo included)
+        tout("Test problem" + fort_I(cfg.test_problem, 4) + " is within" + fort_E(qa, 16, 7) +
+             "% of the expected solution");
+        tout(qa < 0.001 ? " This test is considered PASSED" : " This is test is considered NOT PASSED");
     }
 }
 
@@ -124,6 +226,7 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
                 if (lg)
                     fprintf(lg, "SWITCH %d %d %.17e %.17e %.17e %.17e %.17e\n", step, n, rro, sv.eigmin, sv.eigmax,
                             sv.cn, old_error);
+                tout_switch(n, rro, sv.eigmin, sv.eigmax, sv.cn, old_error);
             }
             if (use_ch) {
                 if (cheby_calc_steps == 0) {
@@ -192,6 +295,8 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         fprintf(lg, "SOLVE %d %d %.17e %.17e %d %d %d %d %d\n", step, n - 1, error / initial_residual,
                 cfg.tl_check_result ? exact_error / initial_residual : NAN, ppcg_inner_iters, cg_calc_steps,
                 cheby_calc_steps, switch_n, est_itc);
+    tout_solve(error / initial_residual, cfg.tl_check_result, exact_error / initial_residual, n - 1,
+               cfg.solver == SOLVER_PPCG, ppcg_inner_iters);
     if (stats) {
         stats->iterations += n > max_iters ? max_iters : n;
         stats->inner += ppcg_inner_iters;
@@ -438,6 +543,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         if (lg)
             fprintf(lg, "SWITCH %d %d %.17e %.17e %.17e %.17e %.17e\n", step, n, h.rro, sv.eigmin, sv.eigmax, sv.cn,
                     h.old_error);
+        tout_switch(n, h.rro, sv.eigmin, sv.eigmax, sv.cn, h.old_error);
         if (use_ch) {
             // tea_leaf_cheby_first_step (tea_solve.f90:477-521)
             double bb, err1;
@@ -509,6 +615,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
     if (lg)
         fprintf(lg, "SOLVE %d %d %.17e %.17e %d %d %d %d %d\n", step, h.exit_n - 1, h.error / h.initial_residual,
                 exact, ppcg_inner, h.cg_calc_steps, h.cheby_calc_steps, h.switch_n, est_itc);
+    tout_solve(h.error / h.initial_residual, cfg.tl_check_result, exact, h.exit_n - 1, use_ppcg, ppcg_inner);
     if (stats) {
         stats->iterations += executed;
         stats->inner += ppcg_inner;
@@ -549,6 +656,7 @@ int solve_refined(tl_chunk *c, const Config &cfg, double dt, int step, FILE *lg,
         fprintf(lg, "REFINE %d %d %d %d %.17e %.9e\n", step, rr.iterations, rr.updates, rr.converged,
                 rr.rel_residual, rr.seconds);
     }
+    tout_solve(rr.rel_residual, false, 0.0, rr.iterations, cfg.solver == SOLVER_PPCG, rr.inner_iterations);
     if (stats) {
         stats->iterations += rr.iterations;
         stats->inner += rr.inner_iterations;
@@ -596,12 +704,13 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
     c->solver = cfg.sc.solver;
     c->timing = (warmup >= 0);
     int rc = start(c, cfg);
-    if (rc == TL_OK) emit_summary(lg, c, 0, false, cfg, nullptr);
+    if (rc == TL_OK) emit_summary(lg, c, 0, 0.0, false, cfg, nullptr);
     Save sv;
     double time = 0.0, dt = cfg.dtinit;
     const double g_small = (double)1.0e-16f;
     int step = 0;
     std::chrono::steady_clock::time_point t0;
+    const auto wall0 = std::chrono::steady_clock::now();
     Stats stats;
     while (rc == TL_OK) {
         step++;
@@ -613,22 +722,29 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
             t0 = std::chrono::steady_clock::now();
         }
         if (cfg.dtinit <= dt) dt = cfg.dtinit;
+        // timestep.f90:58 (the format's trailing i8 has no item)
+        tout(" Step " + fort_I(step, 7) + " time " + fort_F(time, 11, 7) + " timestep  " + fort_1PE(dt, 9, 2));
         rc = cfg.refine ? solve_refined(c, cfg, dt, step, lg, &stats)
              : (mode == TL_MODE_NATIVE) ? solve_native(c, cfg, sv, dt, step, lg, &stats)
                                         : solve_dropin(c, cfg, sv, dt, step, lg, &stats);
         if (rc != TL_OK) break;
         time = time + dt;
+        if (g_tout) {
+            // diffuse.f90:66-69 (list-directed)
+            const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+            tout(" Wall clock " + fort_list(wall));
+        }
         bool done = (time + g_small > cfg.end_time || step >= cfg.end_step);
         if (max_steps > 0 && step >= max_steps) done = true;
         if (cfg.summary_frequency != 0 && step % cfg.summary_frequency == 0)
-            emit_summary(lg, c, step, false, cfg, nullptr);
+            emit_summary(lg, c, step, time, false, cfg, nullptr);
         if (done) {
             if (warmup >= 0 && out) {
                 hipDeviceSynchronize();
                 out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 out->stats = stats;
             }
-            emit_summary(lg, c, step, true, cfg, out ? out->summary : nullptr);
+            emit_summary(lg, c, step, time, true, cfg, out ? out->summary : nullptr);
             break;
         }
     }
@@ -644,7 +760,8 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
 
 }  // namespace
 
-extern "C" int tl_run_deck(const char *deck_path, const char *log_path, int mode, int max_steps)
+extern "C" int tl_run_deck_out(const char *deck_path, const char *log_path, const char *tea_out_path, int mode,
+                               int max_steps)
 {
     Config cfg;
     std::string err;
@@ -660,9 +777,33 @@ extern "C" int tl_run_deck(const char *deck_path, const char *log_path, int mode
             return TL_ERR_ARG;
         }
     }
+    if (tea_out_path && tea_out_path[0]) {
+        g_tout = fopen(tea_out_path, "w");
+        if (!g_tout) {
+            if (lg) fclose(lg);
+            set_error("cannot open %s", tea_out_path);
+            return TL_ERR_ARG;
+        }
+        tout("");
+        tout(" Starting the calculation");
+    }
     const int rc = run(cfg, lg, mode, max_steps, -1, nullptr);
+    if (g_tout) {
+        if (rc == TL_OK) {
+            tout("");
+            tout(" Calculation complete");
+            tout(" Tea is finishing");
+        }
+        fclose(g_tout);
+        g_tout = nullptr;
+    }
     if (lg) fclose(lg);
     return rc;
+}
+
+extern "C" int tl_run_deck(const char *deck_path, const char *log_path, int mode, int max_steps)
+{
+    return tl_run_deck_out(deck_path, log_path, nullptr, mode, max_steps);
 }
 
 extern "C" int tl_bench_deck(const char *deck_path, int mode, int warmup_steps, int steps, tl_bench_result *out)

@@ -196,3 +196,36 @@ def config4_density(n, seed=1, hd=1, sigma=1.0):
     rho = np.zeros((n + 2 * hd, n + 2 * hd))
     rho[hd:hd + n, hd:hd + n] = 10.0 ** (-1.0 + 3.0 * u) * bump
     return rho
+
+
+def config5_states(seed=1, n_rect=8, rect_energy=25.0):
+    """Background (density 100, energy 1e-4) and `n_rect` seeded rectangles
+    (area fraction U[0.05,0.20], aspect U[0.5,2], lower-left corner uniform
+    over the positions where the rectangle fits), each with its own density
+    10^U[-1,2] and the given energy."""
+    rng = SplitMix64(seed)
+    states = [dict(density=100.0, energy=1.0e-4)]
+    for _ in range(n_rect):
+        f = rng.uniform(0.05, 0.20)
+        a = rng.uniform(0.5, 2.0)
+        w = math.sqrt(100.0 * f * a)
+        h = math.sqrt(100.0 * f / a)
+        x0 = rng.uniform(0.0, 10.0 - min(w, 10.0))
+        y0 = rng.uniform(0.0, 10.0 - min(h, 10.0))
+        rho = 10.0 ** rng.uniform(-1.0, 2.0)
+        states.append(dict(density=rho, energy=rect_energy, geometry="rectangle",
+                           xmin=x0, xmax=min(x0 + w, 10.0), ymin=y0, ymax=min(y0 + h, 10.0)))
+    return states
+
+
+def config5_deck(path, precision="reference", n=8192, steps=20, seed=1, eps=1e-10, inner=10, presteps=20, **kw):
+    """The large combined run: PPCG + block Jacobi on config5_states, in the
+    reference FP64 PPCG ("reference") or the refined FP32-inner PPCG
+    ("mixed")."""
+    ex = ["tl_ppcg_inner_steps=%d" % inner, "tl_ch_cg_presteps=%d" % presteps, "tl_ch_cg_epslim=1.0e300"]
+    if precision == "mixed":
+        ex.append("tl_mixed_precision")
+    elif precision != "reference":
+        raise ValueError("precision must be reference or mixed")
+    return write_deck(path, n, n, config5_states(seed), solver="tl_use_ppcg", preconditioner="jac_block",
+                      end_step=steps, eps=eps, extra=ex, **kw)

@@ -116,3 +116,38 @@ def test_reference_kat_large(hiplib, decks_dir, tmp_path, kat):
     """960^2 (87 steps) and 3840^2 (10 steps) CG known-answer tests, native mode."""
     lg = hiplib.run_deck(os.path.join(decks_dir, "kat%d.in" % kat), tmp_path / "k.log", mode=1)
     assert lg["qa"][2] == "PASSED", lg["qa"]
+
+
+def test_tea_out_lines(hiplib, tmp_path):
+    """tl_run_deck_out writes the reference's tea.out lines with its Fortran
+    edit descriptors: the step line (timestep.f90:58), the solve summary
+    (tea_solve.f90:356-371) and the field-summary block with the known-answer
+    verdict (field_summary.f90:41-104), KAT test 1."""
+    import re
+    deck = os.path.join(os.path.dirname(__file__), "golden", "decks", "kat1.in")
+    out = tmp_path / "tea.out"
+    lg = hiplib.run_deck(deck, str(tmp_path / "g.log"), tea_out=str(out))
+    lines = out.read_text().splitlines()
+    e2617 = r"( {2}-0\.\d{17}E[+-]\d{2}| {3}0\.\d{17}E[+-]\d{2})"
+    steps = [l for l in lines if l.startswith(" step:")]
+    assert len(steps) == len(lg["summary"])
+    for l, s in zip(steps, lg["summary"]):
+        assert re.fullmatch(r" step:[ \d]{7}" + e2617 * 5, l), l
+        assert int(l[6:13]) == s[0]
+        vals = [float(l[13 + 26 * i:13 + 26 * (i + 1)]) for i in range(5)]
+        assert all(abs(a - b) <= 1e-16 * abs(b) for a, b in zip(vals, s[1:])), (vals, s)
+    heads = [l for l in lines if "Volume" in l]
+    assert heads and all(l == " " * 13 + "%26s%26s%26s%26s%26s" % ("Volume", "Mass", "Density", "Energy", "U")
+                         for l in heads)
+    step_lines = [l for l in lines if l.startswith(" Step ")]
+    assert len(step_lines) == len(lg["solve"])
+    assert re.fullmatch(r" Step       1 time   0\.0000000 timestep   \d\.\d\dE[+-]\d\d", step_lines[0]), step_lines[0]
+    cond = [l for l in lines if l.startswith("Conduction error ")]
+    assert len(cond) == len(lg["solve"])
+    assert all(re.fullmatch(r"Conduction error  ?-?0\.\d{7}E[+-]\d{2}", l) for l in cond), cond[:2]
+    its = [int(l[16:]) for l in lines if l.startswith("Iteration count ")]
+    assert its == [s[1] for s in lg["solve"]]
+    qa = [l for l in lines if l.startswith("Test problem")]
+    assert len(qa) == 1 and re.fullmatch(r"Test problem   1 is within {2,3}-?0\.\d{7}E[+-]\d{2}% of the expected solution",
+                                         qa[0]), qa
+    assert " This test is considered PASSED" in lines
