@@ -17,6 +17,7 @@ import json
 import os
 import sys
 import tempfile
+import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -56,8 +57,10 @@ def parse():
                     help="run the large combined workload (8192^2 PPCG FP64 vs mixed, adjoint) instead")
     ap.add_argument("--large-n", type=int, default=8192)
     ap.add_argument("--large-steps", type=int, default=20)
-    ap.add_argument("--large-fd", type=int, default=2, help="finite-difference cells of the large adjoint")
+    ap.add_argument("--large-fd", type=int, default=1, help="finite-difference cells of the large adjoint")
     ap.add_argument("--large-skip", default="", help="comma list of large legs to skip: fp64,mixed,adjoint")
+    ap.add_argument("--large-out", default="", help="directory for the large legs' decks, logs and tea.out")
+    ap.add_argument("--large-tol", type=float, default=1e-12, help="solve tolerance of the large adjoint")
     return ap.parse_args()
 
 
@@ -89,6 +92,13 @@ def refine_leg(n, tol=1e-10):
     summaries = {}
     for name, precision in (("fp64", tl.PRECISION_FP64), ("mixed", tl.PRECISION_MIXED)):
         c = tl.Chunk(n, n, 1, tl.PREC_JAC_BLOCK)
+        t_start = time.perf_counter()
+
+        def progress(phase, step, its):
+            print("large adjoint leg: %s solve of step %d done (%d iterations, %.0f s)"
+                  % (phase, step, its, time.perf_counter() - t_start), file=sys.stderr, flush=True)
+
+        c.set_progress(progress)
         try:
             dx = 10.0 / float(np.float32(n))
             c.initialise_chunk(0.0, 0.0, dx, dx)
@@ -138,9 +148,11 @@ def large_leg(args):
     `steps` timesteps, once with the reference FP64 PPCG (device-resident
     driver) and once with the refined FP32-inner PPCG to a true relative
     residual of 1e-10; then the density adjoint over the same timesteps
-    (conductivity = density, CG + block Jacobi solves to 1e-10, checkpoints
-    every 5 steps) with central differences (h = 1e-6 relative) on a few
-    sensitive cells."""
+    (conductivity = density, CG + block Jacobi solves to --large-tol, checkpoints
+    every 5 steps) with central differences on a few sensitive cells
+    (h = 1e-6 and 1e-3 relative) and along one random direction over every
+    cell (1e-6). The deck legs are timed around the whole deck run
+    (chunk set-up included) by the host clock."""
     import numpy as np
     import tealeaf_opencl_amd as tl
     from tealeaf_opencl_amd import adjoint, decks
@@ -150,16 +162,23 @@ def large_leg(args):
     out = {"workload": "%dx%d, 8 seeded rectangles (density 10^U[-1,2], seed 1), PPCG + strip-4 block Jacobi, "
                        "10 inner steps, dt 0.004, %d timesteps" % (n, n, steps), "cells": n * n}
     legs = {}
-    with tempfile.TemporaryDirectory() as d:
-        for name in ("fp64", "mixed"):
-            if name in skip:
-                continue
-            deck = decks.config5_deck(os.path.join(d, name + ".in"), precision="reference" if name == "fp64" else "mixed",
-                                      n=n, steps=steps, summary_frequency=0)
-            r = tl.bench_deck(deck, mode=tl.MODE_NATIVE, warmup=0, steps=steps)
-            legs[name] = {"seconds": r["seconds"], "outer_iterations": r["iterations"],
-                          "inner_iterations": r["inner_iterations"], "final_summary": r["final_summary"]}
-            print("large %s: %.3f s" % (name, r["seconds"]), file=sys.stderr, flush=True)
+    outdir = args.large_out or tempfile.mkdtemp()
+    os.makedirs(outdir, exist_ok=True)
+    for name in ("fp64", "mixed"):
+        if name in skip:
+            continue
+        deck = decks.config5_deck(os.path.join(outdir, name + ".in"), precision="reference" if name == "fp64" else "mixed",
+                                  n=n, steps=steps, summary_frequency=steps)
+        # the whole deck run (chunk set-up and start included), with its log
+        # and tea.out written step by step
+        t0 = time.perf_counter()
+        lg = tl.run_deck(deck, os.path.join(outdir, name + ".log"), mode=tl.MODE_NATIVE,
+                         tea_out=os.path.join(outdir, name + ".tea.out"))
+        secs = time.perf_counter() - t0
+        legs[name] = {"seconds": secs, "outer_iterations": int(sum(x[1] for x in lg["solve"])),
+                      "inner_iterations": int(sum(x[4] for x in lg["solve"])),
+                      "final_summary": lg["summary"][-1][1:]}
+        print("large %s: %.3f s" % (name, secs), file=sys.stderr, flush=True)
     out.update(legs)
     if "fp64" in legs and "mixed" in legs:
         a, b = np.array(legs["fp64"]["final_summary"]), np.array(legs["mixed"]["final_summary"])
@@ -167,38 +186,59 @@ def large_leg(args):
         out["summary_rel_diff_mixed_vs_fp64"] = dict(zip(["volume", "mass", "density", "internal_energy",
                                                           "temperature"], (np.abs(b - a) / np.abs(a)).tolist()))
     if "adjoint" not in skip:
-        dt, coef, tol, every = 0.004, tl.COEF_CONDUCTIVITY, 1e-10, 5
+        dt, coef, tol, every = 0.004, tl.COEF_CONDUCTIVITY, args.large_tol, 5
         c = tl.Chunk(n, n, 1, tl.PREC_JAC_BLOCK)
+        t_start = time.perf_counter()
+
+        def progress(phase, step, its):
+            print("large adjoint leg: %s solve of step %d done (%d iterations, %.0f s)"
+                  % (phase, step, its, time.perf_counter() - t_start), file=sys.stderr, flush=True)
+
+        c.set_progress(progress)
         try:
             dx = 10.0 / float(np.float32(n))
             c.initialise_chunk(0.0, 0.0, dx, dx)
             c.generate_chunk(*decks.state_arrays(decks.config5_states(2), n, n))
             c.set_field()
             e0 = c.get_field("energy1")
-            u_obs = adjoint.forward_u(c, coef, dt, steps, tol)
+            u_obs = adjoint.forward_u(c, coef, dt, steps, tol, progress=progress)
             c.generate_chunk(*decks.state_arrays(decks.config5_states(1), n, n))
             c.set_field()
             rho, e0 = c.get_field("density"), c.get_field("energy1")
             L, grad, info = c.adjoint_density(coef, dt, steps, u_obs, tol=tol, checkpoint_every=every)
             print("large adjoint: forward %.3f s, adjoint %.3f s" % (info["forward_s"], info["adjoint_s"]),
                   file=sys.stderr, flush=True)
+            def loss_at(r):
+                c.put_field("density", r)
+                c.put_field("energy1", e0)
+                return c.forward_loss(coef, dt, steps, u_obs, tol=tol)
+
+            # per cell: central differences with h = 1e-6 rho_c and, to show
+            # where their error comes from, with h = 1e-3 rho_c (truncation
+            # error ~h^2 is still far below 1e-6 there, while the forward
+            # solves' rounding noise in L is divided by a 1000x larger step)
             fd = []
             s_ = np.abs(rho * grad)
             ks, js = np.nonzero(s_ >= 1e-2 * s_.max())
             rng = np.random.default_rng(7)
             for i in rng.choice(len(ks), size=min(args.large_fd, len(ks)), replace=False):
                 k, j = int(ks[i]), int(js[i])
-                h = 1e-6 * rho[k, j]
-                vals = []
-                for sg in (1.0, -1.0):
-                    r2 = rho.copy()
-                    r2[k, j] += sg * h
-                    c.put_field("density", r2)
-                    c.put_field("energy1", e0)
-                    vals.append(c.forward_loss(coef, dt, steps, u_obs, tol=tol))
-                fdv = (vals[0] - vals[1]) / (2 * h)
-                fd.append({"cell": [k, j], "fd": fdv, "adjoint": float(grad[k, j]),
-                           "rel_err": abs(fdv - grad[k, j]) / abs(grad[k, j])})
+                cell = {"cell": [k, j], "adjoint": float(grad[k, j])}
+                for hr in (1e-6, 1e-3):
+                    h = hr * rho[k, j]
+                    r1, r2 = rho.copy(), rho.copy()
+                    r1[k, j] += h
+                    r2[k, j] -= h
+                    fdv = (loss_at(r1) - loss_at(r2)) / (2 * h)
+                    cell["rel_err_h%.0e" % hr] = abs(fdv - grad[k, j]) / abs(grad[k, j])
+                print("large fd cell %d %d done" % (k, j), file=sys.stderr, flush=True)
+                fd.append(cell)
+            # along a random direction v = rho * N(0,1) over every cell, eps 1e-6
+            v = np.zeros_like(rho)
+            v[1:-1, 1:-1] = np.random.default_rng(3).standard_normal((n, n)) * rho[1:-1, 1:-1]
+            dd = float(np.sum(grad * v))
+            fdd = (loss_at(rho + 1e-6 * v) - loss_at(rho - 1e-6 * v)) / 2e-6
+            directional = {"eps": 1e-6, "fd": fdd, "adjoint": dd, "rel_err": abs(fdd - dd) / abs(dd)}
             out["adjoint"] = {"loss": L, "forward_s": info["forward_s"], "adjoint_s": info["adjoint_s"],
                               "adjoint_over_forward": info["adjoint_s"] / info["forward_s"],
                               "checkpoint_every": every, "checkpoints": info["checkpoints"],
@@ -207,7 +247,8 @@ def large_leg(args):
                               "iterations": {"forward": info["forward_iterations"],
                                              "recompute": info["recompute_iterations"],
                                              "adjoint": info["adjoint_iterations"]},
-                              "stalled_solves": info["stalled_solves"], "finite_differences": fd}
+                              "stalled_solves": info["stalled_solves"], "finite_differences": fd,
+                              "directional_derivative": directional}
         finally:
             c.close()
     return out

@@ -289,6 +289,12 @@ typedef struct {
 } tl_adjoint_result;
 int tl_forward_loss(tl_chunk *c, int coefficient, double dt, int nsteps, double tol, int max_iters,
                     const double *u_obs, double *loss);
+/* Progress of long calls (no reference counterpart): once set, tl_forward_loss
+ * and tl_adjoint_density call fn(user, phase, step, iterations) after every
+ * solve, with phase "forward", "recompute" or "adjoint", the timestep and the
+ * solve's iterations. fn may be null to switch it off. */
+typedef void (*tl_progress_fn)(void *user, const char *phase, int step, int iterations);
+int tl_set_progress(tl_chunk *c, tl_progress_fn fn, void *user);
 int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int nsteps, double tol, int max_iters,
                        int checkpoint_every, const double *u_obs, double *grad, tl_adjoint_result *out);
 

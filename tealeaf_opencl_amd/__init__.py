@@ -113,6 +113,7 @@ _SIGS = {
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
     "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
+    "tl_set_progress": (_I, [_P, _P, _P]),
     "tl_solve_refined_ppcg": (_I, [_P, _I, _D, _I, _D, _I, _I, ctypes.POINTER(RefineResult)]),
     "tl_adjoint_density": (_I, [_P, _I, _D, _I, _D, _I, _I, _DP, _DP, ctypes.POINTER(AdjointResult)]),
 }
@@ -319,6 +320,19 @@ class Chunk:
 
     def synchronize(self):
         _check(self.lib.tl_synchronize(self.h), "synchronize")
+
+    _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int)
+
+    def set_progress(self, fn):
+        """Call fn(phase, step, iterations) after every solve of the long
+        calls (forward_loss, adjoint_density); None switches it off."""
+        if fn is None:
+            self._progress_cb = None
+            _check(self.lib.tl_set_progress(self.h, None, None), "set_progress")
+            return
+        self._progress_cb = self._PROGRESS(lambda user, phase, step, its: fn(phase.decode(), step, its))
+        _check(self.lib.tl_set_progress(self.h, ctypes.cast(self._progress_cb, ctypes.c_void_p), None),
+               "set_progress")
 
     def stream_bandwidth(self, kind=1, reps=20):
         """Attainable streaming bytes/s on this chunk's layout (kind 0 copy,

@@ -13,16 +13,19 @@ import numpy as np
 from . import (COEF_CONDUCTIVITY, PRECISION_FP64, Chunk, decks)
 
 
-def forward_u(chunk, coefficient, dt, nsteps, tol=1e-12):
+def forward_u(chunk, coefficient, dt, nsteps, tol=1e-12, progress=None):
     """u after `nsteps` timesteps from the chunk's energy1 (tl_common_init,
     tl_solve_refined in FP64, tl_finalise per step); a solve that stops at its
-    rounding floor above tol is accepted, as in the library's adjoint."""
-    for _ in range(nsteps):
+    rounding floor above tol is accepted, as in the library's adjoint.
+    progress(phase, step, iterations) is called after every solve."""
+    for step in range(1, nsteps + 1):
         chunk.common_init(coefficient, dt)
         res = chunk.solve_refined(PRECISION_FP64, tol=tol, max_iters=20000)
         if not (res["converged"] or res["stalled"]):
             raise RuntimeError("forward solve did not converge: %r" % (res,))
         chunk.finalise()
+        if progress:
+            progress("forward", step, res["iterations"])
     return chunk.get_field("u")
 
 

@@ -199,7 +199,7 @@ struct Counts {
 // One FP64 refined solve; a solve that stops at the rounding floor of the
 // true residual above tol (stalled) is accepted and counted, one that runs
 // out of iterations is an error.
-int solve(tl_chunk *c, double tol, int max_iters, long *iterations, Counts *cnt)
+int solve(tl_chunk *c, double tol, int max_iters, long *iterations, Counts *cnt, const char *phase, int step)
 {
     tl_refine_result rr;
     RC(tl_solve_refined(c, TL_PRECISION_FP64, tol, max_iters, 0.0, &rr));
@@ -210,6 +210,7 @@ int solve(tl_chunk *c, double tol, int max_iters, long *iterations, Counts *cnt)
     }
     cnt->stalled += rr.stalled;
     *iterations += rr.iterations;
+    if (c->progress) c->progress(c->progress_user, phase, step, rr.iterations);
     return TL_OK;
 }
 
@@ -223,7 +224,7 @@ int forward(tl_chunk *c, int coef, double dt, int nsteps, double tol, int max_it
         double rx, ry;
         RC(tl_common_init(c, coef, dt, &rx, &ry, zb, 0));
         if (ck && n % every == 0) copy_interior(c, c->f2[F_U0], ck[n / every]);
-        RC(solve(c, tol, max_iters, &cnt->forward, cnt));
+        RC(solve(c, tol, max_iters, &cnt->forward, cnt, "forward", n + 1));
         RC(tl_finalise(c));
     }
     return TL_OK;
@@ -330,7 +331,7 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
         for (int m = s; m < end; ++m) {
             adj::copy_interior(c, prev, c->f2[F_U0]);
             adj::copy_interior(c, prev, c->f2[F_U]);
-            RC(adj::solve(c, tol, max_iters, &cnt.recompute, &cnt));
+            RC(adj::solve(c, tol, max_iters, &cnt.recompute, &cnt, "recompute", m + 1));
             adj::copy_interior(c, c->f2[F_U], seg[m - s]);
             prev = seg[m - s];
         }
@@ -338,7 +339,7 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
         for (int n = end - 1; n >= s; --n) {
             adj::copy_interior(c, gv, c->f2[F_U0]);
             hipMemsetAsync(c->base2[F_U], 0, sizeof(double) * c->elems2d, c->s);
-            RC(adj::solve(c, tol, max_iters, &cnt.adjoint, &cnt));
+            RC(adj::solve(c, tol, max_iters, &cnt.adjoint, &cnt, "adjoint", n + 1));
             hipLaunchKernelGGL(adj::k_adj_face_grad, dim3(adj::grid_of(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s,
                                c->g, e, c->f2[F_U], seg[n - s], c->f2[F_COND], c->faces, coefficient, gr);
             adj::copy_interior(c, c->f2[F_U], gv);
@@ -370,5 +371,16 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
         out->checkpoints = nck;
         out->stalled_solves = cnt.stalled;
     }
+    return TL_OK;
+}
+
+extern "C" int tl_set_progress(tl_chunk *c, tl_progress_fn fn, void *user)
+{
+    if (!c) {
+        set_error("tl_set_progress: null chunk");
+        return TL_ERR_ARG;
+    }
+    c->progress = fn;
+    c->progress_user = user;
     return TL_OK;
 }

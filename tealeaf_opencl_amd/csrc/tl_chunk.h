@@ -42,6 +42,9 @@ struct tl_chunk {
     double rx = 0, ry = 0, theta = 0, dx = 0, dy = 0;
     // face parameters of the latest common_init (F_COND holds its conductivity)
     tl::Faces faces{};
+    // progress callback of long calls (tl_set_progress)
+    void (*progress)(void *, const char *, int, int) = nullptr;
+    void *progress_user = nullptr;
 
     // profiler_on: per-kernel device time (ms) and calls (ocl_errors.cpp:64-121)
     std::map<std::string, std::pair<double, long>> prof;

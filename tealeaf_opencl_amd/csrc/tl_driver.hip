@@ -733,7 +733,9 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
             // diffuse.f90:66-69 (list-directed)
             const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
             tout(" Wall clock " + fort_list(wall));
+            fflush(g_tout);
         }
+        if (lg) fflush(lg);
         bool done = (time + g_small > cfg.end_time || step >= cfg.end_step);
         if (max_steps > 0 && step >= max_steps) done = true;
         if (cfg.summary_frequency != 0 && step % cfg.summary_frequency == 0)

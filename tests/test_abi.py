@@ -19,6 +19,8 @@ def declared_functions():
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, flags=re.M):
             name = m.group(1)
+            if m.group(0).startswith("typedef"):
+                continue  # a function-pointer type, not a symbol
             if name not in ("if", "while", "for", "sizeof", "typedef"):
                 names.add(name)
     return names

@@ -145,3 +145,22 @@ def test_bad_arguments(hiplib):
             c.adjoint_density(**args)
     with pytest.raises(ValueError):
         c.adjoint_density(1, 0.004, 2, u_obs[:-1], tol=1e-12)
+
+
+def test_progress_callback(hiplib):
+    x, y, nsteps = 48, 40, 4
+    rho, e0, u_obs = random_problem(x, y, 9)
+    c = gpu_chunk(hiplib, rho, e0, x, y, 2)
+    seen = []
+    c.set_progress(lambda phase, step, its: seen.append((phase, step, its)))
+    c.adjoint_density(1, 0.004, nsteps, u_obs, tol=1e-12, checkpoint_every=2)
+    phases = [p for p, _, _ in seen]
+    assert phases.count("forward") == nsteps
+    assert phases.count("recompute") == nsteps and phases.count("adjoint") == nsteps
+    assert [s for p, s, _ in seen if p == "adjoint"] == [4, 3, 2, 1]
+    assert all(its > 0 for _, _, its in seen)
+    c.set_progress(None)
+    seen.clear()
+    c.put_field("energy1", e0)
+    c.forward_loss(1, 0.004, 2, u_obs)
+    assert not seen
