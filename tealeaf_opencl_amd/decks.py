@@ -220,12 +220,15 @@ def config5_states(seed=1, n_rect=8, rect_energy=25.0):
 
 def config5_deck(path, precision="reference", n=8192, steps=20, seed=1, eps=1e-10, inner=10, presteps=20, **kw):
     """The large combined run: PPCG + block Jacobi on config5_states, in the
-    reference FP64 PPCG ("reference") or the refined FP32-inner PPCG
-    ("mixed")."""
+    reference FP64 PPCG ("reference", its own exit test), or the refined PPCG
+    to a true relative residual of eps with FP64 ("fp64") or FP32 ("mixed")
+    inner fields."""
     ex = ["tl_ppcg_inner_steps=%d" % inner, "tl_ch_cg_presteps=%d" % presteps, "tl_ch_cg_epslim=1.0e300"]
     if precision == "mixed":
         ex.append("tl_mixed_precision")
+    elif precision == "fp64":
+        ex.append("tl_fp64_refined")
     elif precision != "reference":
-        raise ValueError("precision must be reference or mixed")
+        raise ValueError("precision must be reference, fp64 or mixed")
     return write_deck(path, n, n, config5_states(seed), solver="tl_use_ppcg", preconditioner="jac_block",
                       end_step=steps, eps=eps, extra=ex, **kw)
