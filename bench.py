@@ -32,6 +32,10 @@ ALG_BYTES = {
     "k_fused_cg_w": 4 * 8,
     # u += a p; r -= a A p; z = M_blk^-1 r; r.z : R u, p, r, w  W u, r, z
     "k_fused_cg_ur": 7 * 8,
+    # r = u0 - A u; p = ca p + cb M_blk^-1 r; u' = u + p : R u, u0, w, p  W p, u'
+    "k_fused_cheby": 6 * 8,
+    # r -= A sd; u += sd; sd' = ca sd + cb M_blk^-1 r : R u, r, w, sd  W u, r, sd'
+    "k_fused_ppcg_inner": 7 * 8,
 }
 # the refined solve's inner kernels (tl_refine.hip): same streams as the fused
 # FP64 kernels, with r, z, p, d and the face field stored in FP64 or FP32
@@ -53,6 +57,7 @@ def parse():
     ap.add_argument("--mode", type=int, default=1, help="1 native (default), 0 drop-in")
     ap.add_argument("--max-iters", type=int, default=5000, help="tl_max_iters of the deck (short profiles)")
     ap.add_argument("--no-e10", action="store_true", help="skip the solve-to-1e-10 leg")
+    ap.add_argument("--no-solvers", action="store_true", help="skip the Chebyshev and PPCG leg")
     ap.add_argument("--large", action="store_true",
                     help="run the large combined workload (8192^2 PPCG FP64 vs mixed, adjoint) instead")
     ap.add_argument("--large-n", type=int, default=8192)
@@ -93,13 +98,6 @@ def refine_leg(n, tol=1e-10):
     summaries = {}
     for name, precision in (("fp64", tl.PRECISION_FP64), ("mixed", tl.PRECISION_MIXED)):
         c = tl.Chunk(n, n, 1, tl.PREC_JAC_BLOCK)
-        t_start = time.perf_counter()
-
-        def progress(phase, step, its):
-            print("large adjoint leg: %s solve of step %d done (%d iterations, %.0f s)"
-                  % (phase, step, its, time.perf_counter() - t_start), file=sys.stderr, flush=True)
-
-        c.set_progress(progress)
         try:
             dx = 10.0 / float(np.float32(n))
             c.initialise_chunk(0.0, 0.0, dx, dx)
@@ -140,6 +138,40 @@ def refine_leg(n, tol=1e-10):
                                                   rel.tolist())),
         "kernels": kernels,
     }
+
+
+def solvers_leg(n, d, steps=1, max_iters=5000):
+    """The other two solvers of the 4096^2 deck: Chebyshev after 20 CG
+    eigenvalue-estimate steps and PPCG with 10 inner steps, both with strip-4
+    block Jacobi, tl_eps 1e-15, run by the device-resident driver (one untimed
+    timestep, then `steps` timed ones). Bytes per iteration are algorithmic:
+    the fused Chebyshev iteration streams u, u0, w, p in and p, u out
+    (48 B/cell); a PPCG outer step is the two fused CG kernels (88 B/cell)
+    plus 10 inner steps of u, r, w, sd in and u, r, sd out (56 B/cell each)."""
+    import tealeaf_opencl_amd as tl
+    from tealeaf_opencl_amd import decks
+
+    out = {}
+    for name, solver in (("chebyshev", "tl_use_chebyshev"), ("ppcg", "tl_use_ppcg")):
+        deck = decks.config2_deck(os.path.join(d, name + ".in"), solver=solver, n=n, steps=1 + steps,
+                                  summary_frequency=0, max_iters=max_iters)
+        r = tl.bench_deck(deck, mode=tl.MODE_NATIVE, warmup=1, steps=steps)
+        its, inner, secs = r["iterations"], r["inner_iterations"], r["seconds"]
+        if name == "chebyshev":
+            # 20 CG steps (two fused CG kernels) then Chebyshev iterations
+            cg = 20 * steps
+            alg = (cg * (ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur"]) +
+                   (its - cg) * ALG_BYTES["k_fused_cheby"]) * n * n
+        else:
+            alg = (its * (ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur"]) +
+                   inner * ALG_BYTES["k_fused_ppcg_inner"]) * n * n
+        out[name] = {"s_per_step": secs / steps, "iterations_per_step": its / steps,
+                     "inner_iterations_per_step": inner / steps, "iterations_per_s": its / secs,
+                     "alg_GBs": alg / secs / 1e9, "alg_frac_of_peak": alg / secs / PEAK_HBM,
+                     "final_summary": r["final_summary"]}
+        print("solvers leg %s: %.3f s per step, %d iterations" % (name, secs / steps, its // steps),
+              file=sys.stderr, flush=True)
+    return out
 
 
 def large_leg(args):
@@ -289,6 +321,7 @@ def main():
         r = tl.bench_deck(deck, mode=args.mode, warmup=args.warmup, steps=args.steps)
         # solve seconds to 1e-10 (one timestep, true residual reported)
         r10 = None if args.no_e10 else refine_leg(n)
+        solvers = None if args.no_solvers else solvers_leg(n, d, max_iters=args.max_iters)
         cpu = None if args.no_cpu else cpu_baseline(deck, args.cpu_iters)
 
     cells = n * n
@@ -329,6 +362,7 @@ def main():
         "hbm_frac_per_iteration": per_iter_bytes * it_s / PEAK_HBM,
         "reference_equiv_GBs": REF_BYTES_PER_ITER * cells * it_s / 1e9,
         "solve_to_1e-10": r10,
+        "other_solvers": solvers,
         "roofline": roof,
         "cpu_baseline": cpu,
         "final_summary": r["final_summary"],

@@ -431,12 +431,9 @@ int cheby_graph(tl_chunk *c)
     const int B = 16;
     const Ext in = ext_interior(c);
     return capture(c, c->g_cheby, c->prec, [&]() {
-        for (int b = 0; b < B; ++b) {
-            launch_cheby_p(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_U0], c->f2[F_P], c->f2[F_R],
-                           nullptr, c->f2[F_CP], c->f2[F_BFP], c->f2[F_MI], c->f2[F_KX], c->f2[F_KY], c->ch_a,
-                           c->ch_b, 0.0, 0, c->st);
-            launch_cheby_u(c->s, c->g, in, c->cap, c->f2[F_U], c->f2[F_P], c->f2[F_R], red_args(c, RED_CHEBY, true));
-        }
+        for (int b = 0; b < B; ++b)
+            launch_fused_cheby(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[F_U], c->f2[F_U2], c->f2[F_P],
+                               c->f2[F_COND], c->faces, c->ch_a, c->ch_b, red_args(c, RED_CHEBY, true));
     });
 }
 
@@ -519,6 +516,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(state_download(c, &h));
     }
     int est_itc = 0;
+    bool cheby_fused = false;
     if (!h.done && h.switch_n > 0 && (use_ch || use_ppcg)) {
         const int n = h.switch_n;
         const int max_cheby_iters = cfg.max_iters - n + 2;
@@ -575,6 +573,8 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
                     h.exit_n = cfg.max_iters + 1;
                 } else {
                     h.n = n + 1;
+                    h.n_fused0 = n + 1;
+                    cheby_fused = true;
                 }
             }
             RC(state_upload(c, h));
@@ -594,7 +594,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(state_download(c, &h));
     }
     const int executed = (h.exit_n > cfg.max_iters) ? cfg.max_iters : h.exit_n;
-    if (use_jac && (executed & 1)) {
+    if ((use_jac && (executed & 1)) || (cheby_fused && ((executed - h.n_fused0 + 1) & 1))) {
         // the last sweep wrote the second buffer
         launch_copy(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U2], c->f2[F_U]);
     }

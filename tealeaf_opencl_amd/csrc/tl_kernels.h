@@ -91,5 +91,8 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
                              double *u, double *r, const double *sd_in, double *sd_out,
                              const double *W, const Faces &fc, const double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra);
+void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
+                        double *uA, double *uB, double *p, const double *W, const Faces &fc,
+                        const double *ch_a, const double *ch_b, const RedArgs &ra);
 
 }  // namespace tl

@@ -958,6 +958,57 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     if (LAST) block_partial<1>(acc, ra.partials);
 }
 
+// Chebyshev iteration: tea_leaf_cheby_solve_calc_p (tea_leaf_cheby_cl.cl:78-145)
+// then tea_leaf_cheby_solve_calc_u (:66-76) in one pass: r = u0 - A u,
+// z = M^-1 r, p = ca p + cb z, u' = u + p, and r.r when tea_solve.f90:223 asks
+// for it; the finisher runs the iteration tail. u is double-buffered so that
+// no neighbour is read after it is overwritten: iteration n reads buffer
+// (n - n_fused0)&1 and writes the other. r lives in registers only (the
+// reference stores it, but nothing in the iteration reads it back).
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby(Geom g, Ext e, const double *u0,
+                                                               double *uA, double *uB, double *p,
+                                                               const double *W, Faces fc,
+                                                               const double *ch_a, const double *ch_b,
+                                                               RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    const int step = st->cheby_calc_steps - 1;
+    const double ca = ch_a[step], cb = ch_b[step];
+    const bool odd = ((st->n - st->n_fused0) & 1) != 0;
+    const double *u = odd ? uB : uA;
+    double *un = odd ? uA : uB;
+    const bool want = cheby_norm_due(st);
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], pv[STRIP], wv[STRIP], uc[STRIP], rv[STRIP],
+        zv[STRIP];
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    col4_nt(u0, i0, P, b);
+    col4_nt(p, i0, P, pv);
+    strip_matvec(u, kx0, kx1, ky, i0, P, wv, uc);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = b[q] - wv[q];
+    mask_rows(ks, e.k1, rv);
+    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        double np;
+        if (PREC == PREC_BLOCK) np = ca * pv[q] + cb * zv[q];
+        else if (PREC == PREC_DIAG) np = ca * pv[q] + cb * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])) * rv[q];
+        else np = ca * pv[q] + cb * rv[q];
+        store_nt(p + i, np);
+        store_nt(un + i, uc[q] + np);
+        if (want) acc[0] += rv[q] * rv[q];
+    }
+    TL_TILE_LOOP_END
+    block_partial<1>(acc, ra.partials);
+}
+
 // ===========================================================================
 // host launchers
 // ===========================================================================
@@ -1250,6 +1301,18 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
         else TL_FPI(PREC_NONE, 0);
     }
 #undef TL_FPI
+}
+
+void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
+                        double *uA, double *uB, double *p, const double *W, const Faces &fc,
+                        const double *ch_a, const double *ch_b, const RedArgs &ra)
+{
+    if (prec == PREC_BLOCK)
+        TL_LAUNCH_RED(k_fused_cheby<PREC_BLOCK>, e, cap, s, 1, ra, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
+    else if (prec == PREC_DIAG)
+        TL_LAUNCH_RED(k_fused_cheby<PREC_DIAG>, e, cap, s, 1, ra, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
+    else
+        TL_LAUNCH_RED(k_fused_cheby<PREC_NONE>, e, cap, s, 1, ra, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
 }
 
 }  // namespace tl

@@ -93,6 +93,21 @@ def test_solvers_small(hiplib, oracle, tmp_path, solver, prec, mode):
     _compare_solves(lg, lo, 2)
 
 
+@pytest.mark.parametrize("prec", ["none", "jac_block"])
+@pytest.mark.parametrize("max_iters", [61, 62])
+def test_cheby_native_iteration_cap(hiplib, oracle, tmp_path, prec, max_iters):
+    """Native Chebyshev stopped by max_iters after an odd and an even number of
+    double-buffered iterations: u must come back from whichever buffer the
+    last iteration wrote, and the fields must match the oracle's."""
+    deck = decks.write_deck(str(tmp_path / "m.in"), 160, 120, decks.seeded_states(4, 2, True),
+                            solver="tl_use_chebyshev", preconditioner=prec, end_step=2, eps=1e-15,
+                            max_iters=max_iters, extra=["tl_ch_cg_presteps=20", "tl_ch_cg_epslim=1.0e300"])
+    lg, lo = _run_both(hiplib, oracle, deck, tmp_path, 1)
+    assert [s[1] for s in lg["solve"]] == [max_iters] * 2
+    _compare_summaries(lg, lo)
+    _compare_solves(lg, lo, 0)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_reflective_and_inverse_density(hiplib, oracle, tmp_path, mode):
     deck = decks.write_deck(str(tmp_path / "r.in"), 130, 90, decks.seeded_states(3, 2, False),
@@ -109,6 +124,30 @@ def test_dropin_and_native_agree(hiplib, tmp_path):
     b = hiplib.run_deck(deck, tmp_path / "b.log", mode=1)
     _compare_summaries(b, a, 1e-12)
     _compare_solves(b, a, 1)
+
+
+@pytest.mark.parametrize("solver", ["tl_use_cg", "tl_use_chebyshev", "tl_use_ppcg"])
+def test_full_size_native_vs_dropin(hiplib, tmp_path, solver):
+    """4096^2 deck (two rectangles and a circle, block Jacobi, eps 1e-15), one
+    timestep with each solver: the fused device-resident kernels against the
+    reference-granularity kernels of the drop-in boundary, which share no
+    iteration kernel with them. At this size eps 1e-15 is below what the
+    Chebyshev residual (a true residual) and the PPCG norm (r.z with the
+    stale z) can reach, so those two stop at tl_max_iters 5000 in both modes;
+    CG converges and conserves the internal energy (sum of vol*u) to its
+    residual floor."""
+    deck = decks.config2_deck(str(tmp_path / "f.in"), solver=solver, steps=1)
+    a = hiplib.run_deck(deck, tmp_path / "a.log", mode=0)
+    b = hiplib.run_deck(deck, tmp_path / "b.log", mode=1)
+    _compare_summaries(b, a, 1e-11)
+    # a PPCG exit in the stagnation region may come some iterations apart
+    _compare_solves(b, a, {"tl_use_cg": 1, "tl_use_chebyshev": 0, "tl_use_ppcg": 50}[solver])
+    if solver == "tl_use_cg":
+        s0, s1 = b["summary"][0], b["summary"][-1]
+        # sum(vol*u) changes by -sum(vol*r) of the true residual, whose floor at
+        # 16.8M cells is a few 1e-12 of the total
+        assert abs(s1[4] - s0[4]) <= 1e-11 * abs(s0[4]), (s0, s1)
+        assert b["solve"][0][1] < 5000
 
 
 @pytest.mark.parametrize("kat", [2, 3])
