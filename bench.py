@@ -36,7 +36,12 @@ ALG_BYTES = {
     "k_fused_cheby": 6 * 8,
     # r -= A sd; u += sd; sd' = ca sd + cb M_blk^-1 r : R u, r, w, sd  W u, r, sd'
     "k_fused_ppcg_inner": 7 * 8,
+    # two such steps per launch : R u, r, sd, Kx, Ky  W u, r, sd
+    "k_fused_ppcg_inner2": 8 * 8,
 }
+# the reference's bytes per cell for the same work with block Jacobi
+# (kernel_files/tea_leaf_{cg,cheby,ppcg}_cl.cl, each array once per kernel)
+REF_BYTES = {"cg": 144, "cheby": 104, "ppcg_inner": 112, "ppcg_outer": 208}
 # the refined solve's inner kernels (tl_refine.hip): same streams as the fused
 # FP64 kernels, with r, z, p, d and the face field stored in FP64 or FP32
 ALG_BYTES_RF = {
@@ -147,7 +152,10 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
     timestep, then `steps` timed ones). Bytes per iteration are algorithmic:
     the fused Chebyshev iteration streams u, u0, w, p in and p, u out
     (48 B/cell); a PPCG outer step is the two fused CG kernels (88 B/cell)
-    plus 10 inner steps of u, r, w, sd in and u, r, sd out (56 B/cell each)."""
+    plus 10 inner steps, two per launch, each launch streaming u, r, sd, Kx,
+    Ky in and u, r, sd out (64 B/cell). `reference_equiv_GBs` counts the
+    reference's bytes for the same iterations instead (Chebyshev 104 B/cell,
+    PPCG outer 208 and inner 112, CG 144)."""
     import tealeaf_opencl_amd as tl
     from tealeaf_opencl_amd import decks
 
@@ -157,18 +165,19 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
                                   summary_frequency=0, max_iters=max_iters)
         r = tl.bench_deck(deck, mode=tl.MODE_NATIVE, warmup=1, steps=steps)
         its, inner, secs = r["iterations"], r["inner_iterations"], r["seconds"]
+        cg_b = ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur"]
         if name == "chebyshev":
-            # 20 CG steps (two fused CG kernels) then Chebyshev iterations
+            # about 20 CG steps (two fused CG kernels) then Chebyshev iterations
             cg = 20 * steps
-            alg = (cg * (ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur"]) +
-                   (its - cg) * ALG_BYTES["k_fused_cheby"]) * n * n
+            alg = (cg * cg_b + (its - cg) * ALG_BYTES["k_fused_cheby"]) * n * n
+            ref = (cg * REF_BYTES["cg"] + (its - cg) * REF_BYTES["cheby"]) * n * n
         else:
-            alg = (its * (ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur"]) +
-                   inner * ALG_BYTES["k_fused_ppcg_inner"]) * n * n
+            alg = (its * cg_b + inner // 2 * ALG_BYTES["k_fused_ppcg_inner2"]) * n * n
+            ref = (its * REF_BYTES["ppcg_outer"] + inner * REF_BYTES["ppcg_inner"]) * n * n
         out[name] = {"s_per_step": secs / steps, "iterations_per_step": its / steps,
                      "inner_iterations_per_step": inner / steps, "iterations_per_s": its / secs,
                      "alg_GBs": alg / secs / 1e9, "alg_frac_of_peak": alg / secs / PEAK_HBM,
-                     "final_summary": r["final_summary"]}
+                     "reference_equiv_GBs": ref / secs / 1e9, "final_summary": r["final_summary"]}
         print("solvers leg %s: %.3f s per step, %d iterations" % (name, secs / steps, its // steps),
               file=sys.stderr, flush=True)
     return out

@@ -176,6 +176,23 @@ int tl_print_profiling(tl_chunk *c);
  * CG update kernel (kind 1, 56 B/cell), over temporary fields. */
 int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
 
+/* Tuning of the native driver's fused kernels. No reference counterpart.
+ * TL_TUNE_PPCG_STEPS_PER_LAUNCH: PPCG inner steps per fused launch, 1 or 2
+ * (default 2); both give bitwise the same fields. With c null the value
+ * becomes the default of chunks created afterwards (tl_run_deck creates its
+ * own chunk). */
+enum { TL_TUNE_PPCG_STEPS_PER_LAUNCH = 1 };
+int tl_set_tuning(tl_chunk *c, int key, int value);
+/* PPCG smoothing steps first_step .. first_step+nsteps-1 (1-based, as the
+ * inner_step of tl_ppcg_inner) with the native
+ * driver's fused kernels (one or two per launch as tuned) on the chunk's u, r
+ * and sd, over the interior: the same fields as nsteps tl_ppcg_inner calls
+ * with bounds_extra 0 (tea_leaf_ppcg_cl.cl:62-155), with the face
+ * coefficients of the last tl_common_init and the coefficients of the last
+ * tl_ppcg_init. No reference counterpart (the reference launches one step per
+ * call). */
+int tl_ppcg_inner_fused(tl_chunk *c, int first_step, int nsteps);
+
 /* Whole-deck driver (start.f90 + diffuse.f90 + tea_solve.f90 restated).
  * Writes a line-oriented log: SUMMARY step vol mass density ie temp /
  * SOLVE step iters conduction_err exact_err ppcg_inner cg_steps cheby_steps

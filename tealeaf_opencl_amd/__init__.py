@@ -28,6 +28,7 @@ PREC_NONE, PREC_JAC_DIAG, PREC_JAC_BLOCK = 1, 2, 3
 SOLVER_JACOBI, SOLVER_CG, SOLVER_CHEBY, SOLVER_PPCG = 1, 2, 3, 4
 COEF_CONDUCTIVITY, COEF_RECIP_CONDUCTIVITY = 1, 2
 MODE_DROPIN, MODE_NATIVE = 0, 1
+TUNE_PPCG_STEPS_PER_LAUNCH = 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 G_RECT, G_CIRC, G_POINT = 1, 2, 3
 
@@ -113,6 +114,8 @@ _SIGS = {
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
     "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
+    "tl_set_tuning": (_I, [_P, _I, _I]),
+    "tl_ppcg_inner_fused": (_I, [_P, _I, _I]),
     "tl_set_progress": (_I, [_P, _P, _P]),
     "tl_solve_refined_ppcg": (_I, [_P, _I, _D, _I, _D, _I, _I, ctypes.POINTER(RefineResult)]),
     "tl_adjoint_density": (_I, [_P, _I, _D, _I, _D, _I, _I, _DP, _DP, ctypes.POINTER(AdjointResult)]),
@@ -304,6 +307,11 @@ class Chunk:
         nb = np.ascontiguousarray(chunk_neighbours, dtype=np.int32)
         _check(self.lib.tl_ppcg_inner(self.h, int(inner_step), int(bounds_extra), _iptr(nb)), "ppcg_inner")
 
+    def ppcg_inner_fused(self, first_step, nsteps):
+        """Inner steps first_step .. first_step+nsteps-1 with the fused kernels
+        of the native driver (tl_ppcg_inner_fused)."""
+        _check(self.lib.tl_ppcg_inner_fused(self.h, int(first_step), int(nsteps)), "ppcg_inner_fused")
+
     def jacobi_solve(self):
         v = _D()
         _check(self.lib.tl_jacobi_solve(self.h, ctypes.byref(v)), "jacobi_solve")
@@ -433,6 +441,13 @@ def run_deck(deck, log_path, mode=MODE_NATIVE, max_steps=0, tea_out=None):
                                str(tea_out).encode() if tea_out else None, int(mode), int(max_steps)),
            "tl_run_deck")
     return parse_log(log_path)
+
+
+def set_tuning(key, value, chunk=None):
+    """tl_set_tuning: on `chunk`, or (chunk None) the default of chunks
+    created afterwards, including those of run_deck / bench_deck."""
+    lib = load_library()
+    _check(lib.tl_set_tuning(chunk.h if chunk is not None else None, int(key), int(value)), "tl_set_tuning")
 
 
 def bench_deck(deck, mode=MODE_NATIVE, warmup=1, steps=1):

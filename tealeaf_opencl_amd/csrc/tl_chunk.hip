@@ -198,6 +198,10 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
     return TL_OK;
 }
 
+// Process-wide default of the tuning keys (tl_set_tuning with no chunk),
+// taken by chunks created later.
+static int g_ppcg_steps_default = 2;
+
 extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int preconditioner, int profiler_on,
                                tl_chunk **out)
 {
@@ -217,6 +221,7 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     c->g.y_max = y_max;
     c->g.hd = halo_depth;
     c->prec = preconditioner;
+    c->ppcg_steps_per_launch = g_ppcg_steps_default;
     c->profiler_on = profiler_on;
     // pitch: padded row plus one slack column, rounded to 16 doubles (128 B)
     const int w = x_max + 2 * halo_depth + 2;
@@ -771,7 +776,6 @@ extern "C" int tl_jacobi_solve(tl_chunk *c, double *error)
     return read_result(c, error, 1);
 }
 
-// ocl_errors.cpp:203-232
 // Attainable streaming bandwidth on the chunk's layout: `reps` launches of a
 // copy (kind 0: 8 B/cell read + 8 B/cell written) or of the 4-read/3-write
 // mix of the CG update kernel (kind 1: 56 B/cell) over the interior, on
@@ -819,6 +823,18 @@ extern "C" int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *byte
     return rc;
 }
 
+extern "C" int tl_set_tuning(tl_chunk *c, int key, int value)
+{
+    if (key != TL_TUNE_PPCG_STEPS_PER_LAUNCH || (value != 1 && value != 2)) {
+        set_error("tl_set_tuning: bad key %d or value %d", key, value);
+        return TL_ERR_ARG;
+    }
+    if (c) c->ppcg_steps_per_launch = value;
+    else g_ppcg_steps_default = value;
+    return TL_OK;
+}
+
+// ocl_errors.cpp:203-232
 extern "C" int tl_print_profiling(tl_chunk *c)
 {
     if (!c || !c->profiler_on) return TL_OK;

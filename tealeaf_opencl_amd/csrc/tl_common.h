@@ -26,7 +26,7 @@ enum Field2D {
     F_U, F_U0, F_P, F_R, F_W, F_MI, F_KX, F_KY, F_SD, F_Z, F_CP, F_BFP,
     NUM_PUBLIC_2D,
     // second buffers of the double-buffered iteration kernels
-    F_P2 = NUM_PUBLIC_2D, F_SD2, F_U2,
+    F_P2 = NUM_PUBLIC_2D, F_SD2, F_U2, F_R2,
     // conductivity w (density or 1/density) of the current timestep, from
     // which the fused iteration kernels recompute Kx, Ky
     F_COND,

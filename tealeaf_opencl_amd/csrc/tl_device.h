@@ -36,11 +36,17 @@ namespace tl {
 
 __device__ __forceinline__ int IX(const Geom &g, int j, int k) { return k * g.pitch + j; }
 
-// 5-point operator of tea_leaf_common_cl.cl:28-32 (same association)
+// 5-point operator of tea_leaf_common_cl.cl:28-32 (same association). The
+// products are fused explicitly: left to the compiler, which of two products
+// summed together becomes the FMA can depend on how the surrounding kernel is
+// laid out in basic blocks, and kernels that apply the operator to the same
+// values would then differ in the last bit.
 __device__ __forceinline__ double opA(double kx0, double kx1, double ky0, double ky1, double vc,
                                       double vw, double ve, double vs, double vn)
 {
-    return (1.0 + (ky1 + ky0) + (kx1 + kx0)) * vc - (ky1 * vn + ky0 * vs) - (kx1 * ve + kx0 * vw);
+    const double ty = fma(ky1, vn, ky0 * vs);
+    const double tx = fma(kx1, ve, kx0 * vw);
+    return fma(1.0 + (ky1 + ky0) + (kx1 + kx0), vc, -ty) - tx;
 }
 
 __device__ __forceinline__ double diag_of(double kx0, double kx1, double ky0, double ky1)

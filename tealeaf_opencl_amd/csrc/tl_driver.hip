@@ -367,7 +367,7 @@ void cg_iteration(tl_chunk *c, hipEvent_t *ev)
                       red_args(c, RED_CG_PW, true), ev ? ev[1] : nullptr);
     if (ev) hipEventRecord(ev[2], c->s);
     launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                       c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_CG_RRN, true),
+                       c->f2[F_R], c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_CG_RRN, true),
                        ev ? ev[3] : nullptr);
 }
 
@@ -437,26 +437,81 @@ int cheby_graph(tl_chunk *c)
     });
 }
 
+// PPCG outer step (tea_solve.f90:231-267) and its inner steps
+// (tea_leaf_run_ppcg_inner_steps, :422-475). With two inner steps per launch
+// the pairs read r and sd from one buffer and write the other; the outer
+// step's update writes r to the second buffer when the number of pairs is
+// odd, so that every outer step starts and ends with r in F_R. An odd inner
+// count ends with one single-step launch (in place for r).
 int ppcg_graph(tl_chunk *c, int inner)
 {
     const int B = std::max(1, 32 / (inner + 2));
     const Ext in = ext_interior(c);
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
-    return capture(c, c->g_ppcg, c->prec * 100000 + inner, [&]() {
+    const int per = c->ppcg_steps_per_launch;
+    const int pairs = per == 2 ? inner / 2 : 0;
+    return capture(c, c->g_ppcg, (c->prec * 100000 + inner) * 4 + per, [&]() {
         for (int b = 0; b < B; ++b) {
             launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
                               red_args(c, RED_PPCG_PW, true));
+            double *r_cur = (pairs & 1) ? c->f2[F_R2] : c->f2[F_R];
+            double *r_oth = (pairs & 1) ? c->f2[F_R] : c->f2[F_R2];
             launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 1, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                               c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_STORE, true));
-            for (int i = 1; i <= inner; ++i) {
-                const double *sin = (i & 1) ? c->f2[F_SD] : c->f2[F_SD2];
-                double *sout = (i & 1) ? c->f2[F_SD2] : c->f2[F_SD];
-                launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner, c->f2[F_U], c->f2[F_R], sin, sout,
-                                        c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
+                               r_cur, c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_STORE, true));
+            double *sd_cur = c->f2[F_SD], *sd_oth = c->f2[F_SD2];
+            int i = 1;
+            for (int k = 0; k < pairs; ++k, i += 2) {
+                launch_fused_ppcg_inner2(c->s, c->g, in, c->cap, c->prec, i + 1 == inner, c->f2[F_U], r_cur, r_oth,
+                                         sd_cur, sd_oth, c->f2[F_KX], c->f2[F_KY], c->f2[F_Z], c->ch_a, c->ch_b,
+                                         i - 1, red_args(c, RED_PPCG_RRN, true));
+                std::swap(r_cur, r_oth);
+                std::swap(sd_cur, sd_oth);
+            }
+            for (; i <= inner; ++i) {
+                launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner, c->f2[F_U], r_cur, sd_cur,
+                                        sd_oth, c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
                                         red_args(c, RED_PPCG_RRN, true));
+                std::swap(sd_cur, sd_oth);
             }
         }
     });
+}
+
+// PPCG inner steps first_step .. first_step+nsteps-1 (1-based, like the
+// inner_step of tl_ppcg_inner) of the smoother with the
+// fused kernels of the native driver (one or two steps per launch, as
+// tuned), on the chunk's u, r and sd; results left in u, r, sd like the same
+// number of tl_ppcg_inner calls over the interior (tea_leaf_ppcg_cl.cl:62-155).
+extern "C" int tl_ppcg_inner_fused(tl_chunk *c, int first_step, int nsteps)
+{
+    if (!c || first_step < 1 || nsteps < 1 || first_step - 1 + nsteps > c->n_coefs) {
+        set_error("tl_ppcg_inner_fused: steps %d..%d outside the %d coefficients", first_step,
+                  first_step + nsteps - 1, c ? c->n_coefs : 0);
+        return TL_ERR_ARG;
+    }
+    SolveState h;
+    memset(&h, 0, sizeof h);
+    RC(state_upload(c, h));  // halt = 0 for the kernels' state test
+    const Ext in = ext_interior(c);
+    const RedArgs ra = red_args(c, RED_STORE, true);
+    double *r_cur = c->f2[F_R], *r_oth = c->f2[F_R2], *sd_cur = c->f2[F_SD], *sd_oth = c->f2[F_SD2];
+    int i = 0;
+    for (; c->ppcg_steps_per_launch == 2 && i + 1 < nsteps; i += 2) {
+        launch_fused_ppcg_inner2(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], r_cur, r_oth, sd_cur, sd_oth,
+                                 c->f2[F_KX], c->f2[F_KY], c->f2[F_Z], c->ch_a, c->ch_b, first_step - 1 + i, ra);
+        std::swap(r_cur, r_oth);
+        std::swap(sd_cur, sd_oth);
+    }
+    for (; i < nsteps; ++i) {
+        launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], r_cur, sd_cur, sd_oth,
+                                c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b, first_step - 1 + i, ra);
+        std::swap(sd_cur, sd_oth);
+    }
+    if (r_cur != c->f2[F_R]) launch_copy(c->s, c->g, in, c->cap, r_cur, c->f2[F_R]);
+    if (sd_cur != c->f2[F_SD]) launch_copy(c->s, c->g, in, c->cap, sd_cur, c->f2[F_SD]);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+    return e == hipSuccess ? TL_OK : hip_fail("tl_ppcg_inner_fused", e);
 }
 
 int jacobi_graph(tl_chunk *c)

@@ -83,14 +83,21 @@ void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, cons
                        double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
                        hipEvent_t ev_main = nullptr);
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
-                        const double *pA, const double *pB, double *r, double *z, double *sd,
-                        const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main = nullptr);
+                        const double *pA, const double *pB, const double *r, double *r_out, double *z,
+                        double *sd, const double *W, const Faces &fc, const RedArgs &ra,
+                        hipEvent_t ev_main = nullptr);
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
                          const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra);
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
                              const double *W, const Faces &fc, const double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra);
+// two PPCG inner steps (`step`, `step`+1) per launch; r and sd are read from
+// one buffer and written to another
+void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last, double *u,
+                              const double *r_in, double *r_out, const double *sd_in, double *sd_out,
+                              const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                              const double *ch_b, int step, const RedArgs &ra);
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                         double *uA, double *uB, double *p, const double *W, const Faces &fc,
                         const double *ch_a, const double *ch_b, const RedArgs &ra);

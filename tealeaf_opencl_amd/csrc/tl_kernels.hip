@@ -872,11 +872,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
 // recomputed, u += alpha p, r -= alpha w, z = M^-1 r with the diagonal or
 // block factors recomputed in registers, r.z. PPCG=1 also writes
 // sd = z/theta (tea_leaf_ppcg_cl.cl:4-60 on the same r) and skips the unused
-// r.z sum.
+// r.z sum. r_out may be r (in place) or a second buffer (the PPCG outer step
+// when the inner steps that follow flip r between two buffers).
 template <int PREC, int PPCG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, double *u,
                                                                const double *pA, const double *pB,
-                                                               double *r, double *z,
+                                                               const double *r, double *r_out, double *z,
                                                                double *sd, const double *W, Faces fc,
                                                                RedArgs ra)
 {
@@ -905,7 +906,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
         store_nt(u + i, uv[q]);
-        store_nt(r + i, rv[q]);
+        store_nt(r_out + i, rv[q]);
         double zz = rv[q];
         if (PREC == PREC_BLOCK) zz = zv[q];
         if (PREC == PREC_DIAG) zz = rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
@@ -915,6 +916,28 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     }
     TL_TILE_LOOP_END
     if (!PPCG) block_partial<1>(acc, ra.partials);
+}
+
+// One PPCG smoothing step on a strip once A sd is known (res = A sd, sc = sd):
+// r -= A sd, then sd' = ca sd + cb M^-1 r (tea_leaf_ppcg_cl.cl:62-155). Shared
+// by the one-step and two-step kernels, with the FMA placed explicitly, so
+// that both round identically.
+template <int PREC>
+__device__ __forceinline__ void ppcg_step(const Geom &g, int ks, int k1, const double *kx0, const double *kx1,
+                                          const double *ky, const double *res, const double *sc, double ca,
+                                          double cb, double *rv, double *sdn)
+{
+    double zv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] -= res[q];
+    mask_rows(ks, k1, rv);
+    if (PREC == PREC_BLOCK) strip_block(g, ks, 0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (PREC == PREC_BLOCK) sdn[q] = fma(ca, sc[q], cb * zv[q]);
+        else if (PREC == PREC_DIAG) sdn[q] = fma(ca, sc[q], cb * rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])));
+        else sdn[q] = fma(ca, sc[q], cb * rv[q]);
+    }
 }
 
 // PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
@@ -931,30 +954,155 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
-    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP],
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], sdn[STRIP],
         zo[STRIP];
     strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(u, i0, P, uv);
     col4_nt(r, i0, P, rv);
     if (LAST && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
     strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) rv[q] -= res[q];
-    mask_rows(ks, e.k1, rv);
-    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+    ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
         store_nt(u + i, uv[q] + sc[q]);
         store_nt(r + i, rv[q]);
-        if (PREC == PREC_BLOCK) store_nt(sd_out + i, ca * sc[q] + cb * zv[q]);
-        else if (PREC == PREC_DIAG)
-            store_nt(sd_out + i, ca * sc[q] + cb * rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])));
-        else store_nt(sd_out + i, ca * sc[q] + cb * rv[q]);
+        store_nt(sd_out + i, sdn[q]);
         if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
     }
     TL_TILE_LOOP_END
+    if (LAST) block_partial<1>(acc, ra.partials);
+}
+
+// sd at ring cell (j, k) when it is not computed in the block: the value the
+// one-step kernels would read there, i.e. the halo cell of the sd buffer
+// (j0-1, j1+1, k0-1 or k1+1; no interior kernel writes it); 0 for cells
+// beyond the halo, which no interior cell reads.
+__device__ __forceinline__ double ring_outside(const Geom &g, const Ext &e, const double *sd, int j, int k)
+{
+    const bool in = j >= e.j0 - 1 && j <= e.j1 + 1 && k >= e.k0 - 1 && k <= e.k1 + 1;
+    return in ? sd[IX(g, j, k)] : 0.0;
+}
+
+// Two PPCG inner steps (steps `step` and `step`+1 of the smoother) in one
+// pass over the fields. Step 1 is computed on the block's 64x16 tile and on
+// the ring around it that step 2 reads: the 4-row strips directly above and
+// below the tile (their block-Jacobi solve needs the whole strip) and the
+// tile's strips in the columns left and right of it. The ring's new sd goes
+// through LDS; the tile keeps r, u and the faces in registers. Every value is
+// formed by the same expressions as the one-step kernel, so two one-step
+// launches and one two-step launch give bitwise the same u, r and sd.
+// The face coefficients are read from Kx, Ky (bitwise what the one-step
+// kernels recompute from the conductivity): with the ring's extra strips this
+// kernel is bound by arithmetic as much as by memory, and reading the faces
+// instead of dividing measured 5 % faster per PPCG outer step at 4096^2.
+// Algorithmic bytes per two steps: R u, r, sd, Kx, Ky  W u, r, sd (64 B/cell)
+// instead of 112. Neighbouring blocks read r and sd of this block's tile in
+// their step 1, so r and sd are both read from one buffer and written to
+// another.
+template <int PREC, int LAST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
+    Geom g, Ext e, double *u, const double *r_in, double *r_out, const double *sd_in, double *sd_out,
+    const double *Kx, const double *Ky, const double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    constexpr int TH = STRIPS_PER_TILE * STRIP;  // tile rows
+    constexpr int RW = TILE_X + 2, RH = TH + 2;   // ring-padded tile
+    __shared__ double ring[RH][RW];
+    const double ca1 = ch_a[step], cb1 = ch_b[step], ca2 = ch_a[step + 1], cb2 = ch_b[step + 1];
+    const int P = g.pitch;
+    const int lane = threadIdx.x & (TILE_X - 1), wv = threadIdx.x / TILE_X;
+    const int nt = e.ntx * e.nty;
+    double acc[1] = {0.0};
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
+        const int j = jt + lane, ks = kt + wv * STRIP;
+        const bool act = j <= e.j1 && ks <= e.k1;
+        const int i0 = IX(g, j, ks);
+        // step 1 on the ring first, so that its registers are free again
+        // before the own strip's state (kept for step 2) goes live: strip
+        // above (wave 0), strip below (wave 1), the tile's strips in the
+        // columns either side (wave 2, lanes 0-7)
+        {
+            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
+            if (wv == 0) {
+                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
+            } else if (wv == 1) {
+                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
+            } else if (wv == 2 && lane < 2 * STRIPS_PER_TILE) {
+                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
+                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
+                row0 = 1 + qs * STRIP;
+            }
+            if (hj >= 0) {
+                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
+                double sdn[STRIP];
+                if (inside) {
+                    const int h0 = IX(g, hj, hk);
+                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP];
+                    strip_k(Kx, Ky, h0, P, hx0, hx1, hy);
+                    col4(r_in, h0, P, hr);
+                    strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
+                    ppcg_step<PREC>(g, hk, e.k1, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn);
+                }
+                if (take < STRIP) {
+                    // one row of a strip above / below the tile
+                    const int k = hk + take;
+                    if (hj <= e.j1 + 1) ring[row0][col] = (inside && k <= e.k1) ? sdn[take] : ring_outside(g, e, sd_out, hj, k);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < STRIP; ++q)
+                        ring[row0 + q][col] = (inside && hk + q <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, hj, hk + q);
+                }
+            }
+        }
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], uv[STRIP];
+        // step 1 on the own strip
+        if (act) {
+            double res[STRIP], sc[STRIP], sdn[STRIP];
+            strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+            col4_nt(u, i0, P, uv);
+            col4_nt(r_in, i0, P, rv);
+            strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                uv[q] += sc[q];
+                ring[1 + wv * STRIP + q][1 + lane] = (ks + q <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, j, ks + q);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) ring[1 + wv * STRIP + q][1 + lane] = ring_outside(g, e, sd_out, j, ks + q);
+        }
+        __syncthreads();
+        // step 2 on the own strip, sd from the ring
+        if (act) {
+            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP];
+#pragma unroll
+            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv * STRIP + q][1 + lane];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                w4[q] = ring[1 + wv * STRIP + q][lane];
+                e4[q] = ring[1 + wv * STRIP + q][lane + 2];
+                sc[q] = c6[q + 1];
+            }
+            if (LAST && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
+            strip_apply(c6, w4, e4, kx0, kx1, ky, res);
+            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks + q > e.k1) continue;
+                const int i = i0 + q * P;
+                store_nt(u + i, uv[q] + sc[q]);
+                store_nt(r_out + i, rv[q]);
+                store_nt(sd_out + i, sdn[q]);
+                if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+            }
+        }
+        __syncthreads();
+    }
     if (LAST) block_partial<1>(acc, ra.partials);
 }
 
@@ -1258,14 +1406,14 @@ void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, co
 }
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
-                        const double *pA, const double *pB, double *r, double *z, double *sd,
-                        const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main)
+                        const double *pA, const double *pB, const double *r, double *r_out, double *z,
+                        double *sd, const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main)
 {
-#define TL_FUR(P, M)                                                                                          \
-    do {                                                                                                      \
-        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, z, sd, W, fc, ra);              \
-        else TL_LAUNCH_RED_EV((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, ev_main, g, e, u, pA, pB, r, z, sd, W,  \
-                              fc, ra);                                                                        \
+#define TL_FUR(P, M)                                                                                           \
+    do {                                                                                                       \
+        if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, r_out, z, sd, W, fc, ra);        \
+        else TL_LAUNCH_RED_EV((k_fused_cg_ur<P, M>), e, cap, s, 1, ra, ev_main, g, e, u, pA, pB, r, r_out, z,   \
+                              sd, W, fc, ra);                                                                  \
     } while (0)
     if (ppcg) {
         if (prec == PREC_BLOCK) TL_FUR(PREC_BLOCK, 1);
@@ -1301,6 +1449,30 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
         else TL_FPI(PREC_NONE, 0);
     }
 #undef TL_FPI
+}
+
+void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last, double *u,
+                              const double *r_in, double *r_out, const double *sd_in, double *sd_out,
+                              const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                              const double *ch_b, int step, const RedArgs &ra)
+{
+#define TL_FPI2(P, L)                                                                                           \
+    do {                                                                                                        \
+        if (L) TL_LAUNCH_RED((k_fused_ppcg_inner2<P, L>), e, cap, s, 1, ra, g, e, u, r_in, r_out, sd_in, sd_out, \
+                             Kx, Ky, z, ch_a, ch_b, step, ra);                                                  \
+        else TL_LAUNCH((k_fused_ppcg_inner2<P, L>), e, cap, s, g, e, u, r_in, r_out, sd_in, sd_out, Kx, Ky, z,   \
+                       ch_a, ch_b, step, ra);                                                                   \
+    } while (0)
+    if (last) {
+        if (prec == PREC_BLOCK) TL_FPI2(PREC_BLOCK, 1);
+        else if (prec == PREC_DIAG) TL_FPI2(PREC_DIAG, 1);
+        else TL_FPI2(PREC_NONE, 1);
+    } else {
+        if (prec == PREC_BLOCK) TL_FPI2(PREC_BLOCK, 0);
+        else if (prec == PREC_DIAG) TL_FPI2(PREC_DIAG, 0);
+        else TL_FPI2(PREC_NONE, 0);
+    }
+#undef TL_FPI2
 }
 
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,

@@ -93,6 +93,37 @@ def test_solvers_small(hiplib, oracle, tmp_path, solver, prec, mode):
     _compare_solves(lg, lo, 2)
 
 
+@pytest.mark.parametrize("case", ["none-3", "jac_diag-4", "jac_block-10", "jac_block-7-reflective",
+                                  "jac_diag-5-halo2"])
+def test_ppcg_two_steps_per_launch_bitwise(hiplib, tmp_path, case):
+    """The native PPCG with two inner steps per launch (ring of the tile
+    recomputed, r and sd flipped between buffers) against one step per launch:
+    bitwise the same field summaries and solve lines, on a grid whose tiles
+    are partial in x and y, with odd and even inner counts, walls with
+    non-zero faces and a deeper halo."""
+    parts = case.split("-")
+    prec, inner = parts[0], int(parts[1])
+    extra = ["tl_ppcg_inner_steps=%d" % inner]
+    kw = {}
+    if "reflective" in parts:
+        extra.append("reflective_boundary")
+        kw["coefficient"] = "tl_coefficient_inverse_density"
+    if "halo2" in parts:
+        kw["halo_depth"] = 2
+    deck = decks.write_deck(str(tmp_path / "p.in"), 200, 148, decks.seeded_states(2, 2, True), solver="tl_use_ppcg",
+                            preconditioner=prec, end_step=2, eps=1e-12, max_iters=2000, extra=extra, **kw)
+    try:
+        hiplib.set_tuning(hiplib.TUNE_PPCG_STEPS_PER_LAUNCH, 1)
+        one = hiplib.run_deck(deck, tmp_path / "one.log", mode=1)
+    finally:
+        hiplib.set_tuning(hiplib.TUNE_PPCG_STEPS_PER_LAUNCH, 2)
+    two = hiplib.run_deck(deck, tmp_path / "two.log", mode=1)
+    # (the exact-error column is NaN without tl_check_result)
+    assert one["solve"] and np.array_equal(np.array(one["solve"], float), np.array(two["solve"], float),
+                                           equal_nan=True), (one["solve"], two["solve"])
+    assert one["summary"] == two["summary"]
+
+
 @pytest.mark.parametrize("prec", ["none", "jac_block"])
 @pytest.mark.parametrize("max_iters", [61, 62])
 def test_cheby_native_iteration_cap(hiplib, oracle, tmp_path, prec, max_iters):

@@ -142,6 +142,40 @@ def test_ppcg_steps(hiplib, oracle, x, y, hd, prec):
     assert scal_err(g.calc_2norm(2), o.calc_2norm(2)) < 1e-12
 
 
+@pytest.mark.parametrize("reflective", [False, True])
+@pytest.mark.parametrize("prec", [1, 2, 3])
+@pytest.mark.parametrize("x,y,hd", SIZES)
+def test_ppcg_fused_steps(hiplib, oracle, x, y, hd, prec, reflective):
+    """The native driver's fused PPCG smoothing kernels (tl_ppcg_inner_fused)
+    against the oracle's tl_ppcg_inner sequence, and two steps per launch
+    (the tile's ring recomputed from the neighbours' r and sd) bitwise
+    against one step per launch."""
+    if prec == 3 and hd > 1:
+        pytest.skip("read_input.f90:299-301 rejects PPCG + block Jacobi with halo_depth > 1")
+    g1, o = pair(hiplib, oracle, x, y, hd, prec)
+    g2 = hiplib.Chunk(x, y, hd, prec)
+    hiplib.set_tuning(hiplib.TUNE_PPCG_STEPS_PER_LAUNCH, 1, chunk=g1)
+    hiplib.set_tuning(hiplib.TUNE_PPCG_STEPS_PER_LAUNCH, 2, chunk=g2)
+    n = 7
+    a = np.linspace(0.2, 0.8, n)
+    b = np.linspace(0.05, 0.09, n)
+    for c in (g1, g2, o):
+        setup(c, seed=4, reflective=reflective)
+        c.calc_residual()
+        c.cg_init()
+        c.ppcg_init(a, b, 2.5, n)
+        c.ppcg_init_sd()
+    g1.ppcg_inner_fused(1, n)
+    g2.ppcg_inner_fused(1, n)
+    for step in range(1, n + 1):
+        o.ppcg_inner(step, 0)
+    for f in ("u", "r", "sd"):
+        f1, f2 = g1.get_field(f), g2.get_field(f)
+        bad = np.argwhere(f1 != f2)
+        assert bad.size == 0, (f, len(bad), bad[:8].tolist())
+        assert rel_err(f2, o.get_field(f)) < 1e-12, f
+
+
 @pytest.mark.parametrize("x,y,hd", SIZES)
 def test_jacobi_sweeps(hiplib, oracle, x, y, hd):
     g, o = pair(hiplib, oracle, x, y, hd, 1)
