@@ -73,8 +73,8 @@ struct tl_chunk {
         int bytes = 0;                    // 8 or 4: storage of the inner fields
         tl::Geom g{};                     // layout of the inner fields
         long off = 0, elems = 0;
-        void *base[8] = {};               // owned allocations
-        void *f[8] = {};                  // conductivity w, r, z, p, p2, d, sd, sd2
+        void *base[9] = {};               // owned allocations
+        void *f[9] = {};                  // conductivity w, r, z, p, p2, d, sd, sd2, r2
         tl::RefineState *st = nullptr;
         tl::RefineState *h_st = nullptr;  // pinned, 2 slots
         GraphSlot graph;                  // CG iterations

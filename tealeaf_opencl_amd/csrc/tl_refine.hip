@@ -47,7 +47,7 @@ namespace tl {
 // internal linkage are not visible to the profiling tools)
 namespace rf {
 
-enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_SD, RW_SD2, RW_N };
+enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_SD, RW_SD2, RW_R2, RW_N };
 
 struct RfArgs {
     double *partials;
@@ -305,8 +305,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
 // r -= alpha A p, z = M^-1 r, sd = z / theta; no reduction (the outer step's
 // r.z is formed after the inner steps with this z, as the reference does).
 template <typename T, int PREC>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
-                                                              T *z, T *sd, const T *W, Faces fc, RfArgs ra)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *d, const T *pA, const T *pB,
+                                                              const T *r, T *r_out, T *z, T *sd, const T *W, Faces fc,
+                                                              RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -332,11 +333,29 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *
         const int i = i0 + q * P;
         const double zt = rnd<T>(zv[q]);
         store_nt(d + i, dv[q]);
-        store_nt(r + i, rv[q]);
+        store_nt(r_out + i, rv[q]);
         store_nt(z + i, zt);
         store_nt(sd + i, zt / theta);
     }
     TL_TILE_LOOP_END
+}
+
+// One PPCG smoothing step on a strip once A sd is known (res = A sd, sc = sd):
+// r -= A sd rounded to the storage, sd' = a_i sd + b_i M^-1 r rounded to the
+// storage (tea_leaf_ppcg_cl.cl:62-155). Shared by the one-step and two-step
+// kernels, with the FMA placed explicitly, so that both round identically.
+template <typename T, int PREC>
+__device__ __forceinline__ void rf_ppcg_step(const Geom &g, int ks, int k1, int i0, const double *kx0,
+                                             const double *kx1, const double *ky, const double *res,
+                                             const double *sc, double ca, double cb, double *rv, double *sdn)
+{
+    double mv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = rnd<T>(rv[q] - res[q]);
+    mask_rows(ks, k1, rv);
+    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, mv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) sdn[q] = rnd<T>(fma(ca, sc[q], cb * mv[q]));
 }
 
 // PPCG inner step (tea_leaf_ppcg_cl.cl:62-155, fused update_r + calc_sd):
@@ -353,30 +372,168 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d,
     double acc[2] = {0.0, 0.0};
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
-    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], dv[STRIP], rv[STRIP], mv[STRIP],
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], dv[STRIP], rv[STRIP], sdn[STRIP],
         zo[STRIP];
     strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     if (LAST) col4_nt(z, i0, P, zo);
     strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) rv[q] = rnd<T>(rv[q] - res[q]);
-    mask_rows(ks, e.k1, rv);
-    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, mv);
+    rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca, cb, rv, sdn);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
         store_nt(d + i, dv[q] + sc[q]);
         store_nt(r + i, rv[q]);
-        store_nt(sd_out + i, ca * sc[q] + cb * mv[q]);
+        store_nt(sd_out + i, sdn[q]);
         if (LAST) {
             acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
             acc[1] += rv[q] * rv[q];
         }
     }
     TL_TILE_LOOP_END
+    if (LAST) block_partial<2>(acc, ra.partials);
+}
+
+// face coefficients of a strip for the two-step kernel: read from the
+// chunk's Kx, Ky for FP64 inner fields (bitwise what the one-step kernel
+// recomputes from the conductivity, without its divisions), formed from
+// 1/(2w) for FP32 inner fields (no division either way)
+template <typename T>
+__device__ __forceinline__ void rf_faces(const Geom &g, const Faces &fc, const T *W, const double *Kx,
+                                         const double *Ky, int i0, int P, int j, int ks, double *kx0, double *kx1,
+                                         double *ky)
+{
+    if (sizeof(T) == 8) strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    else strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+}
+
+// sd at a ring cell the block does not compute: the sd buffer's halo cell
+// (what the one-step kernel reads there), 0 beyond the halo
+template <typename T>
+__device__ __forceinline__ double rf_ring_outside(const Geom &g, const Ext &e, const T *sd, int j, int k)
+{
+    const bool in = j >= e.j0 - 1 && j <= e.j1 + 1 && k >= e.k0 - 1 && k <= e.k1 + 1;
+    return in ? (double)sd[IX(g, j, k)] : 0.0;
+}
+
+// Two PPCG inner steps per launch, as k_fused_ppcg_inner2 does for the FP64
+// driver: step 1 on the 64x16 tile and on the ring step 2 reads (the 4-row
+// strips above and below, the tile's strips in the columns either side), the
+// ring's sd through LDS, step 2 on the tile. Values are rounded to the
+// storage exactly where the one-step kernel stores and reloads them, so two
+// one-step launches and one two-step launch give bitwise the same d, r, sd.
+// r and sd are read from one buffer and written to another (neighbouring
+// blocks read this tile's r and sd in their step 1).
+template <typename T, int PREC, int LAST>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, T *d, const T *r_in, T *r_out,
+                                                             const T *sd_in, T *sd_out, const T *W, Faces fc,
+                                                             const double *Kx, const double *Ky, const T *z,
+                                                             const double *ch_a, const double *ch_b, int step,
+                                                             RfArgs ra)
+{
+    const RefineState *st = ra.st;
+    if (st->halt) return;
+    constexpr int TH = STRIPS_PER_TILE * STRIP;
+    constexpr int RW = TILE_X + 2, RH = TH + 2;
+    __shared__ double ring[RH][RW];
+    const double ca1 = ch_a[step], cb1 = ch_b[step], ca2 = ch_a[step + 1], cb2 = ch_b[step + 1];
+    const int P = g.pitch;
+    const int lane = threadIdx.x & (TILE_X - 1), wv = threadIdx.x / TILE_X;
+    const int nt = e.ntx * e.nty;
+    double acc[2] = {0.0, 0.0};
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
+        const int j = jt + lane, ks = kt + wv * STRIP;
+        const bool act = j <= e.j1 && ks <= e.k1;
+        const int i0 = IX(g, j, ks);
+        // step 1 on the ring: strip above (wave 0), strip below (wave 1), the
+        // tile's strips in the columns either side (wave 2, lanes 0-7)
+        {
+            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
+            if (wv == 0) {
+                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
+            } else if (wv == 1) {
+                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
+            } else if (wv == 2 && lane < 2 * STRIPS_PER_TILE) {
+                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
+                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
+                row0 = 1 + qs * STRIP;
+            }
+            if (hj >= 0) {
+                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
+                double sdn[STRIP];
+                if (inside) {
+                    const int h0 = IX(g, hj, hk);
+                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP];
+                    rf_faces<T>(g, fc, W, Kx, Ky, h0, P, hj, hk, hx0, hx1, hy);
+                    col4(r_in, h0, P, hr);
+                    strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
+                    rf_ppcg_step<T, PREC>(g, hk, e.k1, h0, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn);
+                }
+                if (take < STRIP) {
+                    const int k = hk + take;
+                    if (hj <= e.j1 + 1)
+                        ring[row0][col] = (inside && k <= e.k1) ? sdn[take] : rf_ring_outside(g, e, sd_out, hj, k);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < STRIP; ++q)
+                        ring[row0 + q][col] =
+                            (inside && hk + q <= e.k1) ? sdn[q] : rf_ring_outside(g, e, sd_out, hj, hk + q);
+                }
+            }
+        }
+        // step 1 on the own strip; its faces, r and d are kept for step 2
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], dv[STRIP];
+        if (act) {
+            double res[STRIP], sc[STRIP], sdn[STRIP];
+            rf_faces<T>(g, fc, W, Kx, Ky, i0, P, j, ks, kx0, kx1, ky);
+            col4_nt(d, i0, P, dv);
+            col4_nt(r_in, i0, P, rv);
+            strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+            rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                dv[q] = rnd<T>(dv[q] + sc[q]);
+                ring[1 + wv * STRIP + q][1 + lane] =
+                    (ks + q <= e.k1) ? sdn[q] : rf_ring_outside(g, e, sd_out, j, ks + q);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q)
+                ring[1 + wv * STRIP + q][1 + lane] = rf_ring_outside(g, e, sd_out, j, ks + q);
+        }
+        __syncthreads();
+        // step 2 on the own strip, sd from the ring
+        if (act) {
+            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP];
+#pragma unroll
+            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv * STRIP + q][1 + lane];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                w4[q] = ring[1 + wv * STRIP + q][lane];
+                e4[q] = ring[1 + wv * STRIP + q][lane + 2];
+                sc[q] = c6[q + 1];
+            }
+            if (LAST) col4_nt(z, i0, P, zo);
+            strip_apply(c6, w4, e4, kx0, kx1, ky, res);
+            rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks + q > e.k1) continue;
+                const int i = i0 + q * P;
+                store_nt(d + i, dv[q] + sc[q]);
+                store_nt(r_out + i, rv[q]);
+                store_nt(sd_out + i, sdn[q]);
+                if (LAST) {
+                    acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+                    acc[1] += rv[q] * rv[q];
+                }
+            }
+        }
+        __syncthreads();
+    }
     if (LAST) block_partial<2>(acc, ra.partials);
 }
 
@@ -486,6 +643,7 @@ int rf_alloc_impl(tl_chunk *c, int which)
         w.f[RW_P2] = c->f2[F_P2];
         w.f[RW_SD] = c->f2[F_SD];
         w.f[RW_SD2] = c->f2[F_SD2];
+        w.f[RW_R2] = c->f2[F_R2];
     }
     if ((e = hipMalloc(&w.hist, sizeof(double) * 2 * RF_HIST)) != hipSuccess ||
         (e = hipMalloc(&w.coef, sizeof(double) * 2 * RF_MAX_INNER)) != hipSuccess)
@@ -533,18 +691,38 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
                        f[RW_W], c->faces, ra_w);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
     const double *ca = w.coef, *cb = w.coef + RF_MAX_INNER;
+    // two inner steps per launch: r and sd flip between their two buffers at
+    // every pair; the outer kernel writes r to the second buffer when the
+    // number of pairs is odd, so that every outer step ends with r in RW_R
+    const int pairs = c->ppcg_steps_per_launch == 2 ? inner / 2 : 0;
+    T *r_cur = (pairs & 1) ? f[RW_R2] : f[RW_R], *r_oth = (pairs & 1) ? f[RW_R] : f[RW_R2];
+    T *sd_cur = f[RW_SD], *sd_oth = f[RW_SD2];
+    const double *Kx = c->f2[F_KX], *Ky = c->f2[F_KY];
 #define TL_RFO(P)                                                                                              \
     do {                                                                                                       \
         hipLaunchKernelGGL((k_rf_ur_ppcg<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D],     \
-                           f[RW_P], f[RW_P2], f[RW_R], f[RW_Z], f[RW_SD], f[RW_W], c->faces, ra_u);             \
-        for (int i = 0; i < inner; ++i) {                                                                      \
-            T *sin = (i & 1) ? f[RW_SD2] : f[RW_SD], *sout = (i & 1) ? f[RW_SD] : f[RW_SD2];                   \
+                           f[RW_P], f[RW_P2], f[RW_R], r_cur, f[RW_Z], f[RW_SD], f[RW_W], c->faces, ra_u);      \
+        int i = 0;                                                                                             \
+        for (int k = 0; k < pairs; ++k, i += 2) {                                                              \
+            if (i + 2 == inner)                                                                                \
+                hipLaunchKernelGGL((k_rf_inner2<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
+                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], c->faces, Kx, Ky, f[RW_Z],   \
+                                   ca, cb, i, ra_u);                                                           \
+            else                                                                                               \
+                hipLaunchKernelGGL((k_rf_inner2<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
+                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], c->faces, Kx, Ky, f[RW_Z],   \
+                                   ca, cb, i, ra_u);                                                           \
+            std::swap(r_cur, r_oth);                                                                           \
+            std::swap(sd_cur, sd_oth);                                                                         \
+        }                                                                                                      \
+        for (; i < inner; ++i) {                                                                               \
             if (i == inner - 1)                                                                                \
                 hipLaunchKernelGGL((k_rf_inner<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
-                                   f[RW_D], f[RW_R], sin, sout, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u);   \
+                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u); \
             else                                                                                               \
                 hipLaunchKernelGGL((k_rf_inner<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
-                                   f[RW_D], f[RW_R], sin, sout, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u);   \
+                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u); \
+            std::swap(sd_cur, sd_oth);                                                                         \
         }                                                                                                      \
     } while (0)
     if (c->prec == PREC_BLOCK) TL_RFO(PREC_BLOCK);
@@ -591,7 +769,7 @@ int ppcg_batch(int inner) { return inner + 2 >= 32 ? 1 : 32 / (inner + 2); }
 template <typename T>
 int rf_capture_ppcg(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
 {
-    return capture_into(c, w.graph_ppcg, c->prec * 1000 + inner, [&]() {
+    return capture_into(c, w.graph_ppcg, (c->prec * 1000 + inner) * 4 + c->ppcg_steps_per_launch, [&]() {
         for (int b = 0; b < ppcg_batch(inner); ++b) launch_ppcg_outer<T>(c, w, inner);
     });
 }

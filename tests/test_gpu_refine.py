@@ -155,6 +155,29 @@ def test_ppcg_true_residual(hiplib, oracle, x, y, hd, prec, precision):
     assert res["iterations"] < cg["iterations"], (res, cg)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("prec,inner", [(1, 7), (2, 6), (3, 8), (3, 5)])
+@pytest.mark.parametrize("x,y,hd", [(130, 70, 2), (200, 41, 1)])
+def test_ppcg_two_steps_per_launch_bitwise(hiplib, oracle, x, y, hd, prec, inner, precision):
+    """Refined PPCG with two smoothing steps per launch (the tile's ring
+    recomputed, r and sd flipped between buffers) against one step per
+    launch: bitwise the same solution and the same iteration counts, with
+    FP64 and FP32 inner fields, odd and even inner counts."""
+    if prec == 3 and hd > 1:
+        pytest.skip("read_input.f90:299-301 rejects PPCG + block Jacobi with halo_depth > 1")
+    out = []
+    for per in (1, 2):
+        g, _ = pair(hiplib, oracle, x, y, hd, prec, 2)
+        hiplib.set_tuning(hiplib.TUNE_PPCG_STEPS_PER_LAUNCH, per, chunk=g)
+        res = g.solve_refined_ppcg(precision, tol=1e-10, max_iters=5000, presteps=20, inner_steps=inner)
+        assert res["converged"], res
+        out.append((res, g.get_field("u")))
+    (r1, u1), (r2, u2) = out
+    for k in ("iterations", "inner_iterations", "updates", "rel_residual"):
+        assert r1[k] == r2[k], (k, r1, r2)
+    assert np.array_equal(u1, u2)
+
+
 @pytest.mark.parametrize("eps,bound", [(1e-10, 2e-7), (1e-12, 2e-9)])
 def test_ppcg_deck_summary(hiplib, oracle, tmp_path, eps, bound):
     """A PPCG deck with tl_mixed_precision: field summary against the
