@@ -181,6 +181,26 @@ def test_full_size_native_vs_dropin(hiplib, tmp_path, solver):
         assert b["solve"][0][1] < 5000
 
 
+@pytest.mark.parametrize("name,slack", [("cg", 2), ("chebyshev", 0), ("ppcg", 50)])
+def test_config2_full_size_vs_oracle(hiplib, tmp_path, name, slack):
+    """The 4096^2 deck (two rectangles and a circle, block Jacobi, eps 1e-15)
+    on the device-resident driver against the FP64 CPU oracle's field
+    summaries after every timestep (fixtures written by
+    tests/golden/make_full_size.py: CG and Chebyshev over 20 timesteps, PPCG
+    over 2), to the 1e-11 bar, with the iteration counts."""
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "config2_4096_%s.json" % name)
+    if not os.path.exists(path):
+        pytest.skip("fixture %s not generated" % os.path.basename(path))
+    with open(path) as f:
+        fx = json.load(f)
+    deck = decks.config2_deck(str(tmp_path / "c2.in"), solver=fx["solver"], n=fx["n"], steps=fx["steps"])
+    lg = hiplib.run_deck(deck, tmp_path / "c2.log", mode=1)
+    lo = {"summary": fx["summary"], "solve": fx["solve"]}
+    _compare_summaries(lg, lo)
+    _compare_solves(lg, lo, slack)
+
+
 @pytest.mark.parametrize("kat", [2, 3])
 def test_reference_kat_large(hiplib, decks_dir, tmp_path, kat):
     """960^2 (87 steps) and 3840^2 (10 steps) CG known-answer tests, native mode."""
