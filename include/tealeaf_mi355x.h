@@ -178,10 +178,11 @@ int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
 
 /* Tuning of the native driver's fused kernels. No reference counterpart.
  * TL_TUNE_PPCG_STEPS_PER_LAUNCH: PPCG inner steps per fused launch, 1 or 2
- * (default 2); both give bitwise the same fields. With c null the value
- * becomes the default of chunks created afterwards (tl_run_deck creates its
- * own chunk). */
-enum { TL_TUNE_PPCG_STEPS_PER_LAUNCH = 1 };
+ * (default 2); TL_TUNE_CHEBY_STEPS_PER_LAUNCH: Chebyshev iterations per fused
+ * launch, 1 or 2 (default 2). Either way the fields are bitwise the same.
+ * With c null the value becomes the default of chunks created afterwards
+ * (tl_run_deck creates its own chunk). */
+enum { TL_TUNE_PPCG_STEPS_PER_LAUNCH = 1, TL_TUNE_CHEBY_STEPS_PER_LAUNCH = 2 };
 int tl_set_tuning(tl_chunk *c, int key, int value);
 /* PPCG smoothing steps first_step .. first_step+nsteps-1 (1-based, as the
  * inner_step of tl_ppcg_inner) with the native

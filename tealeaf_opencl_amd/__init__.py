@@ -28,7 +28,7 @@ PREC_NONE, PREC_JAC_DIAG, PREC_JAC_BLOCK = 1, 2, 3
 SOLVER_JACOBI, SOLVER_CG, SOLVER_CHEBY, SOLVER_PPCG = 1, 2, 3, 4
 COEF_CONDUCTIVITY, COEF_RECIP_CONDUCTIVITY = 1, 2
 MODE_DROPIN, MODE_NATIVE = 0, 1
-TUNE_PPCG_STEPS_PER_LAUNCH = 1
+TUNE_PPCG_STEPS_PER_LAUNCH, TUNE_CHEBY_STEPS_PER_LAUNCH = 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 G_RECT, G_CIRC, G_POINT = 1, 2, 3
 

@@ -59,6 +59,8 @@ struct tl_chunk {
     // PPCG inner steps per fused launch in the native driver (1 or 2;
     // tl_set_tuning TL_TUNE_PPCG_STEPS_PER_LAUNCH)
     int ppcg_steps_per_launch = 2;
+    // Chebyshev iterations per fused launch (1 or 2; TL_TUNE_CHEBY_STEPS_PER_LAUNCH)
+    int cheby_steps_per_launch = 2;
     // events around the eagerly launched first CG iteration of each batch
     // (one set per pipeline slot)
     hipEvent_t ev_t[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};

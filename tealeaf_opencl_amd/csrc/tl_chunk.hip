@@ -200,7 +200,7 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
 
 // Process-wide default of the tuning keys (tl_set_tuning with no chunk),
 // taken by chunks created later.
-static int g_ppcg_steps_default = 2;
+static int g_ppcg_steps_default = 2, g_cheby_steps_default = 2;
 
 extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int preconditioner, int profiler_on,
                                tl_chunk **out)
@@ -222,6 +222,7 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     c->g.hd = halo_depth;
     c->prec = preconditioner;
     c->ppcg_steps_per_launch = g_ppcg_steps_default;
+    c->cheby_steps_per_launch = g_cheby_steps_default;
     c->profiler_on = profiler_on;
     // pitch: padded row plus one slack column, rounded to 16 doubles (128 B)
     const int w = x_max + 2 * halo_depth + 2;
@@ -825,12 +826,13 @@ extern "C" int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *byte
 
 extern "C" int tl_set_tuning(tl_chunk *c, int key, int value)
 {
-    if (key != TL_TUNE_PPCG_STEPS_PER_LAUNCH || (value != 1 && value != 2)) {
+    if ((key != TL_TUNE_PPCG_STEPS_PER_LAUNCH && key != TL_TUNE_CHEBY_STEPS_PER_LAUNCH) || (value != 1 && value != 2)) {
         set_error("tl_set_tuning: bad key %d or value %d", key, value);
         return TL_ERR_ARG;
     }
-    if (c) c->ppcg_steps_per_launch = value;
-    else g_ppcg_steps_default = value;
+    int &slot = key == TL_TUNE_PPCG_STEPS_PER_LAUNCH ? (c ? c->ppcg_steps_per_launch : g_ppcg_steps_default)
+                                                     : (c ? c->cheby_steps_per_launch : g_cheby_steps_default);
+    slot = value;
     return TL_OK;
 }
 

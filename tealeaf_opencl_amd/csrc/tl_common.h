@@ -103,6 +103,7 @@ enum RedMode {
     RED_PPCG_RRN,         // zrnorm -> beta, error, exit test (:255-266)
     RED_JACOBI,           // error (:301-307)
     RED_CHEBY,            // optional ||r||^2 then iteration tail (:216-229)
+    RED_CHEBY2,           // two Chebyshev iterations (n odd): two tails
     RED_EXACT             // tl_check_result residual (:334-350)
 };
 

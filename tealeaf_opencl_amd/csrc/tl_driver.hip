@@ -426,11 +426,25 @@ int cg_graph(tl_chunk *c, bool timed)
     });
 }
 
-int cheby_graph(tl_chunk *c)
+// 16 Chebyshev iterations per replay: one per launch, or (pairs) two per
+// launch when the first iteration of the replay is odd, so that the residual
+// norm of tea_solve.f90:223 (iterations divisible by 10) always falls on the
+// second iteration of a pair. Pairs flip u and p between their two buffers
+// at every launch (8 flips per replay).
+int cheby_graph(tl_chunk *c, bool pairs)
 {
     const int B = 16;
     const Ext in = ext_interior(c);
-    return capture(c, c->g_cheby, c->prec, [&]() {
+    return capture(c, c->g_cheby, c->prec * 4 + (pairs ? 2 : 1), [&]() {
+        if (pairs) {
+            for (int b = 0; b < B / 2; ++b) {
+                const bool odd = b & 1;
+                launch_fused_cheby2(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[odd ? F_U2 : F_U],
+                                    c->f2[odd ? F_U : F_U2], c->f2[odd ? F_P2 : F_P], c->f2[odd ? F_P : F_P2],
+                                    c->f2[F_KX], c->f2[F_KY], c->ch_a, c->ch_b, red_args(c, RED_CHEBY2, true));
+            }
+            return;
+        }
         for (int b = 0; b < B; ++b)
             launch_fused_cheby(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[F_U], c->f2[F_U2], c->f2[F_P],
                                c->f2[F_COND], c->faces, c->ch_a, c->ch_b, red_args(c, RED_CHEBY, true));
@@ -571,7 +585,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(state_download(c, &h));
     }
     int est_itc = 0;
-    bool cheby_fused = false;
+    bool cheby_fused = false, cheby_pairs = false;
     if (!h.done && h.switch_n > 0 && (use_ch || use_ppcg)) {
         const int n = h.switch_n;
         const int max_cheby_iters = cfg.max_iters - n + 2;
@@ -634,7 +648,8 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
             }
             RC(state_upload(c, h));
             if (!h.halt) {
-                RC(cheby_graph(c));
+                cheby_pairs = c->cheby_steps_per_launch == 2 && (h.n_fused0 & 1);
+                RC(cheby_graph(c, cheby_pairs));
                 RC(run_batches(c, c->g_cheby.exec, false, cfg.max_iters / 16 + 4));
             }
         } else {
@@ -649,7 +664,11 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(state_download(c, &h));
     }
     const int executed = (h.exit_n > cfg.max_iters) ? cfg.max_iters : h.exit_n;
-    if ((use_jac && (executed & 1)) || (cheby_fused && ((executed - h.n_fused0 + 1) & 1))) {
+    // launches of the double-buffered Chebyshev kernels (a pair cut short by
+    // the exit test still flipped the buffers once)
+    const int cheby_launches =
+        cheby_fused ? (cheby_pairs ? (executed - h.n_fused0 + 2) / 2 : executed - h.n_fused0 + 1) : 0;
+    if ((use_jac && (executed & 1)) || (cheby_launches & 1)) {
         // the last sweep wrote the second buffer
         launch_copy(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U2], c->f2[F_U]);
     }

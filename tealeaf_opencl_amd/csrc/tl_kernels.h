@@ -98,6 +98,11 @@ void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int ca
                               const double *r_in, double *r_out, const double *sd_in, double *sd_out,
                               const double *Kx, const double *Ky, const double *z, const double *ch_a,
                               const double *ch_b, int step, const RedArgs &ra);
+// two Chebyshev iterations (n odd) per launch; u and p are read from one
+// buffer and written to the other
+void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
+                         const double *u_in, double *u_out, const double *p_in, double *p_out, const double *Kx,
+                         const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra);
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                         double *uA, double *uB, double *p, const double *W, const Faces &fc,
                         const double *ch_a, const double *ch_b, const RedArgs &ra);

@@ -117,6 +117,15 @@ __device__ void finish_scalars(const RedArgs &ra, const double *tot)
         s->cheby_calc_steps += 1;
         iteration_tail(s, ra.hist_e);
         break;
+    case RED_CHEBY2:
+        // two iterations; the first (odd n) never forms the norm
+        s->cheby_calc_steps += 1;
+        iteration_tail(s, ra.hist_e);
+        if (s->halt) break;
+        if (cheby_norm_due(s)) s->error = tot[0];
+        s->cheby_calc_steps += 1;
+        iteration_tail(s, ra.hist_e);
+        break;
     case RED_EXACT:
         s->exact_error = sqrt(tot[0]);
         break;
@@ -128,7 +137,7 @@ __device__ void finish_scalars(const RedArgs &ra, const double *tot)
 __device__ __forceinline__ bool iteration_mode(int mode)
 {
     return mode == RED_CG_PW || mode == RED_CG_RRN || mode == RED_PPCG_PW || mode == RED_PPCG_RRN ||
-           mode == RED_JACOBI || mode == RED_CHEBY;
+           mode == RED_JACOBI || mode == RED_CHEBY || mode == RED_CHEBY2;
 }
 
 // Sums the partials of the preceding reduction kernel in block order and runs
@@ -1106,6 +1115,30 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
     if (LAST) block_partial<1>(acc, ra.partials);
 }
 
+// One Chebyshev iteration on a strip once A u is known (wv = A u, uc = u):
+// r = u0 - A u, z = M^-1 r, p' = ca p + cb z, u' = u + p'
+// (tea_leaf_cheby_cl.cl:66-145). Shared by the one- and two-iteration kernels,
+// with the FMA placed explicitly, so that both round identically.
+template <int PREC>
+__device__ __forceinline__ void cheby_step(const Geom &g, int ks, int k1, const double *kx0, const double *kx1,
+                                           const double *ky, const double *b, const double *wv, const double *uc,
+                                           const double *pv, double ca, double cb, double *rv, double *pn,
+                                           double *un)
+{
+    double zv[STRIP];
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = b[q] - wv[q];
+    mask_rows(ks, k1, rv);
+    if (PREC == PREC_BLOCK) strip_block(g, ks, 0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (PREC == PREC_BLOCK) pn[q] = fma(ca, pv[q], cb * zv[q]);
+        else if (PREC == PREC_DIAG) pn[q] = fma(ca, pv[q], cb * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])) * rv[q]);
+        else pn[q] = fma(ca, pv[q], cb * rv[q]);
+        un[q] = uc[q] + pn[q];
+    }
+}
+
 // Chebyshev iteration: tea_leaf_cheby_solve_calc_p (tea_leaf_cheby_cl.cl:78-145)
 // then tea_leaf_cheby_solve_calc_u (:66-76) in one pass: r = u0 - A u,
 // z = M^-1 r, p = ca p + cb z, u' = u + p, and r.r when tea_solve.f90:223 asks
@@ -1132,28 +1165,150 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby(Geom g, Ext e, co
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], pv[STRIP], wv[STRIP], uc[STRIP], rv[STRIP],
-        zv[STRIP];
+        pn[STRIP], uv[STRIP];
     strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(u0, i0, P, b);
     col4_nt(p, i0, P, pv);
     strip_matvec(u, kx0, kx1, ky, i0, P, wv, uc);
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) rv[q] = b[q] - wv[q];
-    mask_rows(ks, e.k1, rv);
-    if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+    cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pv, ca, cb, rv, pn, uv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        double np;
-        if (PREC == PREC_BLOCK) np = ca * pv[q] + cb * zv[q];
-        else if (PREC == PREC_DIAG) np = ca * pv[q] + cb * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])) * rv[q];
-        else np = ca * pv[q] + cb * rv[q];
-        store_nt(p + i, np);
-        store_nt(un + i, uc[q] + np);
+        store_nt(p + i, pn[q]);
+        store_nt(un + i, uv[q]);
         if (want) acc[0] += rv[q] * rv[q];
     }
     TL_TILE_LOOP_END
+    block_partial<1>(acc, ra.partials);
+}
+
+// Two Chebyshev iterations n, n+1 per launch, n odd (tea_solve.f90:223 forms
+// the residual norm only on iterations divisible by 10, so only the second one
+// can need it). The ring scheme of k_fused_ppcg_inner2: iteration n is
+// computed on the 64x16 tile and on the ring iteration n+1 reads (the 4-row
+// strips above and below, the tile's strips in the columns either side), the
+// ring's new u through LDS. u and p are read from one buffer and written to the
+// other. When the tail of iteration n will stop the solve (max_iters, or the
+// stale-error exit test of tea_solve.f90:322-328, both known from the state
+// before the launch) only iteration n is done. Faces are read from Kx, Ky
+// (bitwise what the one-iteration kernel recomputes). The finisher runs both
+// iterations' tails. Algorithmic bytes per two iterations: R u, u0, p, Kx, Ky
+// W u, p (56 B/cell) instead of 96.
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, const double *u0, const double *u_in,
+                                                                double *u_out, const double *p_in, double *p_out,
+                                                                const double *Kx, const double *Ky,
+                                                                const double *ch_a, const double *ch_b, RedArgs ra)
+{
+    const SolveState *st = ra.st;
+    if (st->halt) return;
+    constexpr int TH = STRIPS_PER_TILE * STRIP;
+    constexpr int RW = TILE_X + 2, RH = TH + 2;
+    __shared__ double ring[RH][RW];
+    const int n = st->n, step = st->cheby_calc_steps - 1;
+    const bool two = n < st->max_iters && !(fabs(sqrt(st->error)) < st->eps * st->initial_residual);
+    const bool want = two && n + 1 >= st->est_itc && ((n + 1) % 10) == 0;
+    const double ca1 = ch_a[step], cb1 = ch_b[step];
+    const double ca2 = two ? ch_a[step + 1] : 0.0, cb2 = two ? ch_b[step + 1] : 0.0;
+    const int P = g.pitch;
+    const int lane = threadIdx.x & (TILE_X - 1), wv_ = threadIdx.x / TILE_X;
+    const int nt = e.ntx * e.nty;
+    double acc[1] = {0.0};
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
+        const int j = jt + lane, ks = kt + wv_ * STRIP;
+        const bool act = j <= e.j1 && ks <= e.k1;
+        const int i0 = IX(g, j, ks);
+        if (two) {
+            // iteration n on the ring: strip above (wave 0), strip below (wave
+            // 1), the tile's strips in the columns either side (wave 2, lanes 0-7)
+            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
+            if (wv_ == 0) {
+                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
+            } else if (wv_ == 1) {
+                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
+            } else if (wv_ == 2 && lane < 2 * STRIPS_PER_TILE) {
+                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
+                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
+                row0 = 1 + qs * STRIP;
+            }
+            if (hj >= 0) {
+                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
+                double un[STRIP];
+                if (inside) {
+                    const int h0 = IX(g, hj, hk);
+                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hb[STRIP], hp[STRIP], hw[STRIP], hc[STRIP],
+                        hr[STRIP], hpn[STRIP];
+                    strip_k(Kx, Ky, h0, P, hx0, hx1, hy);
+                    col4(u0, h0, P, hb);
+                    col4(p_in, h0, P, hp);
+                    strip_matvec(u_in, hx0, hx1, hy, h0, P, hw, hc);
+                    cheby_step<PREC>(g, hk, e.k1, hx0, hx1, hy, hb, hw, hc, hp, ca1, cb1, hr, hpn, un);
+                }
+                if (take < STRIP) {
+                    const int k = hk + take;
+                    if (hj <= e.j1 + 1)
+                        ring[row0][col] = (inside && k <= e.k1) ? un[take] : ring_outside(g, e, u_out, hj, k);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < STRIP; ++q)
+                        ring[row0 + q][col] = (inside && hk + q <= e.k1) ? un[q] : ring_outside(g, e, u_out, hj, hk + q);
+                }
+            }
+        }
+        // iteration n on the own strip; faces, u0 and the new p are kept
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], pn[STRIP];
+        if (act) {
+            double pv[STRIP], wv[STRIP], uc[STRIP], rv[STRIP], un[STRIP];
+            strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+            col4_nt(u0, i0, P, b);
+            col4_nt(p_in, i0, P, pv);
+            strip_matvec(u_in, kx0, kx1, ky, i0, P, wv, uc);
+            cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pv, ca1, cb1, rv, pn, un);
+            if (!two) {
+#pragma unroll
+                for (int q = 0; q < STRIP; ++q) {
+                    if (ks + q > e.k1) continue;
+                    store_nt(p_out + i0 + q * P, pn[q]);
+                    store_nt(u_out + i0 + q * P, un[q]);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < STRIP; ++q)
+                    ring[1 + wv_ * STRIP + q][1 + lane] =
+                        (ks + q <= e.k1) ? un[q] : ring_outside(g, e, u_out, j, ks + q);
+            }
+        } else if (two) {
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) ring[1 + wv_ * STRIP + q][1 + lane] = ring_outside(g, e, u_out, j, ks + q);
+        }
+        if (!two) continue;
+        __syncthreads();
+        // iteration n+1 on the own strip, u from the ring
+        if (act) {
+            double c6[STRIP + 2], w4[STRIP], e4[STRIP], wv[STRIP], uc[STRIP], rv[STRIP], pn2[STRIP], un[STRIP];
+#pragma unroll
+            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv_ * STRIP + q][1 + lane];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                w4[q] = ring[1 + wv_ * STRIP + q][lane];
+                e4[q] = ring[1 + wv_ * STRIP + q][lane + 2];
+                uc[q] = c6[q + 1];
+            }
+            strip_apply(c6, w4, e4, kx0, kx1, ky, wv);
+            cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pn, ca2, cb2, rv, pn2, un);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks + q > e.k1) continue;
+                const int i = i0 + q * P;
+                store_nt(p_out + i, pn2[q]);
+                store_nt(u_out + i, un[q]);
+                if (want) acc[0] += rv[q] * rv[q];
+            }
+        }
+        __syncthreads();
+    }
     block_partial<1>(acc, ra.partials);
 }
 
@@ -1473,6 +1628,21 @@ void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int ca
         else TL_FPI2(PREC_NONE, 0);
     }
 #undef TL_FPI2
+}
+
+void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
+                         const double *u_in, double *u_out, const double *p_in, double *p_out, const double *Kx,
+                         const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra)
+{
+    if (prec == PREC_BLOCK)
+        TL_LAUNCH_RED(k_fused_cheby2<PREC_BLOCK>, e, cap, s, 1, ra, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
+                      ch_b, ra);
+    else if (prec == PREC_DIAG)
+        TL_LAUNCH_RED(k_fused_cheby2<PREC_DIAG>, e, cap, s, 1, ra, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
+                      ch_b, ra);
+    else
+        TL_LAUNCH_RED(k_fused_cheby2<PREC_NONE>, e, cap, s, 1, ra, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
+                      ch_b, ra);
 }
 
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,

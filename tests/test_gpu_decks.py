@@ -124,6 +124,38 @@ def test_ppcg_two_steps_per_launch_bitwise(hiplib, tmp_path, case):
     assert one["summary"] == two["summary"]
 
 
+@pytest.mark.parametrize("case", ["none-20-4000", "jac_diag-20-4000", "jac_block-20-4000", "jac_block-20-61",
+                                  "jac_block-20-62", "jac_block-21-4000", "jac_diag-30-4000-reflective"])
+def test_cheby_two_iterations_per_launch_bitwise(hiplib, tmp_path, case):
+    """The native Chebyshev with two iterations per launch (the tile's ring
+    recomputed, u and p flipped between buffers, the second tail run by the
+    same finisher) against one iteration per launch: bitwise the same field
+    summaries and solve lines, converged and stopped by odd and even
+    iteration caps; presteps 21 makes the first fused iteration even, which
+    keeps one iteration per launch."""
+    parts = case.split("-")
+    prec, presteps, max_iters = parts[0], int(parts[1]), int(parts[2])
+    extra = ["tl_ch_cg_presteps=%d" % presteps, "tl_ch_cg_epslim=1.0e300"]
+    kw = {}
+    if "reflective" in parts:
+        extra.append("reflective_boundary")
+        kw["coefficient"] = "tl_coefficient_inverse_density"
+    deck = decks.write_deck(str(tmp_path / "c.in"), 200, 148, decks.seeded_states(2, 2, True),
+                            solver="tl_use_chebyshev", preconditioner=prec, end_step=2, eps=1e-12,
+                            max_iters=max_iters, extra=extra, **kw)
+    try:
+        hiplib.set_tuning(hiplib.TUNE_CHEBY_STEPS_PER_LAUNCH, 1)
+        one = hiplib.run_deck(deck, tmp_path / "one.log", mode=1)
+    finally:
+        hiplib.set_tuning(hiplib.TUNE_CHEBY_STEPS_PER_LAUNCH, 2)
+    two = hiplib.run_deck(deck, tmp_path / "two.log", mode=1)
+    assert one["solve"] and np.array_equal(np.array(one["solve"], float), np.array(two["solve"], float),
+                                           equal_nan=True), (one["solve"], two["solve"])
+    assert one["summary"] == two["summary"]
+    if max_iters < 100:
+        assert [x[1] for x in two["solve"]] == [max_iters] * 2
+
+
 @pytest.mark.parametrize("prec", ["none", "jac_block"])
 @pytest.mark.parametrize("max_iters", [61, 62])
 def test_cheby_native_iteration_cap(hiplib, oracle, tmp_path, prec, max_iters):
