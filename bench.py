@@ -34,6 +34,8 @@ ALG_BYTES = {
     "k_fused_cg_ur": 7 * 8,
     # r = u0 - A u; p = ca p + cb M_blk^-1 r; u' = u + p : R u, u0, w, p  W p, u'
     "k_fused_cheby": 6 * 8,
+    # two such iterations per launch : R u, u0, p, Kx, Ky  W u, p
+    "k_fused_cheby2": 7 * 8,
     # r -= A sd; u += sd; sd' = ca sd + cb M_blk^-1 r : R u, r, w, sd  W u, r, sd'
     "k_fused_ppcg_inner": 7 * 8,
     # two such steps per launch : R u, r, sd, Kx, Ky  W u, r, sd
@@ -150,8 +152,8 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
     eigenvalue-estimate steps and PPCG with 10 inner steps, both with strip-4
     block Jacobi, tl_eps 1e-15, run by the device-resident driver (one untimed
     timestep, then `steps` timed ones). Bytes per iteration are algorithmic:
-    the fused Chebyshev iteration streams u, u0, w, p in and p, u out
-    (48 B/cell); a PPCG outer step is the two fused CG kernels (88 B/cell)
+    the fused Chebyshev iterations, two per launch, stream u, u0, p, Kx, Ky
+    in and u, p out (56 B/cell per two iterations); a PPCG outer step is the two fused CG kernels (88 B/cell)
     plus 10 inner steps, two per launch, each launch streaming u, r, sd, Kx,
     Ky in and u, r, sd out (64 B/cell). `reference_equiv_GBs` counts the
     reference's bytes for the same iterations instead (Chebyshev 104 B/cell,
@@ -169,7 +171,7 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
         if name == "chebyshev":
             # about 20 CG steps (two fused CG kernels) then Chebyshev iterations
             cg = 20 * steps
-            alg = (cg * cg_b + (its - cg) * ALG_BYTES["k_fused_cheby"]) * n * n
+            alg = (cg * cg_b + (its - cg) / 2 * ALG_BYTES["k_fused_cheby2"]) * n * n
             ref = (cg * REF_BYTES["cg"] + (its - cg) * REF_BYTES["cheby"]) * n * n
         else:
             alg = (its * cg_b + inner // 2 * ALG_BYTES["k_fused_ppcg_inner2"]) * n * n
