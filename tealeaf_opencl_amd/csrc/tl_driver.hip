@@ -534,7 +534,7 @@ int jacobi_graph(tl_chunk *c)
     const Ext in = ext_interior(c);
     return capture(c, c->g_jacobi, 1, [&]() {
         for (int b = 0; b < B; ++b)
-            launch_fused_jacobi(c->s, c->g, in, c->cap, c->f2[F_KX], c->f2[F_KY], c->f2[F_U0], c->f2[F_U],
+            launch_fused_jacobi(c->s, c->g, in, c->cap, c->f2[F_COND], c->faces, c->f2[F_U0], c->f2[F_U],
                                 c->f2[F_U2], red_args(c, RED_JACOBI, true));
     });
 }

@@ -86,8 +86,8 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
                         const double *pA, const double *pB, const double *r, double *r_out, double *z,
                         double *sd, const double *W, const Faces &fc, const RedArgs &ra,
                         hipEvent_t ev_main = nullptr);
-void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
-                         const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra);
+void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
+                         const double *u0, double *uA, double *uB, const RedArgs &ra);
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
                              const double *W, const Faces &fc, const double *z, const double *ch_a,

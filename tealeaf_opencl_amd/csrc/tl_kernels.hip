@@ -794,9 +794,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_calc_sd(Geom g, Ext e, c
 // previous iterate. The device-resident driver passes the two buffers of a
 // pair and st: iteration n reads buffer (n-1)&1 (this replaces the
 // reference's copy of u into the Mi buffer, tea_leaf_jacobi_cl.cl:3-13).
+// COND=1 (device-resident driver) forms the faces from the conductivity W
+// (bitwise the stored Kx, Ky) instead of reading Kx, Ky: 32 instead of 40
+// B/cell.
+template <int COND>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const double *Kx,
-                                                          const double *Ky, const double *u0,
-                                                          double *uA, double *uB, RedArgs ra)
+                                                          const double *Ky, const double *W, Faces fc,
+                                                          const double *u0, double *uA, double *uB, RedArgs ra)
 {
     if (halted(ra.st)) return;
     const double *un = uB;
@@ -809,7 +813,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const d
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], c6[STRIP + 2], w4[STRIP], e4[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    if (COND) strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    else strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
     col4(u0, i0, P, b);
     col6(un, i0, P, c6);
     col4(un, i0 - 1, P, w4);
@@ -1544,7 +1549,7 @@ void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
                    const double *u0, double *u1, double *un, const RedArgs &ra)
 {
-    TL_LAUNCH_RED(k_jacobi, e, cap, s, 1, ra, g, e, Kx, Ky, u0, u1, un, ra);
+    TL_LAUNCH_RED(k_jacobi<0>, e, cap, s, 1, ra, g, e, Kx, Ky, nullptr, Faces{}, u0, u1, un, ra);
 }
 
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
@@ -1554,10 +1559,10 @@ void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, cons
     TL_LAUNCH_RED_EV(k_fused_cg_w, e, cap, s, 1, ra, ev_main, g, e, src, pA, pB, W, fc, ra);
 }
 
-void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx,
-                         const double *Ky, const double *u0, double *uA, double *uB, const RedArgs &ra)
+void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
+                         const double *u0, double *uA, double *uB, const RedArgs &ra)
 {
-    TL_LAUNCH_RED(k_jacobi, e, cap, s, 1, ra, g, e, Kx, Ky, u0, uA, uB, ra);
+    TL_LAUNCH_RED(k_jacobi<1>, e, cap, s, 1, ra, g, e, nullptr, nullptr, W, fc, u0, uA, uB, ra);
 }
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
