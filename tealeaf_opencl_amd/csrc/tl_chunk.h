@@ -59,6 +59,11 @@ struct tl_chunk {
     // PPCG inner steps per fused launch in the native driver (1 or 2;
     // tl_set_tuning TL_TUNE_PPCG_STEPS_PER_LAUNCH)
     int ppcg_steps_per_launch = 2;
+    // cp, bfp hold the block-Jacobi factors of the current Kx, Ky (set by
+    // tl_block_init, cleared by whatever rewrites Kx, Ky, cp or bfp): the
+    // drop-in kernels then recompute the factors in registers (bitwise the
+    // stored ones) instead of reading them
+    bool factors_current = false;
     // Chebyshev iterations per fused launch (1 or 2; TL_TUNE_CHEBY_STEPS_PER_LAUNCH)
     int cheby_steps_per_launch = 2;
     // events around the eagerly launched first CG iteration of each batch

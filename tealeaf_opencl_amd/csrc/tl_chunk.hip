@@ -333,6 +333,7 @@ extern "C" int tl_field_set(tl_chunk *c, int field, const double *host)
         return TL_ERR_ARG;
     }
     const int w = c->g.x_max + 2 * c->g.hd, h = c->g.y_max + 2 * c->g.hd;
+    if (field == F_KX || field == F_KY || field == F_CP || field == F_BFP) c->factors_current = false;
     hipError_t e = hipMemcpy2DAsync(c->f2[field], sizeof(double) * c->g.pitch, host, sizeof(double) * w,
                                     sizeof(double) * w, h, hipMemcpyHostToDevice, c->s);
     if (e != hipSuccess) return hip_fail("hipMemcpy2D", e);
@@ -478,6 +479,7 @@ extern "C" int tl_common_init(tl_chunk *c, int coefficient, double dt, double *r
     TL_CHECK_LAUNCH(c, "tea_leaf_init_common");
     prof_end(c, "tea_leaf_init_common");
     // the same coefficients in recomputable form for the fused kernels
+    c->factors_current = false;
     launch_conductivity(c->s, c->g, c->f2[F_DENSITY], c->f2[F_COND], coefficient);
     TL_CHECK_LAUNCH(c, "conductivity");
     c->faces.rx = *rx;
@@ -547,6 +549,7 @@ extern "C" int tl_block_init(tl_chunk *c)
     prof_begin(c);
     launch_block_init(c->s, c->g, ext_interior(c), c->cap, c->f2[F_CP], c->f2[F_BFP], c->f2[F_KX], c->f2[F_KY]);
     TL_CHECK_LAUNCH(c, "tea_leaf_block_init");
+    c->factors_current = true;
     prof_end(c, "tea_leaf_block_init");
     return TL_OK;
 }
@@ -610,11 +613,18 @@ extern "C" int tl_cg_calc_w(tl_chunk *c, double *pw)
     return read_result(c, pw, 1);
 }
 
+// block-Jacobi factors for the per-iteration drop-in kernels: null (recompute
+// from Kx, Ky in registers, bitwise the stored values) while the stored ones
+// belong to the current Kx, Ky; the stored arrays otherwise, as the
+// reference reads them (tea_block_jacobi.cl:44-76)
+static const double *stored_cp(tl_chunk *c) { return c->factors_current ? nullptr : c->f2[F_CP]; }
+static const double *stored_bfp(tl_chunk *c) { return c->factors_current ? nullptr : c->f2[F_BFP]; }
+
 extern "C" int tl_cg_calc_ur(tl_chunk *c, double alpha, double *rrn)
 {
     prof_begin(c);
     launch_cg_calc_ur(c->s, c->g, ext_interior(c), c->cap, c->prec, alpha, c->f2[F_U], c->f2[F_P], c->f2[F_R],
-                      c->f2[F_W], c->f2[F_Z], c->f2[F_CP], c->f2[F_BFP], c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
+                      c->f2[F_W], c->f2[F_Z], stored_cp(c), stored_bfp(c), c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
                       red_args(c, RED_STORE, false));
     TL_CHECK_LAUNCH(c, "tea_leaf_cg_solve_calc_ur");
     prof_end(c, "tea_leaf_cg_solve_calc_ur");
@@ -670,7 +680,7 @@ extern "C" int tl_cheby_iterate(tl_chunk *c, int cheby_calc_step)
     }
     prof_begin(c);
     launch_cheby_p(c->s, c->g, ext_interior(c), c->cap, c->prec, 0, c->f2[F_U], c->f2[F_U0], c->f2[F_P],
-                   c->f2[F_R], c->f2[F_W], c->f2[F_CP], c->f2[F_BFP], c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
+                   c->f2[F_R], c->f2[F_W], stored_cp(c), stored_bfp(c), c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
                    c->ch_a, c->ch_b, c->theta, step, nullptr);
     TL_CHECK_LAUNCH(c, "tea_leaf_cheby_solve_calc_p");
     prof_end(c, "tea_leaf_cheby_solve_calc_p");
@@ -755,7 +765,7 @@ extern "C" int tl_ppcg_inner(tl_chunk *c, int inner_step, int bounds_extra, cons
     TL_CHECK_LAUNCH(c, "tea_leaf_ppcg_solve_update_r");
     prof_end(c, "tea_leaf_ppcg_solve_update_r");
     prof_begin(c);
-    launch_ppcg_calc_sd(c->s, c->g, e, c->cap, c->prec, c->f2[F_R], c->f2[F_SD], c->f2[F_CP], c->f2[F_BFP],
+    launch_ppcg_calc_sd(c->s, c->g, e, c->cap, c->prec, c->f2[F_R], c->f2[F_SD], stored_cp(c), stored_bfp(c),
                         c->f2[F_MI], c->f2[F_KX], c->f2[F_KY], c->ch_a, c->ch_b, step);
     TL_CHECK_LAUNCH(c, "tea_leaf_ppcg_solve_calc_sd");
     prof_end(c, "tea_leaf_ppcg_solve_calc_sd");

@@ -7,9 +7,12 @@
 //  * fused iteration kernels for the device-resident driver: the CG search
 //    direction update is folded into the next matvec (p is double-buffered so
 //    neighbours are read before they are overwritten), w = A p is recomputed
-//    instead of stored, and the diagonal / block-Jacobi factors are
-//    recomputed in registers from Kx, Ky with the same expressions as
-//    tea_leaf_init_jac_diag and tea_leaf_block_init.
+//    instead of stored, the face coefficients are recomputed from the
+//    conductivity (bitwise those tea_leaf_init_common stores), and the
+//    diagonal / block-Jacobi factors are recomputed in registers with the same
+//    expressions as tea_leaf_init_jac_diag and tea_leaf_block_init. The PPCG
+//    smoothing steps and the Chebyshev iterations run two per launch, with
+//    the tile's ring recomputed (k_fused_ppcg_inner2, k_fused_cheby2).
 //
 // A reduction kernel stores one partial per block; the one-block finisher
 // k_finish that follows it sums them and runs the solver's scalar algebra on
@@ -567,7 +570,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_w(Geom g, Ext e, cons
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
-        w[i0 + q * P] = wv[q];
+        store_nt(w + i0 + q * P, wv[q]);
         acc[0] += pc[q] * wv[q];
     }
     TL_TILE_LOOP_END
@@ -587,13 +590,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, dou
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double uv[STRIP], pv[STRIP], rv[STRIP], wv[STRIP], x[STRIP], zv[STRIP];
-    col4(u, i0, P, uv);
-    col4(p, i0, P, pv);
-    col4(r, i0, P, rv);
-    col4(w, i0, P, wv);
+    col4_nt(u, i0, P, uv);
+    col4_nt(p, i0, P, pv);
+    col4_nt(r, i0, P, rv);
+    col4_nt(w, i0, P, wv);
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     if (PREC == PREC_BLOCK) strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
-    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
+    if (PREC == PREC_DIAG) col4_nt(Mi, i0, P, x);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         uv[q] += alpha * pv[q];
@@ -605,14 +608,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, dou
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        u[i] = uv[q];
-        r[i] = rv[q];
+        store_nt(u + i, uv[q]);
+        store_nt(r + i, rv[q]);
         if (PREC == PREC_BLOCK) {
-            z[i] = zv[q];
+            store_nt(z + i, zv[q]);
             acc[0] += rv[q] * zv[q];
         } else if (PREC == PREC_DIAG) {
             const double zz = rv[q] * x[q];
-            z[i] = zz;
+            store_nt(z + i, zz);
             acc[0] += rv[q] * zz;
         } else {
             acc[0] += rv[q] * rv[q];
@@ -630,11 +633,11 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_p(Geom g, Ext e, doub
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double x[STRIP], pv[STRIP];
-    col4(PREC != PREC_NONE ? z : r, i0, P, x);
-    col4(p, i0, P, pv);
+    col4_nt(PREC != PREC_NONE ? z : r, i0, P, x);
+    col4_nt(p, i0, P, pv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q)
-        if (ks + q <= e.k1) p[i0 + q * P] = x[q] + beta * pv[q];
+        if (ks + q <= e.k1) store_nt(p + i0 + q * P, x[q] + beta * pv[q]);
     TL_TILE_LOOP_END
 }
 
