@@ -218,6 +218,38 @@ def test_block_preconditioner_solve(hiplib, oracle, x, y, hd):
         assert rel_err(g.get_field(f), o.get_field(f)) < TOL, f
 
 
+@pytest.mark.parametrize("x,y,hd", [(67, 13, 1), (200, 41, 1)])
+def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
+    """While cp, bfp hold the factors of the current Kx, Ky the drop-in CG,
+    Chebyshev and PPCG kernels recompute them in registers; re-uploading cp
+    (which makes the kernels read the stored arrays, as the reference does)
+    must give bitwise the same u, r, z, p and sums."""
+    out = []
+    for reread in (False, True):
+        g = hiplib.Chunk(x, y, hd, 3)
+        setup(g, seed=5)
+        g.calc_residual()
+        rro = g.cg_init()
+        if reread:
+            g.put_field("cp", g.get_field("cp"))
+        sums = []
+        for it in range(3):
+            pw = g.cg_calc_w()
+            rrn = g.cg_calc_ur(rro / pw)
+            g.cg_calc_p(rrn / rro)
+            sums += [pw, rrn]
+            rro = rrn
+        g.ppcg_init(np.linspace(0.2, 0.8, 3), np.linspace(0.05, 0.09, 3), 2.5, 3)
+        g.ppcg_init_sd()
+        for step in (1, 2, 3):
+            g.ppcg_inner(step, 0)
+        out.append((sums, [g.get_field(f) for f in ("u", "r", "z", "p", "sd")]))
+    (s1, f1), (s2, f2) = out
+    assert s1 == s2
+    for a, b in zip(f1, f2):
+        assert np.array_equal(a, b)
+
+
 def test_finalise_and_summary(hiplib, oracle):
     g, o = pair(hiplib, oracle, 96, 50, 1, 1)
     rng = np.random.default_rng(3)
