@@ -64,6 +64,10 @@ struct tl_chunk {
     // drop-in kernels then recompute the factors in registers (bitwise the
     // stored ones) instead of reading them
     bool factors_current = false;
+    // Kx, Ky are the faces of the last tl_common_init (cleared by an upload of
+    // Kx or Ky): the drop-in CG kernels then form them from the conductivity
+    // (bitwise the stored values) instead of reading both arrays
+    bool faces_current = false;
     // Chebyshev iterations per fused launch (1 or 2; TL_TUNE_CHEBY_STEPS_PER_LAUNCH)
     int cheby_steps_per_launch = 2;
     // events around the eagerly launched first CG iteration of each batch

@@ -334,6 +334,7 @@ extern "C" int tl_field_set(tl_chunk *c, int field, const double *host)
     }
     const int w = c->g.x_max + 2 * c->g.hd, h = c->g.y_max + 2 * c->g.hd;
     if (field == F_KX || field == F_KY || field == F_CP || field == F_BFP) c->factors_current = false;
+    if (field == F_KX || field == F_KY) c->faces_current = false;
     hipError_t e = hipMemcpy2DAsync(c->f2[field], sizeof(double) * c->g.pitch, host, sizeof(double) * w,
                                     sizeof(double) * w, h, hipMemcpyHostToDevice, c->s);
     if (e != hipSuccess) return hip_fail("hipMemcpy2D", e);
@@ -500,6 +501,7 @@ extern "C" int tl_common_init(tl_chunk *c, int coefficient, double dt, double *r
                   c->f2[F_U0]);
     TL_CHECK_LAUNCH(c, "generate_chunk_init_u");
     prof_end(c, "generate_chunk_init_u");
+    c->faces_current = true;
     return TL_OK;
 }
 
@@ -603,29 +605,32 @@ extern "C" int tl_cg_init(tl_chunk *c, double *rro)
     return read_result(c, rro, 1);
 }
 
-extern "C" int tl_cg_calc_w(tl_chunk *c, double *pw)
-{
-    prof_begin(c);
-    launch_cg_calc_w(c->s, c->g, ext_interior(c), c->cap, c->f2[F_P], c->f2[F_W], c->f2[F_KX], c->f2[F_KY],
-                     red_args(c, RED_STORE, false));
-    TL_CHECK_LAUNCH(c, "tea_leaf_cg_solve_calc_w");
-    prof_end(c, "tea_leaf_cg_solve_calc_w");
-    return read_result(c, pw, 1);
-}
-
 // block-Jacobi factors for the per-iteration drop-in kernels: null (recompute
 // from Kx, Ky in registers, bitwise the stored values) while the stored ones
 // belong to the current Kx, Ky; the stored arrays otherwise, as the
 // reference reads them (tea_block_jacobi.cl:44-76)
 static const double *stored_cp(tl_chunk *c) { return c->factors_current ? nullptr : c->f2[F_CP]; }
 static const double *stored_bfp(tl_chunk *c) { return c->factors_current ? nullptr : c->f2[F_BFP]; }
+// conductivity for the drop-in CG kernels while Kx, Ky are its faces, else null
+static const double *faces_cond(tl_chunk *c) { return c->faces_current ? c->f2[F_COND] : nullptr; }
+
+extern "C" int tl_cg_calc_w(tl_chunk *c, double *pw)
+{
+    prof_begin(c);
+    launch_cg_calc_w(c->s, c->g, ext_interior(c), c->cap, c->f2[F_P], c->f2[F_W], c->f2[F_KX], c->f2[F_KY],
+                     faces_cond(c), c->faces, red_args(c, RED_STORE, false));
+    TL_CHECK_LAUNCH(c, "tea_leaf_cg_solve_calc_w");
+    prof_end(c, "tea_leaf_cg_solve_calc_w");
+    return read_result(c, pw, 1);
+}
+
 
 extern "C" int tl_cg_calc_ur(tl_chunk *c, double alpha, double *rrn)
 {
     prof_begin(c);
     launch_cg_calc_ur(c->s, c->g, ext_interior(c), c->cap, c->prec, alpha, c->f2[F_U], c->f2[F_P], c->f2[F_R],
                       c->f2[F_W], c->f2[F_Z], stored_cp(c), stored_bfp(c), c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
-                      red_args(c, RED_STORE, false));
+                      faces_cond(c), c->faces, red_args(c, RED_STORE, false));
     TL_CHECK_LAUNCH(c, "tea_leaf_cg_solve_calc_ur");
     prof_end(c, "tea_leaf_cg_solve_calc_ur");
     return read_result(c, rrn, 1);

@@ -48,11 +48,11 @@ void launch_block_solve(hipStream_t s, const Geom &g, const Ext &e, int cap, con
 void launch_cg_init_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double *p,
                       const double *r, double *z, const double *Mi, const RedArgs &ra);
 void launch_cg_calc_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *p, double *w,
-                      const double *Kx, const double *Ky, const RedArgs &ra);
+                      const double *Kx, const double *Ky, const double *W, const Faces &fc, const RedArgs &ra);
 void launch_cg_calc_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double alpha,
                        double *u, const double *p, double *r, const double *w, double *z,
                        const double *cp, const double *bfp, const double *Mi, const double *Kx,
-                       const double *Ky, const RedArgs &ra);
+                       const double *Ky, const double *W, const Faces &fc, const RedArgs &ra);
 void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double beta,
                       double *p, const double *r, const double *z);
 void launch_cheby_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int init,

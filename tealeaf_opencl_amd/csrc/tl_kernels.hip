@@ -559,13 +559,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_init_p(Geom g, Ext e, doub
 // tea_leaf_cg_solve_calc_w (tea_leaf_cg_cl.cl:45-69)
 __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_w(Geom g, Ext e, const double *p, double *w,
                                                              const double *Kx, const double *Ky,
-                                                             RedArgs ra)
+                                                             const double *W, Faces fc, RedArgs ra)
 {
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    if (W) strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    else strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
@@ -584,7 +585,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, dou
                                                               const double *w, double *z,
                                                               const double *cp, const double *bfp,
                                                               const double *Mi, const double *Kx,
-                                                              const double *Ky, RedArgs ra)
+                                                              const double *Ky, const double *W, Faces fc,
+                                                              RedArgs ra)
 {
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
@@ -595,7 +597,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_calc_ur(Geom g, Ext e, dou
     col4_nt(r, i0, P, rv);
     col4_nt(w, i0, P, wv);
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
-    if (PREC == PREC_BLOCK) strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    if (PREC == PREC_BLOCK) {
+        if (W) strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+        else strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    }
     if (PREC == PREC_DIAG) col4_nt(Mi, i0, P, x);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
@@ -1465,22 +1470,22 @@ void launch_cg_init_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int p
 }
 
 void launch_cg_calc_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *p, double *w,
-                      const double *Kx, const double *Ky, const RedArgs &ra)
+                      const double *Kx, const double *Ky, const double *W, const Faces &fc, const RedArgs &ra)
 {
-    TL_LAUNCH_RED(k_cg_calc_w, e, cap, s, 1, ra, g, e, p, w, Kx, Ky, ra);
+    TL_LAUNCH_RED(k_cg_calc_w, e, cap, s, 1, ra, g, e, p, w, Kx, Ky, W, fc, ra);
 }
 
 void launch_cg_calc_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double alpha,
                        double *u, const double *p, double *r, const double *w, double *z,
                        const double *cp, const double *bfp, const double *Mi, const double *Kx,
-                       const double *Ky, const RedArgs &ra)
+                       const double *Ky, const double *W, const Faces &fc, const RedArgs &ra)
 {
-    if (prec == PREC_BLOCK)
-        TL_LAUNCH_RED(k_cg_calc_ur<PREC_BLOCK>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
-    else if (prec == PREC_DIAG)
-        TL_LAUNCH_RED(k_cg_calc_ur<PREC_DIAG>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
-    else
-        TL_LAUNCH_RED(k_cg_calc_ur<PREC_NONE>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, ra);
+#define TL_CUR(P) \
+    TL_LAUNCH_RED(k_cg_calc_ur<P>, e, cap, s, 1, ra, g, e, alpha, u, p, r, w, z, cp, bfp, Mi, Kx, Ky, W, fc, ra)
+    if (prec == PREC_BLOCK) TL_CUR(PREC_BLOCK);
+    else if (prec == PREC_DIAG) TL_CUR(PREC_DIAG);
+    else TL_CUR(PREC_NONE);
+#undef TL_CUR
 }
 
 void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, double beta,

@@ -221,9 +221,11 @@ def test_block_preconditioner_solve(hiplib, oracle, x, y, hd):
 @pytest.mark.parametrize("x,y,hd", [(67, 13, 1), (200, 41, 1)])
 def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
     """While cp, bfp hold the factors of the current Kx, Ky the drop-in CG,
-    Chebyshev and PPCG kernels recompute them in registers; re-uploading cp
-    (which makes the kernels read the stored arrays, as the reference does)
-    must give bitwise the same u, r, z, p and sums."""
+    Chebyshev and PPCG kernels recompute them in registers, and while Kx, Ky
+    are the faces of the last common_init the drop-in CG kernels form them
+    from the conductivity; re-uploading cp and Kx (which makes the kernels
+    read the stored arrays, as the reference does) must give bitwise the same
+    u, r, z, p and sums."""
     out = []
     for reread in (False, True):
         g = hiplib.Chunk(x, y, hd, 3)
@@ -232,6 +234,7 @@ def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
         rro = g.cg_init()
         if reread:
             g.put_field("cp", g.get_field("cp"))
+            g.put_field("Kx", g.get_field("Kx"))
         sums = []
         for it in range(3):
             pw = g.cg_calc_w()
