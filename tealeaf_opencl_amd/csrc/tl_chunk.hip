@@ -664,8 +664,8 @@ extern "C" int tl_cheby_init(tl_chunk *c, const double *ch_alphas, const double 
     c->theta = theta;
     prof_begin(c);
     launch_cheby_p(c->s, c->g, ext_interior(c), c->cap, c->prec, 1, c->f2[F_U], c->f2[F_U0], c->f2[F_P],
-                   c->f2[F_R], c->f2[F_W], c->f2[F_CP], c->f2[F_BFP], c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
-                   c->ch_a, c->ch_b, theta, 0, nullptr);
+                   c->f2[F_R], c->f2[F_W], stored_cp(c), stored_bfp(c), c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
+                   faces_cond(c), c->faces, c->ch_a, c->ch_b, theta, 0, nullptr);
     TL_CHECK_LAUNCH(c, "tea_leaf_cheby_solve_init_p");
     prof_end(c, "tea_leaf_cheby_solve_init_p");
     prof_begin(c);
@@ -686,7 +686,7 @@ extern "C" int tl_cheby_iterate(tl_chunk *c, int cheby_calc_step)
     prof_begin(c);
     launch_cheby_p(c->s, c->g, ext_interior(c), c->cap, c->prec, 0, c->f2[F_U], c->f2[F_U0], c->f2[F_P],
                    c->f2[F_R], c->f2[F_W], stored_cp(c), stored_bfp(c), c->f2[F_MI], c->f2[F_KX], c->f2[F_KY],
-                   c->ch_a, c->ch_b, c->theta, step, nullptr);
+                   faces_cond(c), c->faces, c->ch_a, c->ch_b, c->theta, step, nullptr);
     TL_CHECK_LAUNCH(c, "tea_leaf_cheby_solve_calc_p");
     prof_end(c, "tea_leaf_cheby_solve_calc_p");
     prof_begin(c);
@@ -766,12 +766,13 @@ extern "C" int tl_ppcg_inner(tl_chunk *c, int inner_step, int bounds_extra, cons
     }
     const Ext e = ppcg_extent(c, bounds_extra, chunk_neighbours);
     prof_begin(c);
-    launch_ppcg_update_r(c->s, c->g, e, c->cap, c->f2[F_U], c->f2[F_R], c->f2[F_KX], c->f2[F_KY], c->f2[F_SD]);
+    launch_ppcg_update_r(c->s, c->g, e, c->cap, c->f2[F_U], c->f2[F_R], c->f2[F_KX], c->f2[F_KY], faces_cond(c),
+                         c->faces, c->f2[F_SD]);
     TL_CHECK_LAUNCH(c, "tea_leaf_ppcg_solve_update_r");
     prof_end(c, "tea_leaf_ppcg_solve_update_r");
     prof_begin(c);
     launch_ppcg_calc_sd(c->s, c->g, e, c->cap, c->prec, c->f2[F_R], c->f2[F_SD], stored_cp(c), stored_bfp(c),
-                        c->f2[F_MI], c->f2[F_KX], c->f2[F_KY], c->ch_a, c->ch_b, step);
+                        c->f2[F_MI], c->f2[F_KX], c->f2[F_KY], faces_cond(c), c->faces, c->ch_a, c->ch_b, step);
     TL_CHECK_LAUNCH(c, "tea_leaf_ppcg_solve_calc_sd");
     prof_end(c, "tea_leaf_ppcg_solve_calc_sd");
     return TL_OK;

@@ -58,19 +58,19 @@ void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int p
 void launch_cheby_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int init,
                     const double *u, const double *u0, double *p, double *r, double *w,
                     const double *cp, const double *bfp, const double *Mi, const double *Kx,
-                    const double *Ky, const double *ch_a, const double *ch_b, double theta, int step,
-                    const SolveState *st);
+                    const double *Ky, const double *W, const Faces &fc, const double *ch_a,
+                    const double *ch_b, double theta, int step, const SolveState *st);
 void launch_cheby_u(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
                     const double *r, const RedArgs &ra);
 void launch_ppcg_init_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
                          double *sd, const double *cp, const double *bfp, const double *Mi,
                          const double *Kx, const double *Ky, double theta);
 void launch_ppcg_update_r(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, double *r,
-                          const double *Kx, const double *Ky, const double *sd);
+                          const double *Kx, const double *Ky, const double *W, const Faces &fc, const double *sd);
 void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
                          double *sd, const double *cp, const double *bfp, const double *Mi,
-                         const double *Kx, const double *Ky, const double *ch_a, const double *ch_b,
-                         int step);
+                         const double *Kx, const double *Ky, const double *W, const Faces &fc,
+                         const double *ch_a, const double *ch_b, int step);
 // without a solver state: u1 <- sweep(un); with one: the pair (u1, un) is
 // double-buffered by the iteration counter
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,

@@ -656,6 +656,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_p(Geom g, Ext e, const 
                                                            double *w, const double *cp,
                                                            const double *bfp, const double *Mi,
                                                            const double *Kx, const double *Ky,
+                                                           const double *W, Faces fc,
                                                            const double *ch_a, const double *ch_b,
                                                            double theta, int step,
                                                            const SolveState *st)
@@ -666,10 +667,11 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_p(Geom g, Ext e, const 
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], b[STRIP], pv[STRIP], x[STRIP], rv[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
-    col4(u0, i0, P, b);
-    if (!INIT) col4(p, i0, P, pv);
-    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
+    if (W) strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    else strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4_nt(u0, i0, P, b);
+    if (!INIT) col4_nt(p, i0, P, pv);
+    if (PREC == PREC_DIAG) col4_nt(Mi, i0, P, x);
     strip_matvec(u, kx0, kx1, ky, i0, P, wv, nullptr);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) rv[q] = b[q] - wv[q];
@@ -680,13 +682,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_p(Geom g, Ext e, const 
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        if (w) w[i] = wv[q];
-        r[i] = rv[q];
+        if (w) store_nt(w + i, wv[q]);
+        store_nt(r + i, rv[q]);
         double np;
         if (PREC == PREC_BLOCK) np = INIT ? zv[q] / theta : ca * pv[q] + cb * zv[q];
         else if (PREC == PREC_DIAG) np = INIT ? (x[q] * rv[q]) / theta : ca * pv[q] + cb * x[q] * rv[q];
         else np = INIT ? rv[q] / theta : ca * pv[q] + cb * rv[q];
-        p[i] = np;
+        store_nt(p + i, np);
     }
     TL_TILE_LOOP_END
 }
@@ -703,13 +705,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cheby_u(Geom g, Ext e, double
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double uv[STRIP], pv[STRIP], rv[STRIP];
-    col4(u, i0, P, uv);
-    col4(p, i0, P, pv);
-    if (want) col4(r, i0, P, rv);
+    col4_nt(u, i0, P, uv);
+    col4_nt(p, i0, P, pv);
+    if (want) col4_nt(r, i0, P, rv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
-        u[i0 + q * P] = uv[q] + pv[q];
+        store_nt(u + i0 + q * P, uv[q] + pv[q]);
         if (want) acc[0] += rv[q] * rv[q];
     }
     TL_TILE_LOOP_END
@@ -748,20 +750,21 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_init_sd(Geom g, Ext e, c
 // tea_leaf_ppcg_solve_update_r (tea_leaf_ppcg_cl.cl:62-88) on the extent e
 __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_update_r(Geom g, Ext e, double *u, double *r,
                                                                  const double *Kx, const double *Ky,
-                                                                 const double *sd)
+                                                                 const double *W, Faces fc, const double *sd)
 {
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP];
-    strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
-    col4(u, i0, P, uv);
-    col4(r, i0, P, rv);
+    if (W) strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    else strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+    col4_nt(u, i0, P, uv);
+    col4_nt(r, i0, P, rv);
     strip_matvec(sd, kx0, kx1, ky, i0, P, res, sc);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
-        u[i0 + q * P] = uv[q] + sc[q];
-        r[i0 + q * P] = rv[q] - res[q];
+        store_nt(u + i0 + q * P, uv[q] + sc[q]);
+        store_nt(r + i0 + q * P, rv[q] - res[q]);
     }
     TL_TILE_LOOP_END
 }
@@ -772,6 +775,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_calc_sd(Geom g, Ext e, c
                                                                 double *sd, const double *cp,
                                                                 const double *bfp, const double *Mi,
                                                                 const double *Kx, const double *Ky,
+                                                                const double *W, Faces fc,
                                                                 const double *ch_a,
                                                                 const double *ch_b, int step)
 {
@@ -779,11 +783,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_ppcg_calc_sd(Geom g, Ext e, c
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double rv[STRIP], sv[STRIP], x[STRIP], zv[STRIP], kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
-    col4(r, i0, P, rv);
+    col4_nt(r, i0, P, rv);
     col4(sd, i0, P, sv);
-    if (PREC == PREC_DIAG) col4(Mi, i0, P, x);
+    if (PREC == PREC_DIAG) col4_nt(Mi, i0, P, x);
     if (PREC == PREC_BLOCK) {
-        strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+        if (W) strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+        else strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
         mask_rows(ks, e.k1, rv);
         strip_block(g, ks, i0, kx0, kx1, ky, cp, bfp, rv, zv);
     }
@@ -1498,11 +1503,11 @@ void launch_cg_calc_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int p
 void launch_cheby_p(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int init,
                     const double *u, const double *u0, double *p, double *r, double *w,
                     const double *cp, const double *bfp, const double *Mi, const double *Kx,
-                    const double *Ky, const double *ch_a, const double *ch_b, double theta, int step,
-                    const SolveState *st)
+                    const double *Ky, const double *W, const Faces &fc, const double *ch_a,
+                    const double *ch_b, double theta, int step, const SolveState *st)
 {
-#define TL_CHP(P, I)                                                                               \
-    TL_LAUNCH((k_cheby_p<P, I>), e, cap, s, g, e, u, u0, p, r, w, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, \
+#define TL_CHP(P, I)                                                                                      \
+    TL_LAUNCH((k_cheby_p<P, I>), e, cap, s, g, e, u, u0, p, r, w, cp, bfp, Mi, Kx, Ky, W, fc, ch_a, ch_b, \
               theta, step, st)
     if (init) {
         if (prec == PREC_BLOCK) TL_CHP(PREC_BLOCK, 1);
@@ -1536,22 +1541,21 @@ void launch_ppcg_init_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
 }
 
 void launch_ppcg_update_r(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, double *r,
-                          const double *Kx, const double *Ky, const double *sd)
+                          const double *Kx, const double *Ky, const double *W, const Faces &fc, const double *sd)
 {
-    TL_LAUNCH(k_ppcg_update_r, e, cap, s, g, e, u, r, Kx, Ky, sd);
+    TL_LAUNCH(k_ppcg_update_r, e, cap, s, g, e, u, r, Kx, Ky, W, fc, sd);
 }
 
 void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *r,
                          double *sd, const double *cp, const double *bfp, const double *Mi,
-                         const double *Kx, const double *Ky, const double *ch_a, const double *ch_b,
-                         int step)
+                         const double *Kx, const double *Ky, const double *W, const Faces &fc,
+                         const double *ch_a, const double *ch_b, int step)
 {
-    if (prec == PREC_BLOCK)
-        TL_LAUNCH(k_ppcg_calc_sd<PREC_BLOCK>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, step);
-    else if (prec == PREC_DIAG)
-        TL_LAUNCH(k_ppcg_calc_sd<PREC_DIAG>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, step);
-    else
-        TL_LAUNCH(k_ppcg_calc_sd<PREC_NONE>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, ch_a, ch_b, step);
+#define TL_CSD(P) TL_LAUNCH(k_ppcg_calc_sd<P>, e, cap, s, g, e, r, sd, cp, bfp, Mi, Kx, Ky, W, fc, ch_a, ch_b, step)
+    if (prec == PREC_BLOCK) TL_CSD(PREC_BLOCK);
+    else if (prec == PREC_DIAG) TL_CSD(PREC_DIAG);
+    else TL_CSD(PREC_NONE);
+#undef TL_CSD
 }
 
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,

@@ -222,14 +222,14 @@ def test_block_preconditioner_solve(hiplib, oracle, x, y, hd):
 def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
     """While cp, bfp hold the factors of the current Kx, Ky the drop-in CG,
     Chebyshev and PPCG kernels recompute them in registers, and while Kx, Ky
-    are the faces of the last common_init the drop-in CG kernels form them
-    from the conductivity; re-uploading cp and Kx (which makes the kernels
+    are the faces of the last common_init those kernels form them from the
+    conductivity; re-uploading cp and Kx (which makes the kernels
     read the stored arrays, as the reference does) must give bitwise the same
     u, r, z, p and sums."""
     out = []
     for reread in (False, True):
         g = hiplib.Chunk(x, y, hd, 3)
-        setup(g, seed=5)
+        rx, ry = setup(g, seed=5)
         g.calc_residual()
         rro = g.cg_init()
         if reread:
@@ -246,7 +246,12 @@ def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
         g.ppcg_init_sd()
         for step in (1, 2, 3):
             g.ppcg_inner(step, 0)
-        out.append((sums, [g.get_field(f) for f in ("u", "r", "z", "p", "sd")]))
+        fields = [g.get_field(f) for f in ("u", "r", "z", "p", "sd")]
+        g.cheby_init(np.linspace(0.3, 0.9, 4), np.linspace(0.01, 0.02, 4), 4, rx, ry, 1.7)
+        for step in (1, 2, 3):
+            g.cheby_iterate(step)
+        fields += [g.get_field(f) for f in ("u", "r", "p", "w")]
+        out.append((sums, fields))
     (s1, f1), (s2, f2) = out
     assert s1 == s2
     for a, b in zip(f1, f2):
