@@ -59,7 +59,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--cpu-iters", type=int, default=200)
+    ap.add_argument("--cpu-iters", type=int, default=1000,
+                    help="CG iterations of the CPU baseline sample (about 12 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=1, help="1 native (default), 0 drop-in")
     ap.add_argument("--max-iters", type=int, default=5000, help="tl_max_iters of the deck (short profiles)")
