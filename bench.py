@@ -30,8 +30,10 @@ METRIC = "solve s to 1e-10 rel resid @4096² (FP64 vs mixed); HBM GB/s % peak; i
 ALG_BYTES = {
     # p_new = z + beta p_old; w = A p_new; p.w : R z, p_old, w  W p_new
     "k_fused_cg_w": 4 * 8,
-    # u += a p; r -= a A p; z = M_blk^-1 r; r.z : R u, p, r, w  W u, r, z
-    "k_fused_cg_ur": 7 * 8,
+    # r -= a A p; z = M_blk^-1 r; r.z : R p, r, w  W r, z on odd iterations;
+    # even ones also add a_{n-1} p_{n-1} + a_n p_n into u : + R u, p_{n-1}  W u.
+    # The timed launches alternate between the two: (40 + 64) / 2 per launch.
+    "k_fused_cg_ur": (5 * 8 + 8 * 8) // 2,
     # r = u0 - A u; p = ca p + cb M_blk^-1 r; u' = u + p : R u, u0, w, p  W p, u'
     "k_fused_cheby": 6 * 8,
     # two such iterations per launch : R u, u0, p, Kx, Ky  W u, p
@@ -40,6 +42,8 @@ ALG_BYTES = {
     "k_fused_ppcg_inner": 7 * 8,
     # two such steps per launch : R u, r, sd, Kx, Ky  W u, r, sd
     "k_fused_ppcg_inner2": 8 * 8,
+    # PPCG outer update (u every step, sd = z / theta) : R u, p, r, w  W u, r, z, sd
+    "k_fused_cg_ur_ppcg": 8 * 8,
 }
 # the reference's bytes per cell for the same work with block Jacobi
 # (kernel_files/tea_leaf_{cg,cheby,ppcg}_cl.cl, each array once per kernel)
@@ -175,7 +179,8 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
             alg = (cg * cg_b + (its - cg) / 2 * ALG_BYTES["k_fused_cheby2"]) * n * n
             ref = (cg * REF_BYTES["cg"] + (its - cg) * REF_BYTES["cheby"]) * n * n
         else:
-            alg = (its * cg_b + inner // 2 * ALG_BYTES["k_fused_ppcg_inner2"]) * n * n
+            alg = (its * (ALG_BYTES["k_fused_cg_w"] + ALG_BYTES["k_fused_cg_ur_ppcg"]) +
+                   inner // 2 * ALG_BYTES["k_fused_ppcg_inner2"]) * n * n
             ref = (its * REF_BYTES["ppcg_outer"] + inner * REF_BYTES["ppcg_inner"]) * n * n
         out[name] = {"s_per_step": secs / steps, "iterations_per_step": its / steps,
                      "inner_iterations_per_step": inner / steps, "iterations_per_s": its / secs,

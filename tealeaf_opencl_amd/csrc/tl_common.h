@@ -79,6 +79,7 @@ struct SolveState {
     int n_fused0;         // first iteration run by a double-buffered kernel
     // running scalars
     double rro, pw, alpha, rrn, beta;
+    double alpha_prev;    // alpha of the previous CG iteration (the fused CG update of u lags one iteration)
     double error, old_error, initial_residual, bb, exact_error;
     double theta;         // Chebyshev / least-squares centre, PPCG init_sd scale
     int n;                // iteration being executed (Fortran loop variable)

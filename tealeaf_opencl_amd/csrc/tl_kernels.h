@@ -86,6 +86,9 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
                         const double *pA, const double *pB, const double *r, double *r_out, double *z,
                         double *sd, const double *W, const Faces &fc, const RedArgs &ra,
                         hipEvent_t ev_main = nullptr);
+// u += alpha p_L after a fused CG phase that ended on an odd iteration
+void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
+                     const SolveState *st);
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
                          const double *u0, double *uA, double *uB, const RedArgs &ra);
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,

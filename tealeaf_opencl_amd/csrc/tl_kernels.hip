@@ -87,6 +87,7 @@ __device__ void finish_scalars(const RedArgs &ra, const double *tot)
     }
     case RED_CG_PW:
         s->pw = tot[0];
+        s->alpha_prev = s->alpha;
         s->alpha = s->rro / s->pw;
         if (ra.hist_a) ra.hist_a[s->n] = s->alpha;
         break;
@@ -901,6 +902,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
 // sd = z/theta (tea_leaf_ppcg_cl.cl:4-60 on the same r) and skips the unused
 // r.z sum. r_out may be r (in place) or a second buffer (the PPCG outer step
 // when the inner steps that follow flip r between two buffers).
+// For CG (PPCG=0) the update of u, which nothing in the iteration reads, is
+// made every second iteration: iteration n odd leaves u alone, iteration n
+// even adds alpha_{n-1} p_{n-1} (still in the other p buffer) and then
+// alpha_n p_n, in that order and with the same FMAs, so u is bitwise what
+// one update per iteration gives; a CG phase that ends on an odd iteration
+// is completed by k_cg_u_fix. This saves u's read and write on every other
+// iteration (8 of 88 B/cell per iteration on average).
 template <int PREC, int PPCG>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, double *u,
                                                                const double *pA, const double *pB,
@@ -910,20 +918,29 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
-    const double *p = (st->n & 1) ? pB : pA;
-    const double alpha = st->alpha;
+    const bool odd = (st->n & 1) != 0;
+    const double *p = odd ? pB : pA;
+    const double *p_prev = odd ? pA : pB;
+    const double alpha = st->alpha, alpha_prev = st->alpha_prev;
     const double theta = PPCG ? st->theta : 1.0;
+    const bool upd = PPCG || !odd;
     double acc[1] = {0.0};
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], uv[STRIP], rv[STRIP], zv[STRIP];
     strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col4_nt(u, i0, P, uv);
+    if (upd) col4_nt(u, i0, P, uv);
     col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
+    if (!PPCG && upd) {
+        double pp[STRIP];
+        col4_nt(p_prev, i0, P, pp);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) uv[q] = fma(alpha_prev, pp[q], uv[q]);
+    }
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        uv[q] += alpha * pc[q];
+        if (upd) uv[q] = fma(alpha, pc[q], uv[q]);
         rv[q] -= alpha * wv[q];
     }
     mask_rows(ks, e.k1, rv);
@@ -932,7 +949,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
-        store_nt(u + i, uv[q]);
+        if (upd) store_nt(u + i, uv[q]);
         store_nt(r_out + i, rv[q]);
         double zz = rv[q];
         if (PREC == PREC_BLOCK) zz = zv[q];
@@ -943,6 +960,20 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     }
     TL_TILE_LOOP_END
     if (!PPCG) block_partial<1>(acc, ra.partials);
+}
+
+// Completes u after a CG phase of the fused kernels that ended on an odd
+// iteration L: u += alpha_L p_L (p_L is in the second p buffer).
+__global__ void __launch_bounds__(BLOCK_THREADS) k_cg_u_fix(Geom g, Ext e, double *u, const double *p,
+                                                            const SolveState *st)
+{
+    const double alpha = st->alpha;
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) u[i0 + q * P] = fma(alpha, p[i0 + q * P], u[i0 + q * P]);
+    TL_TILE_LOOP_END
 }
 
 // One PPCG smoothing step on a strip once A sd is known (res = A sd, sc = sd):
@@ -1569,6 +1600,12 @@ void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, cons
                        hipEvent_t ev_main)
 {
     TL_LAUNCH_RED_EV(k_fused_cg_w, e, cap, s, 1, ra, ev_main, g, e, src, pA, pB, W, fc, ra);
+}
+
+void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
+                     const SolveState *st)
+{
+    TL_LAUNCH(k_cg_u_fix, e, cap, s, g, e, u, p, st);
 }
 
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
