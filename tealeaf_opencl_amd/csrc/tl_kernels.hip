@@ -905,8 +905,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, con
 // For CG (PPCG=0) the update of u, which nothing in the iteration reads, is
 // made every second iteration: iteration n odd leaves u alone, iteration n
 // even adds alpha_{n-1} p_{n-1} (still in the other p buffer) and then
-// alpha_n p_n, in that order and with the same FMAs, so u is bitwise what
-// one update per iteration gives; a CG phase that ends on an odd iteration
+// alpha_n p_n, each as one FMA and in that order, so u is bitwise what one
+// fused update per iteration gives; a CG phase that ends on an odd iteration
 // is completed by k_cg_u_fix. This saves u's read and write on every other
 // iteration (8 of 88 B/cell per iteration on average).
 template <int PREC, int PPCG>
