@@ -786,8 +786,8 @@ extern "C" int tl_jacobi_solve(tl_chunk *c, double *error)
     TL_CHECK_LAUNCH(c, "tea_leaf_jacobi_copy_u");
     prof_end(c, "tea_leaf_jacobi_copy_u");
     prof_begin(c);
-    launch_jacobi(c->s, c->g, ext_interior(c), c->cap, c->f2[F_KX], c->f2[F_KY], c->f2[F_U0], c->f2[F_U],
-                  c->f2[F_MI], red_args(c, RED_STORE, false));
+    launch_jacobi(c->s, c->g, ext_interior(c), c->cap, c->f2[F_KX], c->f2[F_KY], faces_cond(c), c->faces,
+                  c->f2[F_U0], c->f2[F_U], c->f2[F_MI], red_args(c, RED_STORE, false));
     TL_CHECK_LAUNCH(c, "tea_leaf_jacobi_solve");
     prof_end(c, "tea_leaf_jacobi_solve");
     return read_result(c, error, 1);

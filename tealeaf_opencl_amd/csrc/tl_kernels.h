@@ -73,8 +73,9 @@ void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
                          const double *ch_a, const double *ch_b, int step);
 // without a solver state: u1 <- sweep(un); with one: the pair (u1, un) is
 // double-buffered by the iteration counter
+// W non-null: faces from the conductivity (bitwise the stored Kx, Ky)
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
-                   const double *u0, double *u1, double *un, const RedArgs &ra);
+                   const double *W, const Faces &fc, const double *u0, double *u1, double *un, const RedArgs &ra);
 // p is double-buffered (pA, pB); the iteration counter in the state picks
 // which buffer is read and which is written. ev_main, when given, is recorded
 // after the iteration kernel and before its finisher. The fused kernels

@@ -1590,9 +1590,10 @@ void launch_ppcg_calc_sd(hipStream_t s, const Geom &g, const Ext &e, int cap, in
 }
 
 void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *Kx, const double *Ky,
-                   const double *u0, double *u1, double *un, const RedArgs &ra)
+                   const double *W, const Faces &fc, const double *u0, double *u1, double *un, const RedArgs &ra)
 {
-    TL_LAUNCH_RED(k_jacobi<0>, e, cap, s, 1, ra, g, e, Kx, Ky, nullptr, Faces{}, u0, u1, un, ra);
+    if (W) TL_LAUNCH_RED(k_jacobi<1>, e, cap, s, 1, ra, g, e, nullptr, nullptr, W, fc, u0, u1, un, ra);
+    else TL_LAUNCH_RED(k_jacobi<0>, e, cap, s, 1, ra, g, e, Kx, Ky, nullptr, Faces{}, u0, u1, un, ra);
 }
 
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,

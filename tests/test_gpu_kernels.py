@@ -222,8 +222,8 @@ def test_block_preconditioner_solve(hiplib, oracle, x, y, hd):
 def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
     """While cp, bfp hold the factors of the current Kx, Ky the drop-in CG,
     Chebyshev and PPCG kernels recompute them in registers, and while Kx, Ky
-    are the faces of the last common_init those kernels form them from the
-    conductivity; re-uploading cp and Kx (which makes the kernels
+    are the faces of the last common_init those kernels (and Jacobi) form them
+    from the conductivity; re-uploading cp and Kx (which makes the kernels
     read the stored arrays, as the reference does) must give bitwise the same
     u, r, z, p and sums."""
     out = []
@@ -251,6 +251,8 @@ def test_dropin_factors_recomputed_bitwise(hiplib, oracle, x, y, hd):
         for step in (1, 2, 3):
             g.cheby_iterate(step)
         fields += [g.get_field(f) for f in ("u", "r", "p", "w")]
+        sums += [g.jacobi_solve() for _ in range(2)]
+        fields.append(g.get_field("u"))
         out.append((sums, fields))
     (s1, f1), (s2, f2) = out
     assert s1 == s2
