@@ -84,8 +84,9 @@ struct tl_chunk {
         int bytes = 0;                    // 8 or 4: storage of the inner fields
         tl::Geom g{};                     // layout of the inner fields
         long off = 0, elems = 0;
-        void *base[9] = {};               // owned allocations
-        void *f[9] = {};                  // conductivity w, r, z, p, p2, d, sd, sd2, r2
+        void *base[10] = {};              // owned allocations
+        void *f[10] = {};                 // faces (FP64: conductivity w; FP32: Kx), r, z, p, p2, d, sd,
+                                          // sd2, r2, FP32 Ky
         tl::RefineState *st = nullptr;
         tl::RefineState *h_st = nullptr;  // pinned, 2 slots
         GraphSlot graph;                  // CG iterations

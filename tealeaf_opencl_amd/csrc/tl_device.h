@@ -64,9 +64,20 @@ __device__ __forceinline__ int block_top(const Geom &g, int row)
     return STRIP;
 }
 
+// 1/x from the FP32 hardware reciprocal (about 1 ulp of FP32), for the
+// preconditioner of the FP32-inner solve, whose vectors are rounded to FP32
+// anyway. Used for pivots x >= 1 (diagonally dominant blocks), well inside
+// the FP32 range.
+__device__ __forceinline__ double rcp_fp32(double x)
+{
+    return (double)__builtin_amdgcn_rcpf((float)x);
+}
+
 // block_solve_func (tea_block_jacobi.cl:6-42) on one strip held in registers.
 // b[k] = COEF_B, a[k] = COEF_A of strip row k; cp/bfp as produced by
-// tea_leaf_block_init; rows >= upper are left at zero.
+// tea_leaf_block_init; rows >= upper are left at zero. FAST: bfp[0] holds
+// 1/b[0] (strip_factors<true>), used in place of the division.
+template <bool FAST = false>
 __device__ __forceinline__ void strip_solve(int upper, const double *r, const double *b,
                                             const double *a, const double *cp,
                                             const double *bfp, double *z)
@@ -75,7 +86,7 @@ __device__ __forceinline__ void strip_solve(int upper, const double *r, const do
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) z[k] = 0.0;
     if (upper <= 0) return;
-    dp[0] = r[0] / b[0];
+    dp[0] = FAST ? r[0] * bfp[0] : r[0] / b[0];
 #pragma unroll
     for (int k = 1; k < STRIP; ++k)
         if (k < upper) dp[k] = (r[k] - a[k] * dp[k - 1]) * bfp[k];
@@ -88,16 +99,27 @@ __device__ __forceinline__ void strip_solve(int upper, const double *r, const do
     }
 }
 
-// Thomas factors of tea_leaf_block_init (tea_block_jacobi.cl:78-105)
+// Thomas factors of tea_leaf_block_init (tea_block_jacobi.cl:78-105).
+// FAST: the pivots' reciprocals from rcp_fp32 (no FP64 division) and
+// bfp[0] = 1/b[0]. The factorisation stays exact for the matrix it defines:
+// its off-diagonals are (1/bfp[k]) * cp[k] = c[k] and a[k] as before (still
+// symmetric), its diagonal differs from b[k] by the reciprocal's error times
+// the pivot, about 1e-7 relative.
+template <bool FAST = false>
 __device__ __forceinline__ void strip_factors(int upper, const double *b, const double *a,
                                               const double *c, double *cp, double *bfp)
 {
-    cp[0] = c[0] / b[0];
-    bfp[0] = 0.0;
+    if (FAST) {
+        bfp[0] = rcp_fp32(b[0]);
+        cp[0] = c[0] * bfp[0];
+    } else {
+        cp[0] = c[0] / b[0];
+        bfp[0] = 0.0;
+    }
 #pragma unroll
     for (int k = 1; k < STRIP; ++k) {
         if (k < upper) {
-            bfp[k] = 1.0 / (b[k] - a[k] * cp[k - 1]);
+            bfp[k] = FAST ? rcp_fp32(b[k] - a[k] * cp[k - 1]) : 1.0 / (b[k] - a[k] * cp[k - 1]);
             cp[k] = c[k] * bfp[k];
         } else {
             bfp[k] = 0.0;
@@ -243,44 +265,33 @@ __device__ __forceinline__ double face_coef(double wa, double wb, double scale)
     return v;
 }
 
-// The same face from h = 1/(2w) stored per cell: scale * (h_a + h_b), equal
-// to face_coef in exact arithmetic, with no division (used by the inner
-// iterations of the mixed-precision solve, whose operator is the FP32-rounded
-// one anyway).
-__device__ __forceinline__ double face_coef_h(double ha, double hb, double scale)
-{
-    return scale * (ha + hb);
-}
-
 __device__ __forceinline__ bool outside_init(const Geom &g, int j, int k)
 {
     return j < 1 || j > g.x_max + 2 * g.hd - 1 || k < 1 || k > g.y_max + 2 * g.hd - 1;
 }
 
-template <bool HALF_INV = false>
 __device__ __forceinline__ double kx_at(const Geom &g, const Faces &f, int j, int k, double wl, double wc)
 {
     const int hd = g.hd;
     const bool zero = outside_init(g, j, k) || (f.zl && j <= hd) || (f.zr && j > g.x_max - 1 + hd) ||
                       (f.zb && k < hd) || (f.zt && k > g.y_max - 1 + hd);
-    const double v = HALF_INV ? face_coef_h(wl, wc, f.rx) : face_coef(wl, wc, f.rx);
+    const double v = face_coef(wl, wc, f.rx);
     return zero ? 0.0 : v;
 }
 
-template <bool HALF_INV = false>
 __device__ __forceinline__ double ky_at(const Geom &g, const Faces &f, int j, int k, double wd, double wc)
 {
     const int hd = g.hd;
     const bool zero = outside_init(g, j, k) || (f.zl && j < hd) || (f.zr && j > g.x_max - 1 + hd) ||
                       (f.zb && k <= hd) || (f.zt && k > g.y_max - 1 + hd);
-    const double v = HALF_INV ? face_coef_h(wd, wc, f.ry) : face_coef(wd, wc, f.ry);
+    const double v = face_coef(wd, wc, f.ry);
     return zero ? 0.0 : v;
 }
 
 // face coefficients of the strip (same outputs as strip_k) recomputed from a
 // per-cell field stored as T: the conductivity w (bitwise the init_common
-// faces), or with HALF_INV its 1/(2w)
-template <bool HALF_INV = false, typename T>
+// faces)
+template <typename T>
 __device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, const T *w, int i0, int P, int j, int ks,
                                              double *kx0, double *kx1, double *ky)
 {
@@ -296,21 +307,21 @@ __device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, cons
     if (inner) {
 #pragma unroll
         for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = HALF_INV ? face_coef_h(ww[q], wc[q + 1], f.rx) : face_coef(ww[q], wc[q + 1], f.rx);
-            kx1[q] = HALF_INV ? face_coef_h(wc[q + 1], we[q], f.rx) : face_coef(wc[q + 1], we[q], f.rx);
+            kx0[q] = face_coef(ww[q], wc[q + 1], f.rx);
+            kx1[q] = face_coef(wc[q + 1], we[q], f.rx);
         }
 #pragma unroll
         for (int q = 0; q < STRIP + 1; ++q)
-            ky[q] = HALF_INV ? face_coef_h(wc[q], wc[q + 1], f.ry) : face_coef(wc[q], wc[q + 1], f.ry);
+            ky[q] = face_coef(wc[q], wc[q + 1], f.ry);
         return;
     }
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        kx0[q] = kx_at<HALF_INV>(g, f, j, ks + q, ww[q], wc[q + 1]);
-        kx1[q] = kx_at<HALF_INV>(g, f, j + 1, ks + q, wc[q + 1], we[q]);
+        kx0[q] = kx_at(g, f, j, ks + q, ww[q], wc[q + 1]);
+        kx1[q] = kx_at(g, f, j + 1, ks + q, wc[q + 1], we[q]);
     }
 #pragma unroll
-    for (int q = 0; q < STRIP + 1; ++q) ky[q] = ky_at<HALF_INV>(g, f, j, ks + q, wc[q], wc[q + 1]);
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = ky_at(g, f, j, ks + q, wc[q], wc[q + 1]);
 }
 
 // w = A v on the strip: c6 = centre column rows ks-1..ks+4, w4/e4 = west/east
@@ -376,17 +387,29 @@ __device__ __forceinline__ void mask_rows(int ks, int k1, double *v)
 }
 
 // z = M^-1 r for one strip with the preconditioner recomputed from Kx, Ky
-// (rv already masked past the extent)
-template <int PREC>
+// (rv already masked past the extent). FAST: reciprocals from rcp_fp32 (the
+// FP32-inner solve's preconditioner).
+template <int PREC, bool FAST = false>
 __device__ __forceinline__ void strip_precondition(const Geom &g, int ks, int i0, const double *kx0,
                                                    const double *kx1, const double *ky, const double *rv,
                                                    double *zv)
 {
     if (PREC == PREC_BLOCK) {
-        strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+        if (FAST) {
+            double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
+            strip_abc(kx0, kx1, ky, a, b, c);
+            const int upper = block_top(g, ks);
+            strip_factors<true>(upper, b, a, c, cpv, bfv);
+            strip_solve<true>(upper, rv, b, a, cpv, bfv, zv);
+        } else {
+            strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+        }
     } else if (PREC == PREC_DIAG) {
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q) zv[q] = rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+        for (int q = 0; q < STRIP; ++q) {
+            const double dg = diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
+            zv[q] = rv[q] * (FAST ? rcp_fp32(dg) : 1.0 / dg);
+        }
     } else {
 #pragma unroll
         for (int q = 0; q < STRIP; ++q) zv[q] = rv[q];

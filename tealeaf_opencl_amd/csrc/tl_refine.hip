@@ -1,9 +1,13 @@
 // Refined solve of A u = u0 for one implicit timestep (after common_init has
 // formed Kx, Ky, the conductivity and u = u0): a preconditioned CG whose
-// vectors (r, z, p, the solution increment d) and the per-cell field the
-// kernels form face coefficients from (the conductivity, or 1/(2w) in FP32)
-// are stored in FP32 or FP64, with every arithmetic operation, every dot
-// product and every scalar in FP64,
+// vectors (r, z, p, the solution increment d) are stored in FP32 or FP64.
+// FP64 inner fields: the operator's faces are recomputed from the
+// conductivity (bitwise those of tea_leaf_init_common) and the
+// preconditioner's pivots divided exactly. FP32 inner fields: the operator's
+// faces are Kx, Ky rounded to FP32, read from two FP32 fields, and the
+// preconditioner's pivot reciprocals come from the FP32 hardware reciprocal.
+// Every other arithmetic operation, every dot product and every scalar is
+// FP64. The inner solve is
 // wrapped in FP64 reliable updates: whenever the recursive residual has
 // dropped by `delta` since the last update, or has reached the tolerance, the
 // increment is added into the FP64 solution (u += d, d = 0) and the residual
@@ -47,7 +51,7 @@ namespace tl {
 // internal linkage are not visible to the profiling tools)
 namespace rf {
 
-enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_SD, RW_SD2, RW_R2, RW_N };
+enum { RW_W = 0, RW_R, RW_Z, RW_P, RW_P2, RW_D, RW_SD, RW_SD2, RW_R2, RW_KY, RW_N };
 
 struct RfArgs {
     double *partials;
@@ -137,25 +141,40 @@ __device__ __forceinline__ double rnd(double v)
     return (double)(T)v;
 }
 
-// Field from which the inner kernels form their face coefficients: the FP64
-// conductivity itself for FP64 inner fields (faces bitwise those of
-// tea_leaf_init_common), 1/(2w) rounded to FP32 for FP32 inner fields (faces
-// without a division).
+// Face coefficients of a strip for the inner operator. FP64 inner fields:
+// recomputed from the conductivity W (bitwise the faces tea_leaf_init_common
+// stores). FP32 inner fields: read from W, Wy, the chunk's Kx, Ky rounded to
+// FP32 (k_rf_narrow), so that the inner operator is the FP64 one with its
+// faces rounded to FP32 and its kernels carry no face formulas or wall tests.
 template <typename T>
-constexpr bool inner_half_inv()
+__device__ __forceinline__ void rf_strip_faces(const Geom &g, const Faces &fc, const T *W, const T *Wy, int i0,
+                                               int P, int j, int ks, double *kx0, double *kx1, double *ky)
 {
-    return sizeof(T) == 4;
+    if (sizeof(T) == 4) strip_k(W, Wy, i0, P, kx0, kx1, ky);
+    else strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
 }
 
-// 1/(2w) over the whole padded index space, rounded to FP32
-__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *w8, float *h4)
+// The chunk's Kx, Ky rounded to FP32 over the whole padded index space
+__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *kx8, const double *ky8,
+                                                   float *kx4, float *ky4)
 {
     const int w = g8.x_max + 2 * g8.hd, h = g8.y_max + 2 * g8.hd;
     const long n = (long)w * h;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const int j = (int)(t % w), k = (int)(t / w);
-        h4[IX(g4, j, k)] = (float)(0.5 / w8[IX(g8, j, k)]);
+        kx4[IX(g4, j, k)] = (float)kx8[IX(g8, j, k)];
+        ky4[IX(g4, j, k)] = (float)ky8[IX(g8, j, k)];
     }
+}
+
+// Preconditioner reciprocals of the inner operator: FP64 divisions for FP64
+// inner fields, the FP32 hardware reciprocal for FP32 inner fields (whose
+// M^-1 r is rounded to FP32 anyway). Every kernel of one precision uses the
+// same one, so the preconditioner is one fixed symmetric operator.
+template <typename T>
+constexpr bool inner_fast_rcp()
+{
+    return sizeof(T) == 4;
 }
 
 // Reliable update: u_out = u_in + d (FP64), r = u0 - A u_out with the FP64
@@ -167,8 +186,8 @@ __global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const doubl
 template <typename T, int PREC>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, Ext e, const double *u_in,
                                                              double *u_out, const T *d, const double *u0,
-                                                             const double *cond8, const T *cond_in, Faces fc,
-                                                             T *r, T *z, RfArgs ra)
+                                                             const double *cond8, const T *cond_in,
+                                                             const T *cond_in_y, Faces fc, T *r, T *z, RfArgs ra)
 {
     double acc[2] = {0.0, 0.0};
     TL_TILE_LOOP(e)
@@ -199,8 +218,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
     double kx0t[STRIP], kx1t[STRIP], kyt[STRIP + 1];
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) rt[q] = rnd<T>(rv[q]);
-    strip_k_cond<inner_half_inv<T>()>(g4, fc, cond_in, i4, P4, j, ks, kx0t, kx1t, kyt);
-    strip_precondition<PREC>(g4, ks, i4, kx0t, kx1t, kyt, rt, zv);
+    rf_strip_faces<T>(g4, fc, cond_in, cond_in_y, i4, P4, j, ks, kx0t, kx1t, kyt);
+    strip_precondition<PREC, inner_fast_rcp<T>()>(g4, ks, i4, kx0t, kx1t, kyt, rt, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
@@ -219,7 +238,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
 // double-buffered by the parity of the iteration counter.
 template <typename T>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *src, T *pA, T *pB,
-                                                        const T *W, Faces fc, RfArgs ra)
+                                                        const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -232,7 +251,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
-    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col6(src, i0, P, sc);
     col6(p_old, i0, P, pc);
     col4(src, i0 - 1, P, sw);
@@ -264,7 +283,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
 // coefficients; r.z and r.r of the stored values.
 template <typename T, int PREC>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
-                                                         T *z, const T *W, Faces fc, RfArgs ra)
+                                                         T *z, const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -274,7 +293,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
-    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -284,7 +303,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
         rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
     }
     mask_rows(ks, e.k1, rv);
-    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, zv);
+    strip_precondition<PREC, inner_fast_rcp<T>()>(g, ks, i0, kx0, kx1, ky, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
@@ -306,8 +325,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
 // r.z is formed after the inner steps with this z, as the reference does).
 template <typename T, int PREC>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *d, const T *pA, const T *pB,
-                                                              const T *r, T *r_out, T *z, T *sd, const T *W, Faces fc,
-                                                              RfArgs ra)
+                                                              const T *r, T *r_out, T *z, T *sd, const T *W,
+                                                              const T *Wy, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -316,7 +335,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
-    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -326,7 +345,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *
         rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
     }
     mask_rows(ks, e.k1, rv);
-    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, zv);
+    strip_precondition<PREC, inner_fast_rcp<T>()>(g, ks, i0, kx0, kx1, ky, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
@@ -353,7 +372,7 @@ __device__ __forceinline__ void rf_ppcg_step(const Geom &g, int ks, int k1, int 
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) rv[q] = rnd<T>(rv[q] - res[q]);
     mask_rows(ks, k1, rv);
-    strip_precondition<PREC>(g, ks, i0, kx0, kx1, ky, rv, mv);
+    strip_precondition<PREC, inner_fast_rcp<T>()>(g, ks, i0, kx0, kx1, ky, rv, mv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) sdn[q] = rnd<T>(fma(ca, sc[q], cb * mv[q]));
 }
@@ -363,8 +382,9 @@ __device__ __forceinline__ void rf_ppcg_step(const Geom &g, int ks, int k1, int 
 // also sums r.z with the z of the outer step and r.r.
 template <typename T, int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d, T *r, const T *sd_in, T *sd_out,
-                                                            const T *W, Faces fc, const T *z, const double *ch_a,
-                                                            const double *ch_b, int step, RfArgs ra)
+                                                            const T *W, const T *Wy, Faces fc, const T *z,
+                                                            const double *ch_a, const double *ch_b, int step,
+                                                            RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
@@ -374,7 +394,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d,
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], dv[STRIP], rv[STRIP], sdn[STRIP],
         zo[STRIP];
-    strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     if (LAST) col4_nt(z, i0, P, zo);
@@ -398,15 +418,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d,
 
 // face coefficients of a strip for the two-step kernel: read from the
 // chunk's Kx, Ky for FP64 inner fields (bitwise what the one-step kernel
-// recomputes from the conductivity, without its divisions), formed from
-// 1/(2w) for FP32 inner fields (no division either way)
+// recomputes from the conductivity, without its divisions), from their FP32
+// copies for FP32 inner fields (what the one-step kernel reads)
 template <typename T>
-__device__ __forceinline__ void rf_faces(const Geom &g, const Faces &fc, const T *W, const double *Kx,
-                                         const double *Ky, int i0, int P, int j, int ks, double *kx0, double *kx1,
-                                         double *ky)
+__device__ __forceinline__ void rf_faces(const T *W, const T *Wy, const double *Kx, const double *Ky, int i0, int P,
+                                         double *kx0, double *kx1, double *ky)
 {
     if (sizeof(T) == 8) strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
-    else strip_k_cond<inner_half_inv<T>()>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    else strip_k(W, Wy, i0, P, kx0, kx1, ky);
 }
 
 // sd at a ring cell the block does not compute: the sd buffer's halo cell
@@ -428,7 +447,7 @@ __device__ __forceinline__ double rf_ring_outside(const Geom &g, const Ext &e, c
 // blocks read this tile's r and sd in their step 1).
 template <typename T, int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, T *d, const T *r_in, T *r_out,
-                                                             const T *sd_in, T *sd_out, const T *W, Faces fc,
+                                                             const T *sd_in, T *sd_out, const T *W, const T *Wy,
                                                              const double *Kx, const double *Ky, const T *z,
                                                              const double *ch_a, const double *ch_b, int step,
                                                              RfArgs ra)
@@ -467,7 +486,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, T *d
                 if (inside) {
                     const int h0 = IX(g, hj, hk);
                     double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP];
-                    rf_faces<T>(g, fc, W, Kx, Ky, h0, P, hj, hk, hx0, hx1, hy);
+                    rf_faces<T>(W, Wy, Kx, Ky, h0, P, hx0, hx1, hy);
                     col4(r_in, h0, P, hr);
                     strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
                     rf_ppcg_step<T, PREC>(g, hk, e.k1, h0, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn);
@@ -488,7 +507,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, T *d
         double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], dv[STRIP];
         if (act) {
             double res[STRIP], sc[STRIP], sdn[STRIP];
-            rf_faces<T>(g, fc, W, Kx, Ky, i0, P, j, ks, kx0, kx1, ky);
+            rf_faces<T>(W, Wy, Kx, Ky, i0, P, kx0, kx1, ky);
             col4_nt(d, i0, P, dv);
             col4_nt(r_in, i0, P, rv);
             strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
@@ -557,7 +576,7 @@ void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, dou
     T *const *f = reinterpret_cast<T *const *>(w.f);
 #define TL_RFU(P)                                                                                           \
     hipLaunchKernelGGL((k_rf_update<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, u_out, \
-                       f[RW_D], c->f2[F_U0], c->f2[F_COND], f[RW_W], c->faces, f[RW_R], f[RW_Z], ra)
+                       f[RW_D], c->f2[F_U0], c->f2[F_COND], f[RW_W], f[RW_KY], c->faces, f[RW_R], f[RW_Z], ra)
     if (c->prec == PREC_BLOCK) TL_RFU(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFU(PREC_DIAG);
     else TL_RFU(PREC_NONE);
@@ -580,13 +599,13 @@ void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nul
         ra_u{c->partials, w.st, RF_RRN, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) hipEventRecord(ev[0], c->s);
     hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
-                       f[RW_W], c->faces, ra_w);
+                       f[RW_W], f[RW_KY], c->faces, ra_w);
     if (ev) hipEventRecord(ev[1], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
     if (ev) hipEventRecord(ev[2], c->s);
 #define TL_RFI(P)                                                                                       \
     hipLaunchKernelGGL((k_rf_ur<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
-                       f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], c->faces, ra_u)
+                       f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], f[RW_KY], c->faces, ra_u)
     if (c->prec == PREC_BLOCK) TL_RFI(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
     else TL_RFI(PREC_NONE);
@@ -688,7 +707,7 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
     const RfArgs ra_w{c->partials, w.st, RF_PW, nullptr, nullptr, 0},
         ra_u{c->partials, w.st, RF_RRN, nullptr, nullptr, 0};
     hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
-                       f[RW_W], c->faces, ra_w);
+                       f[RW_W], f[RW_KY], c->faces, ra_w);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
     const double *ca = w.coef, *cb = w.coef + RF_MAX_INNER;
     // two inner steps per launch: r and sd flip between their two buffers at
@@ -701,16 +720,17 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
 #define TL_RFO(P)                                                                                              \
     do {                                                                                                       \
         hipLaunchKernelGGL((k_rf_ur_ppcg<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D],     \
-                           f[RW_P], f[RW_P2], f[RW_R], r_cur, f[RW_Z], f[RW_SD], f[RW_W], c->faces, ra_u);      \
+                           f[RW_P], f[RW_P2], f[RW_R], r_cur, f[RW_Z], f[RW_SD], f[RW_W], f[RW_KY], c->faces, \
+                           ra_u);                                                                              \
         int i = 0;                                                                                             \
         for (int k = 0; k < pairs; ++k, i += 2) {                                                              \
             if (i + 2 == inner)                                                                                \
                 hipLaunchKernelGGL((k_rf_inner2<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
-                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], c->faces, Kx, Ky, f[RW_Z],   \
+                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], f[RW_KY], Kx, Ky, f[RW_Z],   \
                                    ca, cb, i, ra_u);                                                           \
             else                                                                                               \
                 hipLaunchKernelGGL((k_rf_inner2<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
-                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], c->faces, Kx, Ky, f[RW_Z],   \
+                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], f[RW_KY], Kx, Ky, f[RW_Z],   \
                                    ca, cb, i, ra_u);                                                           \
             std::swap(r_cur, r_oth);                                                                           \
             std::swap(sd_cur, sd_oth);                                                                         \
@@ -718,10 +738,12 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
         for (; i < inner; ++i) {                                                                               \
             if (i == inner - 1)                                                                                \
                 hipLaunchKernelGGL((k_rf_inner<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
-                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u); \
+                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], f[RW_KY], c->faces, f[RW_Z], ca, cb, \
+                                   i, ra_u);                                                                   \
             else                                                                                               \
                 hipLaunchKernelGGL((k_rf_inner<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
-                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], c->faces, f[RW_Z], ca, cb, i, ra_u); \
+                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], f[RW_KY], c->faces, f[RW_Z], ca, cb, \
+                                   i, ra_u);                                                                   \
             std::swap(sd_cur, sd_oth);                                                                         \
         }                                                                                                      \
     } while (0)
@@ -878,8 +900,8 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
     hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
     hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
     if (which == 1)
-        hipLaunchKernelGGL(k_rf_narrow, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_COND],
-                           static_cast<float *>(w.f[RW_W]));
+        hipLaunchKernelGGL(k_rf_narrow, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_KX], c->f2[F_KY],
+                           static_cast<float *>(w.f[RW_W]), static_cast<float *>(w.f[RW_KY]));
     double *ubuf[2] = {c->f2[F_U], c->f2[F_U2]};
     int cur = 0;
     launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);
