@@ -50,6 +50,8 @@ ALG_BYTES = {
 REF_BYTES = {"cg": 144, "cheby": 104, "ppcg_inner": 112, "ppcg_outer": 208}
 # the refined solve's inner kernels (tl_refine.hip): same streams as the fused
 # FP64 kernels, with r, z, p, d and the face field stored in FP64 or FP32
+# refined CG kernels: faces from one per-cell field (FP64: the conductivity,
+# FP32: 1/(2w))
 ALG_BYTES_RF = {
     "fp64": {"k_rf_w": 4 * 8, "k_rf_ur": 7 * 8},
     "mixed": {"k_rf_w": 4 * 4, "k_rf_ur": 7 * 4},

@@ -85,8 +85,9 @@ struct tl_chunk {
         tl::Geom g{};                     // layout of the inner fields
         long off = 0, elems = 0;
         void *base[10] = {};              // owned allocations
-        void *f[10] = {};                 // faces (FP64: conductivity w; FP32: Kx), r, z, p, p2, d, sd,
-                                          // sd2, r2, FP32 Ky
+        void *f[10] = {};                 // faces (FP64: conductivity w; FP32: h = 1/(2w) or Kx), r, z,
+                                          // p, p2, d, sd, sd2, r2, FP32 Ky
+        int faces = 0;                    // face representation of the solve in progress
         tl::RefineState *st = nullptr;
         tl::RefineState *h_st = nullptr;  // pinned, 2 slots
         GraphSlot graph;                  // CG iterations

@@ -265,33 +265,43 @@ __device__ __forceinline__ double face_coef(double wa, double wb, double scale)
     return v;
 }
 
+// The same face from h = 1/(2w) stored per cell: scale * (h_a + h_b), equal
+// to face_coef in exact arithmetic, with no division (the CG kernels of the
+// FP32-inner solve, whose operator is an FP32-rounded one anyway).
+__device__ __forceinline__ double face_coef_h(double ha, double hb, double scale)
+{
+    return scale * (ha + hb);
+}
+
 __device__ __forceinline__ bool outside_init(const Geom &g, int j, int k)
 {
     return j < 1 || j > g.x_max + 2 * g.hd - 1 || k < 1 || k > g.y_max + 2 * g.hd - 1;
 }
 
+template <bool HALF_INV = false>
 __device__ __forceinline__ double kx_at(const Geom &g, const Faces &f, int j, int k, double wl, double wc)
 {
     const int hd = g.hd;
     const bool zero = outside_init(g, j, k) || (f.zl && j <= hd) || (f.zr && j > g.x_max - 1 + hd) ||
                       (f.zb && k < hd) || (f.zt && k > g.y_max - 1 + hd);
-    const double v = face_coef(wl, wc, f.rx);
+    const double v = HALF_INV ? face_coef_h(wl, wc, f.rx) : face_coef(wl, wc, f.rx);
     return zero ? 0.0 : v;
 }
 
+template <bool HALF_INV = false>
 __device__ __forceinline__ double ky_at(const Geom &g, const Faces &f, int j, int k, double wd, double wc)
 {
     const int hd = g.hd;
     const bool zero = outside_init(g, j, k) || (f.zl && j < hd) || (f.zr && j > g.x_max - 1 + hd) ||
                       (f.zb && k <= hd) || (f.zt && k > g.y_max - 1 + hd);
-    const double v = face_coef(wd, wc, f.ry);
+    const double v = HALF_INV ? face_coef_h(wd, wc, f.ry) : face_coef(wd, wc, f.ry);
     return zero ? 0.0 : v;
 }
 
 // face coefficients of the strip (same outputs as strip_k) recomputed from a
 // per-cell field stored as T: the conductivity w (bitwise the init_common
-// faces)
-template <typename T>
+// faces), or with HALF_INV its 1/(2w)
+template <bool HALF_INV = false, typename T>
 __device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, const T *w, int i0, int P, int j, int ks,
                                              double *kx0, double *kx1, double *ky)
 {
@@ -307,21 +317,21 @@ __device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, cons
     if (inner) {
 #pragma unroll
         for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = face_coef(ww[q], wc[q + 1], f.rx);
-            kx1[q] = face_coef(wc[q + 1], we[q], f.rx);
+            kx0[q] = HALF_INV ? face_coef_h(ww[q], wc[q + 1], f.rx) : face_coef(ww[q], wc[q + 1], f.rx);
+            kx1[q] = HALF_INV ? face_coef_h(wc[q + 1], we[q], f.rx) : face_coef(wc[q + 1], we[q], f.rx);
         }
 #pragma unroll
         for (int q = 0; q < STRIP + 1; ++q)
-            ky[q] = face_coef(wc[q], wc[q + 1], f.ry);
+            ky[q] = HALF_INV ? face_coef_h(wc[q], wc[q + 1], f.ry) : face_coef(wc[q], wc[q + 1], f.ry);
         return;
     }
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        kx0[q] = kx_at(g, f, j, ks + q, ww[q], wc[q + 1]);
-        kx1[q] = kx_at(g, f, j + 1, ks + q, wc[q + 1], we[q]);
+        kx0[q] = kx_at<HALF_INV>(g, f, j, ks + q, ww[q], wc[q + 1]);
+        kx1[q] = kx_at<HALF_INV>(g, f, j + 1, ks + q, wc[q + 1], we[q]);
     }
 #pragma unroll
-    for (int q = 0; q < STRIP + 1; ++q) ky[q] = ky_at(g, f, j, ks + q, wc[q], wc[q + 1]);
+    for (int q = 0; q < STRIP + 1; ++q) ky[q] = ky_at<HALF_INV>(g, f, j, ks + q, wc[q], wc[q + 1]);
 }
 
 // w = A v on the strip: c6 = centre column rows ks-1..ks+4, w4/e4 = west/east

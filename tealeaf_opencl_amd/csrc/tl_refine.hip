@@ -141,22 +141,50 @@ __device__ __forceinline__ double rnd(double v)
     return (double)(T)v;
 }
 
-// Face coefficients of a strip for the inner operator. FP64 inner fields:
-// recomputed from the conductivity W (bitwise the faces tea_leaf_init_common
-// stores). FP32 inner fields: read from W, Wy, the chunk's Kx, Ky rounded to
-// FP32 (k_rf_narrow), so that the inner operator is the FP64 one with its
-// faces rounded to FP32 and its kernels carry no face formulas or wall tests.
-template <typename T>
+// Face coefficients of the inner operator, in the representation the solve
+// uses in every one of its kernels (so that its operator is one fixed
+// matrix): FP64 inner fields recompute them from the conductivity (bitwise
+// the faces tea_leaf_init_common stores); an FP32-inner CG solve forms them
+// from h = 1/(2w) rounded to FP32 (one field and no division: its kernels are
+// bound by memory traffic); an FP32-inner PPCG solve reads the chunk's Kx, Ky
+// rounded to FP32 (two fields, but no face formula and no wall test: its
+// smoothing kernels are bound by arithmetic).
+enum { RF_FACES_COND = 0, RF_FACES_HALF = 1, RF_FACES_K32 = 2 };
+
+template <typename T, int FACES>
 __device__ __forceinline__ void rf_strip_faces(const Geom &g, const Faces &fc, const T *W, const T *Wy, int i0,
                                                int P, int j, int ks, double *kx0, double *kx1, double *ky)
 {
-    if (sizeof(T) == 4) strip_k(W, Wy, i0, P, kx0, kx1, ky);
-    else strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    if (FACES == RF_FACES_K32) strip_k(W, Wy, i0, P, kx0, kx1, ky);
+    else strip_k_cond<FACES == RF_FACES_HALF>(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+}
+
+// the representation of a CG solve's and of a PPCG solve's faces
+template <typename T>
+constexpr int cg_faces()
+{
+    return sizeof(T) == 4 ? RF_FACES_HALF : RF_FACES_COND;
+}
+template <typename T>
+constexpr int ppcg_faces()
+{
+    return sizeof(T) == 4 ? RF_FACES_K32 : RF_FACES_COND;
+}
+
+// h = 1/(2w) over the whole padded index space, rounded to FP32
+__global__ void __launch_bounds__(256) k_rf_narrow_half(Geom g8, Geom g4, const double *w8, float *h4)
+{
+    const int w = g8.x_max + 2 * g8.hd, h = g8.y_max + 2 * g8.hd;
+    const long n = (long)w * h;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int j = (int)(t % w), k = (int)(t / w);
+        h4[IX(g4, j, k)] = (float)(0.5 / w8[IX(g8, j, k)]);
+    }
 }
 
 // The chunk's Kx, Ky rounded to FP32 over the whole padded index space
-__global__ void __launch_bounds__(256) k_rf_narrow(Geom g8, Geom g4, const double *kx8, const double *ky8,
-                                                   float *kx4, float *ky4)
+__global__ void __launch_bounds__(256) k_rf_narrow_faces(Geom g8, Geom g4, const double *kx8, const double *ky8,
+                                                         float *kx4, float *ky4)
 {
     const int w = g8.x_max + 2 * g8.hd, h = g8.y_max + 2 * g8.hd;
     const long n = (long)w * h;
@@ -183,7 +211,7 @@ constexpr bool inner_fast_rcp()
 // preconditioner (faces formed like the inner kernels form them). Sums:
 // r.z of the stored values, and the FP64 r.r. d is read at stencil
 // neighbours, so it is cleared after the kernel, not in it.
-template <typename T, int PREC>
+template <typename T, int PREC, int FACES>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, Ext e, const double *u_in,
                                                              double *u_out, const T *d, const double *u0,
                                                              const double *cond8, const T *cond_in,
@@ -218,7 +246,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
     double kx0t[STRIP], kx1t[STRIP], kyt[STRIP + 1];
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) rt[q] = rnd<T>(rv[q]);
-    rf_strip_faces<T>(g4, fc, cond_in, cond_in_y, i4, P4, j, ks, kx0t, kx1t, kyt);
+    rf_strip_faces<T, FACES>(g4, fc, cond_in, cond_in_y, i4, P4, j, ks, kx0t, kx1t, kyt);
     strip_precondition<PREC, inner_fast_rcp<T>()>(g4, ks, i4, kx0t, kx1t, kyt, rt, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
@@ -236,7 +264,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_update(Geom g8, Geom g4, E
 
 // p = z + beta p_old (rounded to the storage), w = A p, p.w. p is
 // double-buffered by the parity of the iteration counter.
-template <typename T>
+template <typename T, int FACES>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *src, T *pA, T *pB,
                                                         const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
@@ -251,7 +279,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
     double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
-    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T, FACES>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col6(src, i0, P, sc);
     col6(p_old, i0, P, pc);
     col4(src, i0 - 1, P, sw);
@@ -281,7 +309,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, const T *
 // d += alpha p, r -= alpha A p (A p recomputed from the stored p),
 // z = M^-1 r with the preconditioner recomputed from the stored
 // coefficients; r.z and r.r of the stored values.
-template <typename T, int PREC>
+template <typename T, int PREC, int FACES>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
                                                          T *z, const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
@@ -293,7 +321,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
-    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T, FACES>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -335,7 +363,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur_ppcg(Geom g, Ext e, T *
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
-    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T, ppcg_faces<T>()>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
@@ -394,7 +422,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d,
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], dv[STRIP], rv[STRIP], sdn[STRIP],
         zo[STRIP];
-    rf_strip_faces<T>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
+    rf_strip_faces<T, ppcg_faces<T>()>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     if (LAST) col4_nt(z, i0, P, zo);
@@ -567,16 +595,17 @@ inline int grid_of(const Ext &e, int cap)
     return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
 }
 
-template <typename T>
-void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, double *u_out)
+template <typename T, int FACES>
+void launch_update_f(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, double *u_out)
 {
     const Ext e = ext_interior(c);
     const int grid = grid_of(e, c->cap);
     const RfArgs ra{c->partials, w.st, RF_UPDATE, nullptr, nullptr, 0};
     T *const *f = reinterpret_cast<T *const *>(w.f);
 #define TL_RFU(P)                                                                                           \
-    hipLaunchKernelGGL((k_rf_update<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, u_out, \
-                       f[RW_D], c->f2[F_U0], c->f2[F_COND], f[RW_W], f[RW_KY], c->faces, f[RW_R], f[RW_Z], ra)
+    hipLaunchKernelGGL((k_rf_update<T, P, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, \
+                       u_out, f[RW_D], c->f2[F_U0], c->f2[F_COND], f[RW_W], f[RW_KY], c->faces, f[RW_R], f[RW_Z], \
+                       ra)
     if (c->prec == PREC_BLOCK) TL_RFU(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFU(PREC_DIAG);
     else TL_RFU(PREC_NONE);
@@ -586,10 +615,18 @@ void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, dou
     hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
 }
 
+// with the faces of the solve in progress (w.faces)
+template <typename T>
+void launch_update(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, double *u_out)
+{
+    if (w.faces == RF_FACES_K32) launch_update_f<T, ppcg_faces<T>()>(c, w, u_in, u_out);
+    else launch_update_f<T, cg_faces<T>()>(c, w, u_in, u_out);
+}
+
 // One inner iteration; with `ev`, ev[0..1] bracket k_rf_w and ev[2..3]
 // k_rf_ur (their finishers outside the brackets).
-template <typename T>
-void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nullptr)
+template <typename T, int FACES>
+void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
 {
     const Ext e = ext_interior(c);
     const int grid = grid_of(e, c->cap);
@@ -598,13 +635,13 @@ void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nul
     const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
         ra_u{c->partials, w.st, RF_RRN, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) hipEventRecord(ev[0], c->s);
-    hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
+    hipLaunchKernelGGL((k_rf_w<T, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
                        f[RW_W], f[RW_KY], c->faces, ra_w);
     if (ev) hipEventRecord(ev[1], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
     if (ev) hipEventRecord(ev[2], c->s);
 #define TL_RFI(P)                                                                                       \
-    hipLaunchKernelGGL((k_rf_ur<T, P>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
+    hipLaunchKernelGGL((k_rf_ur<T, P, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
                        f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], f[RW_KY], c->faces, ra_u)
     if (c->prec == PREC_BLOCK) TL_RFI(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
@@ -612,6 +649,13 @@ void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nul
 #undef TL_RFI
     if (ev) hipEventRecord(ev[3], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
+}
+
+template <typename T>
+void launch_iteration(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev = nullptr)
+{
+    if (w.faces == RF_FACES_K32) launch_iteration_f<T, ppcg_faces<T>()>(c, w, ev);
+    else launch_iteration_f<T, cg_faces<T>()>(c, w, ev);
 }
 
 constexpr int RF_BATCH = 16;
@@ -706,8 +750,8 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW, nullptr, nullptr, 0},
         ra_u{c->partials, w.st, RF_RRN, nullptr, nullptr, 0};
-    hipLaunchKernelGGL(k_rf_w<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src, f[RW_P], f[RW_P2],
-                       f[RW_W], f[RW_KY], c->faces, ra_w);
+    hipLaunchKernelGGL((k_rf_w<T, ppcg_faces<T>()>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, src,
+                       f[RW_P], f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
     const double *ca = w.coef, *cb = w.coef + RF_MAX_INNER;
     // two inner steps per launch: r and sd flip between their two buffers at
@@ -781,7 +825,7 @@ int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::funct
 template <typename T>
 int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
 {
-    return capture_into(c, w.graph, c->prec, [&]() {
+    return capture_into(c, w.graph, c->prec * 4 + w.faces, [&]() {
         for (int b = 0; b < RF_BATCH - 1; ++b) launch_iteration<T>(c, w);
     });
 }
@@ -885,6 +929,7 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
         hipEvent_t a, b;
         ~EvGuard() { hipEventDestroy(a); hipEventDestroy(b); }
     } guard{t0, t1};
+    w.faces = inner > 0 ? ppcg_faces<T>() : cg_faces<T>();
     if ((rc = rf_capture<T>(c, w)) != TL_OK) return rc;
     if (inner > 0 && (rc = rf_capture_ppcg<T>(c, w, inner)) != TL_OK) return rc;
 
@@ -899,9 +944,12 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
     hipEventRecord(t0, c->s);
     hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
     hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
-    if (which == 1)
-        hipLaunchKernelGGL(k_rf_narrow, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_KX], c->f2[F_KY],
-                           static_cast<float *>(w.f[RW_W]), static_cast<float *>(w.f[RW_KY]));
+    if (w.faces == RF_FACES_HALF)
+        hipLaunchKernelGGL(k_rf_narrow_half, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_COND],
+                           static_cast<float *>(w.f[RW_W]));
+    else if (w.faces == RF_FACES_K32)
+        hipLaunchKernelGGL(k_rf_narrow_faces, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_KX],
+                           c->f2[F_KY], static_cast<float *>(w.f[RW_W]), static_cast<float *>(w.f[RW_KY]));
     double *ubuf[2] = {c->f2[F_U], c->f2[F_U2]};
     int cur = 0;
     launch_update<T>(c, w, ubuf[cur], ubuf[1 - cur]);

@@ -87,6 +87,39 @@ def test_deterministic_and_reentrant(hiplib, oracle):
     assert np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_cg_and_ppcg_on_one_chunk(hiplib, oracle, precision):
+    """CG, then PPCG, then CG again on one chunk: the FP32-inner CG solve
+    forms its faces from 1/(2w) and the PPCG solve reads FP32 copies of Kx,
+    Ky, so the workspace's face fields and the captured CG graph change
+    between the solves. Each solve gives bitwise the solution of the same
+    solve on a fresh chunk."""
+    def fresh():
+        g, _ = pair(hiplib, oracle, 200, 41, 1, 3, 2)
+        return g
+
+    def cg(g):
+        res = g.solve_refined(precision, tol=1e-10)
+        assert res["converged"], res
+        return res["iterations"], g.get_field("u")
+
+    def ppcg(g):
+        res = g.solve_refined_ppcg(precision, tol=1e-10, max_iters=5000, presteps=20, inner_steps=6)
+        assert res["converged"], res
+        return res["iterations"], g.get_field("u")
+
+    ref_cg, ref_pp = cg(fresh()), ppcg(fresh())
+    g = fresh()
+    got = [cg(g)]
+    g.common_init(2, 0.004)
+    got.append(ppcg(g))
+    g.common_init(2, 0.004)
+    got.append(cg(g))
+    for (n, u), (n_ref, u_ref) in zip(got, [ref_cg, ref_pp, ref_cg]):
+        assert n == n_ref
+        assert np.array_equal(u, u_ref)
+
+
 def test_iteration_cap(hiplib, oracle):
     g, _ = pair(hiplib, oracle, 200, 41, 1, 2, 2)
     res = g.solve_refined(1, tol=1e-12, max_iters=7)
