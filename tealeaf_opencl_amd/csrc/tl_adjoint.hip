@@ -58,11 +58,15 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_adj_seed(Geom g, Ext e, const
     block_partial<1>(acc, partials);
 }
 
-__global__ void __launch_bounds__(FINISH_THREADS) k_adj_sum(const double *partials, int nblocks, double *result)
+// the loss to the host as the drop-in reductions return theirs (read_result)
+__global__ void __launch_bounds__(FINISH_THREADS) k_adj_sum(const double *partials, int nblocks, RedArgs ra)
 {
     double tot[MAX_RED];
     sum_partials(partials, nblocks, 1, tot);
-    if (threadIdx.x == 0) result[0] = tot[0];
+    if (threadIdx.x == 0) {
+        ra.result[0] = tot[0];
+        __hip_atomic_store(ra.flag, ra.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // whether tea_leaf_zero_boundary leaves the face zero (the conditions of
@@ -236,7 +240,8 @@ int loss_and_seed(tl_chunk *c, const double *u_obs, double *gvec, double *loss)
     const int grid = grid_of(e, c->cap);
     hipLaunchKernelGGL(k_adj_seed, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, e, c->f2[F_U], u_obs, gvec,
                        c->partials);
-    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(FINISH_THREADS), 0, c->s, c->partials, grid, c->result);
+    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(FINISH_THREADS), 0, c->s, c->partials, grid,
+                       red_args(c, RED_STORE, false));
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return hip_fail("launch adjoint seed", err);
     return read_result(c, loss, 1);

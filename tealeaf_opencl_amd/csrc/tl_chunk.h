@@ -35,7 +35,11 @@ struct tl_chunk {
     int ch_cap = 0;
     int n_coefs = 0;
 
-    double *h_result = nullptr;      // pinned
+    double *h_result = nullptr;      // pinned, coherent: totals of host-returned reductions
+    double *d_result = nullptr;      // h_result's device address
+    unsigned int *h_flag = nullptr;  // pinned, coherent: sequence number of the latest totals
+    unsigned int *d_flag = nullptr;
+    unsigned int seq = 0;            // sequence number of the latest host-returned reduction
     tl::SolveState *h_st = nullptr;  // pinned, 2 slots
 
     // host copies of scalars the reference keeps in kernel arguments

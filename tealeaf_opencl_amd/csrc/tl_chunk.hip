@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -69,7 +70,10 @@ RedArgs red_args(tl_chunk *c, int mode, bool with_state)
 {
     RedArgs ra;
     ra.partials = c->partials;
-    ra.result = c->result;
+    // without a device state the totals go to the host (read_result)
+    ra.result = with_state ? c->result : c->d_result;
+    ra.flag = with_state ? nullptr : c->d_flag;
+    ra.seq = with_state ? 0u : ++c->seq;
     ra.st = with_state ? c->st : nullptr;
     ra.hist_a = with_state ? c->hist_a : nullptr;
     ra.hist_b = with_state ? c->hist_b : nullptr;
@@ -137,11 +141,28 @@ int ensure_hist_capacity(tl_chunk *c, int n)
     return TL_OK;
 }
 
+// Totals of the latest reduction launched with red_args(c, mode, false): its
+// finisher wrote them to pinned host memory and then stored its sequence
+// number into the pinned flag, so the host polls the flag instead of
+// copying and synchronising (the round trip of each blocking scalar the
+// reference's Fortran driver asks for). After 20 ms without the flag (a long
+// kernel queue ahead, or an error) it waits for the stream, which also
+// reports any error.
 int read_result(tl_chunk *c, double *out, int nq)
 {
-    hipError_t e = hipMemcpyAsync(c->h_result, c->result, sizeof(double) * MAX_RED, hipMemcpyDeviceToHost, c->s);
-    if (e != hipSuccess) return hip_fail("hipMemcpyAsync result", e);
-    if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("hipStreamSynchronize", e);
+    const unsigned int want = c->seq;
+    bool seen = __atomic_load_n(c->h_flag, __ATOMIC_ACQUIRE) == want;
+    if (!seen) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (long spin = 1; !seen; ++spin) {
+            seen = __atomic_load_n(c->h_flag, __ATOMIC_ACQUIRE) == want;
+            if ((spin & 1023) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) break;
+        }
+    }
+    if (!seen) {
+        hipError_t e = hipStreamSynchronize(c->s);
+        if (e != hipSuccess) return hip_fail("hipStreamSynchronize", e);
+    }
     for (int q = 0; q < nq; ++q) out[q] = c->h_result[q];
     return TL_OK;
 }
@@ -187,6 +208,7 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
                     (void *)c->ch_b})
         if (p) hipFree(p);
     if (c->h_result) hipHostFree(c->h_result);
+    if (c->h_flag) hipHostFree(c->h_flag);
     if (c->h_st) hipHostFree(c->h_st);
     for (hipEvent_t ev : {c->pe0, c->pe1, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[0][3],
                           c->ev_t[1][0], c->ev_t[1][1], c->ev_t[1][2], c->ev_t[1][3], c->ev_batch[0],
@@ -257,12 +279,16 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     if ((e = hipMalloc(&c->partials, sizeof(double) * MAX_RED * c->cap)) != hipSuccess ||
         (e = hipMalloc(&c->result, sizeof(double) * MAX_RED)) != hipSuccess ||
         (e = hipMalloc(&c->st, sizeof(SolveState))) != hipSuccess ||
-        (e = hipHostMalloc(&c->h_result, sizeof(double) * MAX_RED, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(&c->h_result, sizeof(double) * MAX_RED, hipHostMallocCoherent)) != hipSuccess ||
+        (e = hipHostMalloc(&c->h_flag, 64, hipHostMallocCoherent)) != hipSuccess ||
+        (e = hipHostGetDevicePointer((void **)&c->d_result, c->h_result, 0)) != hipSuccess ||
+        (e = hipHostGetDevicePointer((void **)&c->d_flag, c->h_flag, 0)) != hipSuccess ||
         (e = hipHostMalloc(&c->h_st, sizeof(SolveState) * 2, hipHostMallocDefault)) != hipSuccess) {
         tl_chunk_destroy(c);
         return hip_fail("hipMalloc scratch", e);
     }
     hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
+    *c->h_flag = 0u;
     for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[0][3],
                            &c->ev_t[1][0], &c->ev_t[1][1], &c->ev_t[1][2], &c->ev_t[1][3]})
         if ((e = hipEventCreate(ev)) != hipSuccess) {

@@ -116,6 +116,10 @@ struct RedArgs {
     double *hist_b;       // cg_betas
     double *hist_e;       // residual trajectory
     int mode;
+    // host-returned totals (drop-in calls): result points at pinned host
+    // memory and, once it holds them, the finisher stores seq into *flag
+    unsigned int *flag;
+    unsigned int seq;
 };
 
 // Device-resident scalars of the refined solve (tl_refine.hip): an inner

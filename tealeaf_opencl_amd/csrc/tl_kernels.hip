@@ -64,6 +64,8 @@ __device__ void finish_scalars(const RedArgs &ra, const double *tot)
 {
 #pragma unroll
     for (int q = 0; q < MAX_RED; ++q) ra.result[q] = tot[q];
+    // the release orders the totals before the flag for the host's reader
+    if (ra.flag) __hip_atomic_store(ra.flag, ra.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     SolveState *s = ra.st;
     if (!s) return;
     switch (ra.mode) {
