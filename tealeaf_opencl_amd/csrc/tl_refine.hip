@@ -4,10 +4,10 @@
 // FP64 inner fields: the operator's faces are recomputed from the
 // conductivity (bitwise those of tea_leaf_init_common) and the
 // preconditioner's pivots divided exactly. FP32 inner fields: the operator's
-// faces are Kx, Ky rounded to FP32, read from two FP32 fields, and the
-// preconditioner's pivot reciprocals come from the FP32 hardware reciprocal.
-// Every other arithmetic operation, every dot product and every scalar is
-// FP64. The inner solve is
+// faces come from FP32 fields (1/(2w) in a CG solve, Kx and Ky rounded to
+// FP32 in a PPCG solve) and the preconditioner's pivot reciprocals from the
+// FP32 hardware reciprocal. Every other arithmetic operation, every dot
+// product and every scalar is FP64. The inner solve is
 // wrapped in FP64 reliable updates: whenever the recursive residual has
 // dropped by `delta` since the last update, or has reached the tolerance, the
 // increment is added into the FP64 solution (u += d, d = 0) and the residual
