@@ -491,6 +491,26 @@ void tlo_block_solve(tlo_chunk *c)
         }
 }
 
+/* z = M^-1 r over the interior: the block solve above, or Mi r (the z of
+ * tea_leaf_cg_cl.cl:8-41 for the diagonal preconditioner); nothing without a
+ * preconditioner. The fresh-z PPCG option calls it before r.z. */
+void tlo_precondition_r(tlo_chunk *c)
+{
+    const int hd = c->hd;
+    if (c->preconditioner == TLO_PREC_JAC_BLOCK) {
+        tlo_block_solve(c);
+    } else if (c->preconditioner == TLO_PREC_JAC_DIAG) {
+        const double *r = c->f2[TLO_R], *Mi = c->f2[TLO_MI];
+        double *z = c->f2[TLO_Z];
+#pragma omp parallel for schedule(static)
+        for (int k = hd; k < c->y_max + hd; k++)
+            for (int j = hd; j < c->x_max + hd; j++) {
+                const long i = IDX(c, j, k);
+                z[i] = r[i] * Mi[i];
+            }
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* CG: tea_leaf_kernel_ocl.cpp:167-212, tea_leaf_cg_cl.cl:8-160              */
 /* ------------------------------------------------------------------------ */
@@ -1201,6 +1221,7 @@ int tlo_parse_deck(const char *path, tlo_config *cfg)
             else if (!strcmp(w, "tl_ppcg_inner_steps")) IVAL(cfg->tl_ppcg_inner_steps);
             else if (!strcmp(w, "tl_ch_cg_epslim")) RVAL(cfg->tl_ch_cg_epslim);
             else if (!strcmp(w, "tl_check_result")) cfg->tl_check_result = 1;
+            else if (!strcmp(w, "tl_ppcg_fresh_z")) cfg->ppcg_fresh_z = 1;
             else if (!strcmp(w, "tl_preconditioner_type")) {
                 const char *v;
                 while ((v = get_word(&dc, 0))) {
@@ -1412,6 +1433,7 @@ static int tea_leaf_solve(tlo_chunk *c, const tlo_config *cfg, solve_save *sv, d
                     }
                 }
                 ppcg_inner_iters += cfg->tl_ppcg_inner_steps;
+                if (cfg->ppcg_fresh_z) tlo_precondition_r(c);
                 rrn = tlo_calc_2norm(c, 2);
                 beta = rrn / rro;
                 tlo_cg_calc_p(c, beta);

@@ -66,6 +66,11 @@ typedef struct {
     int verbose_on, profiler_on;
     int halo_depth;           /* Fortran halo_depth keyword */
     int test_problem;
+    /* tl_ppcg_fresh_z (an option of this library, not a reference keyword):
+     * PPCG forms r.z with z = M^-1 r of the final r of the inner steps, as the
+     * Fortran TeaLeaf's calc_zrnorm does, instead of the OpenCL reference's
+     * stale z (tea_leaf_ppcg_cl.cl:52,145 never write z) */
+    int ppcg_fresh_z;
     int nstates;
     double state_density[TLO_MAX_STATES], state_energy[TLO_MAX_STATES];
     double state_xmin[TLO_MAX_STATES], state_xmax[TLO_MAX_STATES];
@@ -117,6 +122,9 @@ double tlo_jacobi_solve(tlo_chunk *c);
 /* single building blocks exposed for invariant tests */
 void tlo_block_init(tlo_chunk *c);
 void tlo_block_solve(tlo_chunk *c);
+/* z = M^-1 r over the interior with the chunk's preconditioner (the
+ * fresh-z PPCG option) */
+void tlo_precondition_r(tlo_chunk *c);
 void tlo_init_jac_diag(tlo_chunk *c);
 
 /* ---- host-side scalar routines (tea_leaf_cheby.f90) ---- */

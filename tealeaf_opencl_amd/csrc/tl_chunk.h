@@ -63,6 +63,9 @@ struct tl_chunk {
     // PPCG inner steps per fused launch in the native driver (1 or 2;
     // tl_set_tuning TL_TUNE_PPCG_STEPS_PER_LAUNCH)
     int ppcg_steps_per_launch = 2;
+    // PPCG forms r.z with z = M^-1 r of the final r of the inner steps (deck
+    // keyword tl_ppcg_fresh_z) instead of the reference's stale z
+    bool ppcg_fresh_z = false;
     // cp, bfp hold the block-Jacobi factors of the current Kx, Ky (set by
     // tl_block_init, cleared by whatever rewrites Kx, Ky, cp or bfp): the
     // drop-in kernels then recompute the factors in registers (bitwise the
@@ -116,6 +119,10 @@ void prof_end(tl_chunk *c, const char *name);
 int ensure_coef_capacity(tl_chunk *c, int n);
 int ensure_hist_capacity(tl_chunk *c, int n);
 int read_result(tl_chunk *c, double *out, int nq);
+// z = M^-1 r over the interior with the chunk's preconditioner (stored block
+// factors or Mi; nothing without one): the fresh-z PPCG option in the
+// reference call sequence
+int precondition_r(tl_chunk *c);
 // frees the refined-solve workspaces (tl_refine.hip)
 void refine_release(tl_chunk *c);
 

@@ -592,6 +592,16 @@ extern "C" int tl_block_solve(tl_chunk *c)
     return TL_OK;
 }
 
+int tl::precondition_r(tl_chunk *c)
+{
+    if (c->prec == PREC_BLOCK) return tl_block_solve(c);
+    if (c->prec == PREC_DIAG) {
+        launch_diag_apply(c->s, c->g, ext_interior(c), c->cap, c->f2[F_R], c->f2[F_MI], c->f2[F_Z]);
+        TL_CHECK_LAUNCH(c, "diag_apply");
+    }
+    return TL_OK;
+}
+
 extern "C" int tl_init_jac_diag(tl_chunk *c)
 {
     const int hd = c->g.hd;

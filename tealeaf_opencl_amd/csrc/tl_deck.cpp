@@ -256,6 +256,7 @@ int parse_deck(const std::string &path, Config *cfg, std::string *err)
             else if (k == "tl_mixed_precision") cfg->refine = 2;
             else if (k == "tl_fp64_refined") cfg->refine = 1;
             else if (k == "tl_refine_delta") ok = getr(&cfg->refine_delta);
+            else if (k == "tl_ppcg_fresh_z") cfg->ppcg_fresh_z = 1;
             else if (k == "test_problem") ok = geti(&cfg->test_problem);
             else if (k == "state") {
                 int sn = 0;

@@ -41,6 +41,9 @@ struct Config {
     // the default of the precision)
     int refine = 0;
     double refine_delta = -1.0;
+    // tl_ppcg_fresh_z (library keyword): PPCG's r.z with z = M^-1 r of the
+    // final r of the inner steps, as the Fortran TeaLeaf forms it
+    int ppcg_fresh_z = 0;
     int nstates = 0;
     std::vector<double> density, energy, sxmin, sxmax, symin, symax, radius;
     std::vector<int> geometry;

@@ -260,6 +260,7 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
                     }
                 }
                 ppcg_inner_iters += cfg.tl_ppcg_inner_steps;
+                if (cfg.ppcg_fresh_z) RC(precondition_r(c));
                 RC(tl_calc_2norm(c, 2, &rrn));
                 beta = rrn / rro;
                 RC(tl_cg_calc_p(c, beta));
@@ -467,7 +468,8 @@ int ppcg_graph(tl_chunk *c, int inner)
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
     const int per = c->ppcg_steps_per_launch;
     const int pairs = per == 2 ? inner / 2 : 0;
-    return capture(c, c->g_ppcg, (c->prec * 100000 + inner) * 4 + per, [&]() {
+    const int last = c->ppcg_fresh_z ? 2 : 1;  // the last step's r.z: fresh or stale z
+    return capture(c, c->g_ppcg, ((c->prec * 100000 + inner) * 4 + per) * 2 + (last - 1), [&]() {
         for (int b = 0; b < B; ++b) {
             launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
                               red_args(c, RED_PPCG_PW, true));
@@ -478,16 +480,16 @@ int ppcg_graph(tl_chunk *c, int inner)
             double *sd_cur = c->f2[F_SD], *sd_oth = c->f2[F_SD2];
             int i = 1;
             for (int k = 0; k < pairs; ++k, i += 2) {
-                launch_fused_ppcg_inner2(c->s, c->g, in, c->cap, c->prec, i + 1 == inner, c->f2[F_U], r_cur, r_oth,
-                                         sd_cur, sd_oth, c->f2[F_KX], c->f2[F_KY], c->f2[F_Z], c->ch_a, c->ch_b,
-                                         i - 1, red_args(c, RED_PPCG_RRN, true));
+                launch_fused_ppcg_inner2(c->s, c->g, in, c->cap, c->prec, i + 1 == inner ? last : 0, c->f2[F_U],
+                                         r_cur, r_oth, sd_cur, sd_oth, c->f2[F_KX], c->f2[F_KY], c->f2[F_Z],
+                                         c->ch_a, c->ch_b, i - 1, red_args(c, RED_PPCG_RRN, true));
                 std::swap(r_cur, r_oth);
                 std::swap(sd_cur, sd_oth);
             }
             for (; i <= inner; ++i) {
-                launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner, c->f2[F_U], r_cur, sd_cur,
-                                        sd_oth, c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
-                                        red_args(c, RED_PPCG_RRN, true));
+                launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner ? last : 0, c->f2[F_U], r_cur,
+                                        sd_cur, sd_oth, c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b,
+                                        i - 1, red_args(c, RED_PPCG_RRN, true));
                 std::swap(sd_cur, sd_oth);
             }
         }
@@ -785,6 +787,7 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
     tl_chunk *c = nullptr;
     RC(tl_chunk_create(cfg.x_cells, cfg.y_cells, cfg.sc.halo_depth, cfg.sc.preconditioner, cfg.sc.profiler_on, &c));
     c->solver = cfg.sc.solver;
+    c->ppcg_fresh_z = cfg.ppcg_fresh_z != 0;
     c->timing = (warmup >= 0);
     int rc = start(c, cfg);
     if (rc == TL_OK) emit_summary(lg, c, 0, 0.0, false, cfg, nullptr);

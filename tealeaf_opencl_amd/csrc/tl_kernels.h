@@ -35,6 +35,9 @@ void launch_init_u(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
                    double *u, double *u0);
 void launch_init_jac_diag(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Mi, const double *Kx,
                           const double *Ky);
+// z = Mi r over e
+void launch_diag_apply(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *r, const double *Mi,
+                       double *z);
 void launch_calc_residual(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *u,
                           const double *u0, double *r, const double *Kx, const double *Ky);
 void launch_dot(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *a, const double *b,
@@ -92,15 +95,18 @@ void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double
                      const SolveState *st);
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
                          const double *u0, double *uA, double *uB, const RedArgs &ra);
+// last: 0 a smoothing step, 1 the last one forming r.z with the outer step's
+// z (the reference's stale z), 2 the last one storing the fresh z = M^-1 r and
+// forming r.z with it (the fresh-z option)
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
-                             const double *W, const Faces &fc, const double *z, const double *ch_a,
+                             const double *W, const Faces &fc, double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra);
 // two PPCG inner steps (`step`, `step`+1) per launch; r and sd are read from
 // one buffer and written to another
 void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last, double *u,
                               const double *r_in, double *r_out, const double *sd_in, double *sd_out,
-                              const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                              const double *Kx, const double *Ky, double *z, const double *ch_a,
                               const double *ch_b, int step, const RedArgs &ra);
 // two Chebyshev iterations (n odd) per launch; u and p are read from one
 // buffer and written to the other

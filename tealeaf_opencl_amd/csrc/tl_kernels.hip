@@ -425,6 +425,19 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_u(Geom g, Ext e, const d
 }
 
 // tea_leaf_init_jac_diag (tea_leaf_common_cl.cl:127-142), launch bounds (-hd, hd)
+// z = Mi r (the z that tea_leaf_cg_cl.cl:8-41 forms with the diagonal
+// preconditioner), for the fresh-z PPCG option
+__global__ void __launch_bounds__(BLOCK_THREADS) k_diag_apply(Geom g, Ext e, const double *r, const double *Mi,
+                                                              double *z)
+{
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q)
+        if (ks + q <= e.k1) z[i0 + q * P] = r[i0 + q * P] * Mi[i0 + q * P];
+    TL_TILE_LOOP_END
+}
+
 __global__ void __launch_bounds__(BLOCK_THREADS) k_init_jac_diag(Geom g, Ext e, double *Mi,
                                                                  const double *Kx, const double *Ky)
 {
@@ -979,13 +992,14 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_cg_u_fix(Geom g, Ext e, doubl
 }
 
 // One PPCG smoothing step on a strip once A sd is known (res = A sd, sc = sd):
-// r -= A sd, then sd' = ca sd + cb M^-1 r (tea_leaf_ppcg_cl.cl:62-155). Shared
-// by the one-step and two-step kernels, with the FMA placed explicitly, so
-// that both round identically.
+// r -= A sd, then sd' = ca sd + cb M^-1 r (tea_leaf_ppcg_cl.cl:62-155); zf
+// gets M^-1 r of the new r (r itself without a preconditioner). Shared by the
+// one-step and two-step kernels, with the FMA placed explicitly, so that both
+// round identically.
 template <int PREC>
 __device__ __forceinline__ void ppcg_step(const Geom &g, int ks, int k1, const double *kx0, const double *kx1,
                                           const double *ky, const double *res, const double *sc, double ca,
-                                          double cb, double *rv, double *sdn)
+                                          double cb, double *rv, double *sdn, double *zf)
 {
     double zv[STRIP];
 #pragma unroll
@@ -994,19 +1008,38 @@ __device__ __forceinline__ void ppcg_step(const Geom &g, int ks, int k1, const d
     if (PREC == PREC_BLOCK) strip_block(g, ks, 0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
-        if (PREC == PREC_BLOCK) sdn[q] = fma(ca, sc[q], cb * zv[q]);
-        else if (PREC == PREC_DIAG) sdn[q] = fma(ca, sc[q], cb * rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])));
-        else sdn[q] = fma(ca, sc[q], cb * rv[q]);
+        if (PREC == PREC_BLOCK) {
+            sdn[q] = fma(ca, sc[q], cb * zv[q]);
+            zf[q] = zv[q];
+        } else if (PREC == PREC_DIAG) {
+            const double mi = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
+            sdn[q] = fma(ca, sc[q], cb * rv[q] * mi);
+            zf[q] = rv[q] * mi;
+        } else {
+            sdn[q] = fma(ca, sc[q], cb * rv[q]);
+            zf[q] = rv[q];
+        }
     }
 }
 
+// r.z of the last smoothing step: with the outer step's z (LAST = 1, the
+// reference's stale z: tea_leaf_ppcg.f90:41-60 via calc_2norm(2) reads a z
+// that calc_sd never writes), or with the fresh M^-1 r of this step, which is
+// then stored as z for the next outer step (LAST = 2, the fresh-z option);
+// r.r without a preconditioner
+template <int PREC, int LAST>
+__device__ __forceinline__ double ppcg_rz(double rv, double zo, double zf)
+{
+    return rv * (PREC == PREC_NONE ? rv : LAST == 2 ? zf : zo);
+}
+
 // PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
-// sd is double-buffered. LAST=1 also forms r.z with the z of the outer step
-// (tea_leaf_ppcg.f90:41-60 via calc_2norm(2)); its finisher runs the tail.
+// sd is double-buffered. LAST = 1, 2 also form r.z (ppcg_rz); the finisher
+// runs the tail.
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *W, Faces fc,
-    const double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
+    double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
@@ -1015,13 +1048,13 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], sdn[STRIP],
-        zo[STRIP];
+        zo[STRIP] = {}, zf[STRIP];
     strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
     col4_nt(u, i0, P, uv);
     col4_nt(r, i0, P, rv);
-    if (LAST && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
+    if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
     strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
-    ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn);
+    ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn, zf);
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (ks + q > e.k1) continue;
@@ -1029,7 +1062,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
         store_nt(u + i, uv[q] + sc[q]);
         store_nt(r + i, rv[q]);
         store_nt(sd_out + i, sdn[q]);
-        if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+        if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
+        if (LAST) acc[0] += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
     }
     TL_TILE_LOOP_END
     if (LAST) block_partial<1>(acc, ra.partials);
@@ -1064,7 +1098,7 @@ __device__ __forceinline__ double ring_outside(const Geom &g, const Ext &e, cons
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
     Geom g, Ext e, double *u, const double *r_in, double *r_out, const double *sd_in, double *sd_out,
-    const double *Kx, const double *Ky, const double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
+    const double *Kx, const double *Ky, double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
@@ -1101,11 +1135,11 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
                 double sdn[STRIP];
                 if (inside) {
                     const int h0 = IX(g, hj, hk);
-                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP];
+                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP], zf[STRIP];
                     strip_k(Kx, Ky, h0, P, hx0, hx1, hy);
                     col4(r_in, h0, P, hr);
                     strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
-                    ppcg_step<PREC>(g, hk, e.k1, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn);
+                    ppcg_step<PREC>(g, hk, e.k1, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn, zf);
                 }
                 if (take < STRIP) {
                     // one row of a strip above / below the tile
@@ -1121,12 +1155,12 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
         double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], uv[STRIP];
         // step 1 on the own strip
         if (act) {
-            double res[STRIP], sc[STRIP], sdn[STRIP];
+            double res[STRIP], sc[STRIP], sdn[STRIP], zf[STRIP];
             strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
             col4_nt(u, i0, P, uv);
             col4_nt(r_in, i0, P, rv);
             strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
-            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn);
+            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn, zf);
 #pragma unroll
             for (int q = 0; q < STRIP; ++q) {
                 uv[q] += sc[q];
@@ -1139,7 +1173,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
         __syncthreads();
         // step 2 on the own strip, sd from the ring
         if (act) {
-            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP];
+            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP] = {},
+                zf[STRIP];
 #pragma unroll
             for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv * STRIP + q][1 + lane];
 #pragma unroll
@@ -1148,9 +1183,9 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
                 e4[q] = ring[1 + wv * STRIP + q][lane + 2];
                 sc[q] = c6[q + 1];
             }
-            if (LAST && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
+            if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
             strip_apply(c6, w4, e4, kx0, kx1, ky, res);
-            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn);
+            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn, zf);
 #pragma unroll
             for (int q = 0; q < STRIP; ++q) {
                 if (ks + q > e.k1) continue;
@@ -1158,7 +1193,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
                 store_nt(u + i, uv[q] + sc[q]);
                 store_nt(r_out + i, rv[q]);
                 store_nt(sd_out + i, sdn[q]);
-                if (LAST) acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+                if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
+                if (LAST) acc[0] += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
             }
         }
         __syncthreads();
@@ -1469,6 +1505,12 @@ void launch_init_jac_diag(hipStream_t s, const Geom &g, const Ext &e, int cap, d
     TL_LAUNCH(k_init_jac_diag, e, cap, s, g, e, Mi, Kx, Ky);
 }
 
+void launch_diag_apply(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *r, const double *Mi,
+                       double *z)
+{
+    TL_LAUNCH(k_diag_apply, e, cap, s, g, e, r, Mi, z);
+}
+
 void launch_calc_residual(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *u,
                           const double *u0, double *r, const double *Kx, const double *Ky)
 {
@@ -1641,7 +1683,7 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
 
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
-                             const double *W, const Faces &fc, const double *z, const double *ch_a,
+                             const double *W, const Faces &fc, double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra)
 {
 #define TL_FPI(P, L)                                                                                         \
@@ -1651,7 +1693,11 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
         else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, W, fc, z, ch_a, ch_b,  \
                        step, ra);                                                                            \
     } while (0)
-    if (last) {
+    if (last == 2) {
+        if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 2);
+        else if (prec == PREC_DIAG) TL_FPI(PREC_DIAG, 2);
+        else TL_FPI(PREC_NONE, 1);
+    } else if (last) {
         if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 1);
         else if (prec == PREC_DIAG) TL_FPI(PREC_DIAG, 1);
         else TL_FPI(PREC_NONE, 1);
@@ -1665,7 +1711,7 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
 
 void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last, double *u,
                               const double *r_in, double *r_out, const double *sd_in, double *sd_out,
-                              const double *Kx, const double *Ky, const double *z, const double *ch_a,
+                              const double *Kx, const double *Ky, double *z, const double *ch_a,
                               const double *ch_b, int step, const RedArgs &ra)
 {
 #define TL_FPI2(P, L)                                                                                           \
@@ -1675,7 +1721,11 @@ void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int ca
         else TL_LAUNCH((k_fused_ppcg_inner2<P, L>), e, cap, s, g, e, u, r_in, r_out, sd_in, sd_out, Kx, Ky, z,   \
                        ch_a, ch_b, step, ra);                                                                   \
     } while (0)
-    if (last) {
+    if (last == 2) {
+        if (prec == PREC_BLOCK) TL_FPI2(PREC_BLOCK, 2);
+        else if (prec == PREC_DIAG) TL_FPI2(PREC_DIAG, 2);
+        else TL_FPI2(PREC_NONE, 1);
+    } else if (last) {
         if (prec == PREC_BLOCK) TL_FPI2(PREC_BLOCK, 1);
         else if (prec == PREC_DIAG) TL_FPI2(PREC_DIAG, 1);
         else TL_FPI2(PREC_NONE, 1);

@@ -93,8 +93,29 @@ def test_solvers_small(hiplib, oracle, tmp_path, solver, prec, mode):
     _compare_solves(lg, lo, 2)
 
 
+@pytest.mark.parametrize("prec", ["jac_diag", "jac_block"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ppcg_fresh_z(hiplib, oracle, tmp_path, prec, mode):
+    """The fresh-z PPCG option (deck keyword tl_ppcg_fresh_z: r.z formed with
+    z = M^-1 r of the final r of the inner steps, as the Fortran TeaLeaf's
+    calc_zrnorm does, instead of the reference's stale z) against the
+    oracle's, in the reference call sequence and in the fused driver (whose
+    last smoothing step stores the fresh z)."""
+    extra = ["tl_ppcg_inner_steps=10", "tl_ppcg_fresh_z"]
+    deck = decks.write_deck(str(tmp_path / "s.in"), 200, 148, decks.seeded_states(2, 2, True), solver="tl_use_ppcg",
+                            preconditioner=prec, end_step=3, eps=1e-12, max_iters=4000, extra=extra)
+    lg, lo = _run_both(hiplib, oracle, deck, tmp_path, mode)
+    _compare_summaries(lg, lo)
+    _compare_solves(lg, lo, 2)
+    # the option changes the solve: the stale-z run takes other iteration counts
+    stale = oracle.run_deck(decks.write_deck(str(tmp_path / "t.in"), 200, 148, decks.seeded_states(2, 2, True),
+                                             solver="tl_use_ppcg", preconditioner=prec, end_step=3, eps=1e-12,
+                                             max_iters=4000, extra=extra[:1]), tmp_path / "t.log")
+    assert [r[1] for r in stale["solve"]] != [r[1] for r in lo["solve"]]
+
+
 @pytest.mark.parametrize("case", ["none-3", "jac_diag-4", "jac_block-10", "jac_block-7-reflective",
-                                  "jac_diag-5-halo2"])
+                                  "jac_diag-5-halo2", "jac_block-10-fresh", "jac_diag-7-fresh"])
 def test_ppcg_two_steps_per_launch_bitwise(hiplib, tmp_path, case):
     """The native PPCG with two inner steps per launch (ring of the tile
     recomputed, r and sd flipped between buffers) against one step per launch:
@@ -110,6 +131,8 @@ def test_ppcg_two_steps_per_launch_bitwise(hiplib, tmp_path, case):
         kw["coefficient"] = "tl_coefficient_inverse_density"
     if "halo2" in parts:
         kw["halo_depth"] = 2
+    if "fresh" in parts:
+        extra.append("tl_ppcg_fresh_z")
     deck = decks.write_deck(str(tmp_path / "p.in"), 200, 148, decks.seeded_states(2, 2, True), solver="tl_use_ppcg",
                             preconditioner=prec, end_step=2, eps=1e-12, max_iters=2000, extra=extra, **kw)
     try:

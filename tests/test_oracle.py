@@ -146,6 +146,7 @@ def test_jacobi_matches_dense_solve(oracle):
 @pytest.mark.parametrize("solver,extra", [
     ("tl_use_chebyshev", ["tl_ch_cg_presteps=8", "tl_ch_cg_epslim=1.0e300"]),
     ("tl_use_ppcg", ["tl_ppcg_inner_steps=6"]),
+    ("tl_use_ppcg", ["tl_ppcg_inner_steps=6", "tl_ppcg_fresh_z"]),
     ("tl_use_cg", []),
 ])
 @pytest.mark.parametrize("prec", ["none", "jac_diag", "jac_block"])
