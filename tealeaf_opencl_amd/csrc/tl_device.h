@@ -73,11 +73,45 @@ __device__ __forceinline__ double rcp_fp32(double x)
     return (double)__builtin_amdgcn_rcpf((float)x);
 }
 
+// a / b for the operands of the iteration kernels: divisors that are positive
+// normal numbers of moderate exponent (face denominators 2 w_a w_b of
+// positive conductivities, Thomas pivots and diagonals >= 1) and finite
+// dividends. The hardware reciprocal, two Newton steps and one residual
+// correction (Markstein's method): the operations of the compiler's IEEE FP64
+// division less its range scaling (v_div_scale / v_div_fmas) and special-value
+// fix-up (v_div_fixup), which leave such operands unchanged, so the quotient
+// is the correctly rounded one bit for bit, in 8 instructions instead of 11.
+// The recomputed faces and block factors built on it are compared bitwise
+// with the stored ones, which tea_leaf_init_common / tea_leaf_block_init form
+// with the plain division.
+__device__ __forceinline__ double div_nr(double a, double b)
+{
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    const double q = a * y;
+    const double r = fma(-b, q, a);
+    return fma(r, y, q);
+}
+
+// divisions of the block-Jacobi factors and solve: the plain IEEE division,
+// div_nr (bitwise the same for these operands), or the FP32 reciprocal of
+// the FP32-inner solve (rcp_fp32)
+enum { DIV_IEEE = 0, DIV_NR = 1, DIV_RCP32 = 2 };
+
+template <int DIV>
+__device__ __forceinline__ double quot(double a, double b)
+{
+    return DIV == DIV_NR ? div_nr(a, b) : a / b;
+}
+
 // block_solve_func (tea_block_jacobi.cl:6-42) on one strip held in registers.
 // b[k] = COEF_B, a[k] = COEF_A of strip row k; cp/bfp as produced by
-// tea_leaf_block_init; rows >= upper are left at zero. FAST: bfp[0] holds
-// 1/b[0] (strip_factors<true>), used in place of the division.
-template <bool FAST = false>
+// tea_leaf_block_init; rows >= upper are left at zero. DIV_RCP32: bfp[0]
+// holds 1/b[0] (strip_factors<DIV_RCP32>), used in place of the division.
+template <int DIV = DIV_IEEE>
 __device__ __forceinline__ void strip_solve(int upper, const double *r, const double *b,
                                             const double *a, const double *cp,
                                             const double *bfp, double *z)
@@ -86,7 +120,7 @@ __device__ __forceinline__ void strip_solve(int upper, const double *r, const do
 #pragma unroll
     for (int k = 0; k < STRIP; ++k) z[k] = 0.0;
     if (upper <= 0) return;
-    dp[0] = FAST ? r[0] * bfp[0] : r[0] / b[0];
+    dp[0] = DIV == DIV_RCP32 ? r[0] * bfp[0] : quot<DIV>(r[0], b[0]);
 #pragma unroll
     for (int k = 1; k < STRIP; ++k)
         if (k < upper) dp[k] = (r[k] - a[k] * dp[k - 1]) * bfp[k];
@@ -100,26 +134,26 @@ __device__ __forceinline__ void strip_solve(int upper, const double *r, const do
 }
 
 // Thomas factors of tea_leaf_block_init (tea_block_jacobi.cl:78-105).
-// FAST: the pivots' reciprocals from rcp_fp32 (no FP64 division) and
+// DIV_RCP32: the pivots' reciprocals from rcp_fp32 (no FP64 division) and
 // bfp[0] = 1/b[0]. The factorisation stays exact for the matrix it defines:
 // its off-diagonals are (1/bfp[k]) * cp[k] = c[k] and a[k] as before (still
 // symmetric), its diagonal differs from b[k] by the reciprocal's error times
 // the pivot, about 1e-7 relative.
-template <bool FAST = false>
+template <int DIV = DIV_IEEE>
 __device__ __forceinline__ void strip_factors(int upper, const double *b, const double *a,
                                               const double *c, double *cp, double *bfp)
 {
-    if (FAST) {
+    if (DIV == DIV_RCP32) {
         bfp[0] = rcp_fp32(b[0]);
         cp[0] = c[0] * bfp[0];
     } else {
-        cp[0] = c[0] / b[0];
+        cp[0] = quot<DIV>(c[0], b[0]);
         bfp[0] = 0.0;
     }
 #pragma unroll
     for (int k = 1; k < STRIP; ++k) {
         if (k < upper) {
-            bfp[k] = FAST ? rcp_fp32(b[k] - a[k] * cp[k - 1]) : 1.0 / (b[k] - a[k] * cp[k - 1]);
+            bfp[k] = DIV == DIV_RCP32 ? rcp_fp32(b[k] - a[k] * cp[k - 1]) : quot<DIV>(1.0, b[k] - a[k] * cp[k - 1]);
             cp[k] = c[k] * bfp[k];
         } else {
             bfp[k] = 0.0;
@@ -265,6 +299,15 @@ __device__ __forceinline__ double face_coef(double wa, double wb, double scale)
     return v;
 }
 
+// the same for two positive conductivities (faces strictly inside the
+// chunk), with div_nr: bitwise face_coef
+__device__ __forceinline__ double face_coef_nr(double wa, double wb, double scale)
+{
+    double v = div_nr(wa + wb, 2.0 * wa * wb);
+    v *= scale;
+    return v;
+}
+
 // The same face from h = 1/(2w) stored per cell: scale * (h_a + h_b), equal
 // to face_coef in exact arithmetic, with no division (the CG kernels of the
 // FP32-inner solve, whose operator is an FP32-rounded one anyway).
@@ -317,12 +360,12 @@ __device__ __forceinline__ void strip_k_cond(const Geom &g, const Faces &f, cons
     if (inner) {
 #pragma unroll
         for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = HALF_INV ? face_coef_h(ww[q], wc[q + 1], f.rx) : face_coef(ww[q], wc[q + 1], f.rx);
-            kx1[q] = HALF_INV ? face_coef_h(wc[q + 1], we[q], f.rx) : face_coef(wc[q + 1], we[q], f.rx);
+            kx0[q] = HALF_INV ? face_coef_h(ww[q], wc[q + 1], f.rx) : face_coef_nr(ww[q], wc[q + 1], f.rx);
+            kx1[q] = HALF_INV ? face_coef_h(wc[q + 1], we[q], f.rx) : face_coef_nr(wc[q + 1], we[q], f.rx);
         }
 #pragma unroll
         for (int q = 0; q < STRIP + 1; ++q)
-            ky[q] = HALF_INV ? face_coef_h(wc[q], wc[q + 1], f.ry) : face_coef(wc[q], wc[q + 1], f.ry);
+            ky[q] = HALF_INV ? face_coef_h(wc[q], wc[q + 1], f.ry) : face_coef_nr(wc[q], wc[q + 1], f.ry);
         return;
     }
 #pragma unroll
@@ -383,9 +426,9 @@ __device__ __forceinline__ void strip_block(const Geom &g, int ks, int i0, const
         col4(cp, i0, g.pitch, cpv);
         col4(bfp, i0, g.pitch, bfv);
     } else {
-        strip_factors(upper, b, a, c, cpv, bfv);
+        strip_factors<DIV_NR>(upper, b, a, c, cpv, bfv);
     }
-    strip_solve(upper, rv, b, a, cpv, bfv, zv);
+    strip_solve<DIV_NR>(upper, rv, b, a, cpv, bfv, zv);
 }
 
 // the strip's right-hand side with rows past k1 zeroed (the reference's
@@ -409,8 +452,8 @@ __device__ __forceinline__ void strip_precondition(const Geom &g, int ks, int i0
             double a[STRIP], b[STRIP], c[STRIP], cpv[STRIP], bfv[STRIP];
             strip_abc(kx0, kx1, ky, a, b, c);
             const int upper = block_top(g, ks);
-            strip_factors<true>(upper, b, a, c, cpv, bfv);
-            strip_solve<true>(upper, rv, b, a, cpv, bfv, zv);
+            strip_factors<DIV_RCP32>(upper, b, a, c, cpv, bfv);
+            strip_solve<DIV_RCP32>(upper, rv, b, a, cpv, bfv, zv);
         } else {
             strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
         }
@@ -418,7 +461,7 @@ __device__ __forceinline__ void strip_precondition(const Geom &g, int ks, int i0
 #pragma unroll
         for (int q = 0; q < STRIP; ++q) {
             const double dg = diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
-            zv[q] = rv[q] * (FAST ? rcp_fp32(dg) : 1.0 / dg);
+            zv[q] = rv[q] * (FAST ? rcp_fp32(dg) : div_nr(1.0, dg));
         }
     } else {
 #pragma unroll

@@ -968,7 +968,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
         store_nt(r_out + i, rv[q]);
         double zz = rv[q];
         if (PREC == PREC_BLOCK) zz = zv[q];
-        if (PREC == PREC_DIAG) zz = rv[q] * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+        if (PREC == PREC_DIAG) zz = rv[q] * div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
         if (PREC != PREC_NONE) store_nt(z + i, zz);
         if (PPCG) store_nt(sd + i, zz / theta);
         else acc[0] += rv[q] * zz;
@@ -1012,7 +1012,7 @@ __device__ __forceinline__ void ppcg_step(const Geom &g, int ks, int k1, const d
             sdn[q] = fma(ca, sc[q], cb * zv[q]);
             zf[q] = zv[q];
         } else if (PREC == PREC_DIAG) {
-            const double mi = 1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]);
+            const double mi = div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
             sdn[q] = fma(ca, sc[q], cb * rv[q] * mi);
             zf[q] = rv[q] * mi;
         } else {
@@ -1220,7 +1220,8 @@ __device__ __forceinline__ void cheby_step(const Geom &g, int ks, int k1, const 
 #pragma unroll
     for (int q = 0; q < STRIP; ++q) {
         if (PREC == PREC_BLOCK) pn[q] = fma(ca, pv[q], cb * zv[q]);
-        else if (PREC == PREC_DIAG) pn[q] = fma(ca, pv[q], cb * (1.0 / diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])) * rv[q]);
+        else if (PREC == PREC_DIAG)
+            pn[q] = fma(ca, pv[q], cb * div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1])) * rv[q]);
         else pn[q] = fma(ca, pv[q], cb * rv[q]);
         un[q] = uc[q] + pn[q];
     }
