@@ -175,6 +175,11 @@ int tl_print_profiling(tl_chunk *c);
  * copy (kind 0, 16 B/cell) or of a 4-read/3-write stream with the mix of the
  * CG update kernel (kind 1, 56 B/cell), over temporary fields. */
 int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
+/* Self-check of the division the iteration kernels use for faces, block
+ * factors and diagonals (no reference counterpart): for i < n, quot[i] =
+ * num[i] / den[i] by the compiler's IEEE FP64 division and fast[i] by the
+ * kernels' reciprocal-Newton sequence, both on the GPU; host arrays. */
+int tl_check_division(const double *num, const double *den, double *quot, double *fast, long n);
 
 /* Tuning of the native driver's fused kernels. No reference counterpart.
  * TL_TUNE_PPCG_STEPS_PER_LAUNCH: PPCG inner steps per fused launch, 1 or 2

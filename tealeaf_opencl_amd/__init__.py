@@ -114,6 +114,7 @@ _SIGS = {
     "tl_solve_refined": (_I, [_P, _I, _D, _I, _D, ctypes.POINTER(RefineResult)]),
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
     "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
+    "tl_check_division": (_I, [_DP, _DP, _DP, _DP, ctypes.c_long]),
     "tl_set_tuning": (_I, [_P, _I, _I]),
     "tl_ppcg_inner_fused": (_I, [_P, _I, _I]),
     "tl_set_progress": (_I, [_P, _P, _P]),
@@ -441,6 +442,20 @@ def run_deck(deck, log_path, mode=MODE_NATIVE, max_steps=0, tea_out=None):
                                str(tea_out).encode() if tea_out else None, int(mode), int(max_steps)),
            "tl_run_deck")
     return parse_log(log_path)
+
+
+def check_division(num, den):
+    """tl_check_division: (IEEE quotient, the kernels' reciprocal-Newton
+    quotient) of two float64 arrays, both computed on the GPU."""
+    import numpy as np
+    lib = load_library()
+    num = np.ascontiguousarray(num, dtype=np.float64)
+    den = np.ascontiguousarray(den, dtype=np.float64)
+    if num.shape != den.shape:
+        raise ValueError("num and den differ in shape")
+    quot, fast = np.empty_like(num), np.empty_like(num)
+    _check(lib.tl_check_division(_dptr(num), _dptr(den), _dptr(quot), _dptr(fast), num.size), "tl_check_division")
+    return quot, fast
 
 
 def set_tuning(key, value, chunk=None):

@@ -833,6 +833,30 @@ extern "C" int tl_jacobi_solve(tl_chunk *c, double *error)
 // copy (kind 0: 8 B/cell read + 8 B/cell written) or of the 4-read/3-write
 // mix of the CG update kernel (kind 1: 56 B/cell) over the interior, on
 // temporary fields, timed with events on the chunk's stream.
+extern "C" int tl_check_division(const double *num, const double *den, double *quot, double *fast, long n)
+{
+    if (!num || !den || !quot || !fast || n < 1 || n > (1L << 26)) {
+        set_error("tl_check_division: null array or n %ld outside 1..2^26", n);
+        return TL_ERR_ARG;
+    }
+    const size_t bytes = sizeof(double) * (size_t)n;
+    double *d[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipError_t e = hipSuccess;
+    for (double *&p : d)
+        if (e == hipSuccess) e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) e = hipMemcpy(d[0], num, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d[1], den, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        launch_check_division(nullptr, d[0], d[1], d[2], d[3], n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(quot, d[2], bytes, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(fast, d[3], bytes, hipMemcpyDeviceToHost);
+    for (double *p : d)
+        if (p) hipFree(p);
+    return e == hipSuccess ? TL_OK : hip_fail("tl_check_division", e);
+}
+
 extern "C" int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s)
 {
     if (!c || (kind != 0 && kind != 1) || reps < 1 || !bytes_per_s) {

@@ -35,6 +35,8 @@ void launch_init_u(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
                    double *u, double *u0);
 void launch_init_jac_diag(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Mi, const double *Kx,
                           const double *Ky);
+// quot = num / den (IEEE) and fast = div_nr(num, den) for i < n
+void launch_check_division(hipStream_t s, const double *num, const double *den, double *quot, double *fast, long n);
 // z = Mi r over e
 void launch_diag_apply(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *r, const double *Mi,
                        double *z);

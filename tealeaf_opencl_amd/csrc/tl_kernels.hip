@@ -1506,6 +1506,21 @@ void launch_init_jac_diag(hipStream_t s, const Geom &g, const Ext &e, int cap, d
     TL_LAUNCH(k_init_jac_diag, e, cap, s, g, e, Mi, Kx, Ky);
 }
 
+// num / den by the compiler's division and by div_nr (tl_check_division)
+__global__ void __launch_bounds__(256) k_check_division(const double *num, const double *den, double *quot,
+                                                        double *fast, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        quot[i] = num[i] / den[i];
+        fast[i] = div_nr(num[i], den[i]);
+    }
+}
+
+void launch_check_division(hipStream_t s, const double *num, const double *den, double *quot, double *fast, long n)
+{
+    hipLaunchKernelGGL(k_check_division, dim3(1024), dim3(256), 0, s, num, den, quot, fast, n);
+}
+
 void launch_diag_apply(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *r, const double *Mi,
                        double *z)
 {
