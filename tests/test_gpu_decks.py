@@ -93,9 +93,9 @@ def test_solvers_small(hiplib, oracle, tmp_path, solver, prec, mode):
     _compare_solves(lg, lo, 2)
 
 
-@pytest.mark.parametrize("prec", ["jac_diag", "jac_block"])
+@pytest.mark.parametrize("prec,hd", [("jac_diag", 1), ("jac_block", 1), ("jac_diag", 2)])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_ppcg_fresh_z(hiplib, oracle, tmp_path, prec, mode):
+def test_ppcg_fresh_z(hiplib, oracle, tmp_path, prec, hd, mode):
     """The fresh-z PPCG option (deck keyword tl_ppcg_fresh_z: r.z formed with
     z = M^-1 r of the final r of the inner steps, as the Fortran TeaLeaf's
     calc_zrnorm does, instead of the reference's stale z) against the
@@ -103,14 +103,14 @@ def test_ppcg_fresh_z(hiplib, oracle, tmp_path, prec, mode):
     last smoothing step stores the fresh z)."""
     extra = ["tl_ppcg_inner_steps=10", "tl_ppcg_fresh_z"]
     deck = decks.write_deck(str(tmp_path / "s.in"), 200, 148, decks.seeded_states(2, 2, True), solver="tl_use_ppcg",
-                            preconditioner=prec, end_step=3, eps=1e-12, max_iters=4000, extra=extra)
+                            preconditioner=prec, end_step=3, eps=1e-12, max_iters=4000, extra=extra, halo_depth=hd)
     lg, lo = _run_both(hiplib, oracle, deck, tmp_path, mode)
     _compare_summaries(lg, lo)
     _compare_solves(lg, lo, 2)
     # the option changes the solve: the stale-z run takes other iteration counts
     stale = oracle.run_deck(decks.write_deck(str(tmp_path / "t.in"), 200, 148, decks.seeded_states(2, 2, True),
                                              solver="tl_use_ppcg", preconditioner=prec, end_step=3, eps=1e-12,
-                                             max_iters=4000, extra=extra[:1]), tmp_path / "t.log")
+                                             max_iters=4000, extra=extra[:1], halo_depth=hd), tmp_path / "t.log")
     assert [r[1] for r in stale["solve"]] != [r[1] for r in lo["solve"]]
 
 
