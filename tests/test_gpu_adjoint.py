@@ -91,6 +91,35 @@ def test_finite_differences_config4(hiplib):
     assert worst <= 1e-6, worst
 
 
+def test_finite_differences_config4_full_size(hiplib):
+    """The adjoint workload at its own size, 1024x1024, 5 timesteps: 32
+    seeded sensitive cells, central differences with h = 1e-4 relative. At
+    this size a central difference with h = 1e-6 resolves the loss only to
+    the FP64 forward solve's rounding floor (median 6e-6, worst 8e-5 against
+    the adjoint); the error falls as 1/h down to h = 1e-4 (median 8e-8), where
+    the O(h^2) truncation is still far below 1e-6."""
+    from tealeaf_opencl_amd import adjoint
+    n, nsteps, dt, coef = 1024, 5, 0.004, 1
+    c = hiplib.Chunk(n, n, 1, 3)
+    rho, e0, u_obs = adjoint.setup_problem(c, seed=1, obs_seed=2, dt=dt, nsteps=nsteps, tol=1e-14)
+    L, grad, _ = c.adjoint_density(coef, dt, nsteps, u_obs, tol=1e-14, checkpoint_every=2)
+    cells = sensitive_cells(rho, grad, 32, 2024)
+    assert len(cells) == 32
+    errs = []
+    for (k, j) in cells:
+        h = 1e-4 * rho[k, j]
+        vals = []
+        for s in (1.0, -1.0):
+            r2 = rho.copy()
+            r2[k, j] += s * h
+            c.put_field("density", r2)
+            c.put_field("energy1", e0)
+            vals.append(c.forward_loss(coef, dt, nsteps, u_obs, tol=1e-14))
+        fd = (vals[0] - vals[1]) / (2.0 * h)
+        errs.append(abs(fd - grad[k, j]) / abs(grad[k, j]))
+    assert max(errs) <= 1e-6, (np.median(errs), max(errs))
+
+
 def test_torch_wrapper(hiplib):
     torch = pytest.importorskip("torch")
     from tealeaf_opencl_amd import adjoint
