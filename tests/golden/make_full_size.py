@@ -1,12 +1,16 @@
-"""Writes the FP64 CPU oracle's results for the 4096x4096 deck (two seeded
-rectangles and a circle, seed 1, strip-4 block Jacobi, tl_eps 1e-15,
-tl_max_iters 5000, dt 0.004) as fixtures: the field summary after every
-timestep and the solve lines (iterations, error), for each solver over the
-number of timesteps given below. The oracle (oracle/tl_oracle.c, OpenMP)
-needs minutes to an hour per solver on a few cores, so the GPU tests compare
+"""Writes the FP64 CPU oracle's results for the 4096x4096 decks as fixtures:
+the field summary after every timestep and the solve lines (iterations,
+error). "cg", "chebyshev", "ppcg": the deck with two seeded rectangles and a
+circle (seed 1, strip-4 block Jacobi, tl_eps 1e-15, tl_max_iters 5000, dt
+0.004) over the number of timesteps given below. "config3": the two seeded
+rectangles with conductivity 1/density (contrast 1e3), CG + strip-4 block
+Jacobi for one timestep to tl_eps 1e-15 (capped at 8000 iterations, past the
+FP64 rounding floor of the residual), the reference against which the refined
+solves to a true residual of 1e-10 are measured. The oracle (oracle/tl_oracle.c, OpenMP)
+needs minutes to an hour per run on a few cores, so the GPU tests compare
 against these files instead of rerunning it.
 
-Usage: python tests/golden/make_full_size.py [cg|chebyshev|ppcg ...]
+Usage: python tests/golden/make_full_size.py [cg|chebyshev|ppcg|config3 ...]
 """
 import json
 import os
@@ -20,11 +24,14 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 # solver name -> (deck keyword, timesteps)
-RUNS = {"cg": ("tl_use_cg", 20), "chebyshev": ("tl_use_chebyshev", 20), "ppcg": ("tl_use_ppcg", 2)}
+RUNS = {"cg": ("tl_use_cg", 20), "chebyshev": ("tl_use_chebyshev", 20), "ppcg": ("tl_use_ppcg", 2),
+        "config3": ("tl_use_cg", 1)}
 N = 4096
 
 
 def fixture_path(name):
+    if name == "config3":
+        return os.path.join(HERE, "config3_%d_cg.json" % N)
     return os.path.join(HERE, "config2_%d_%s.json" % (N, name))
 
 
@@ -36,7 +43,11 @@ def main(names):
     for name in names:
         solver, steps = RUNS[name]
         with tempfile.TemporaryDirectory() as d:
-            deck = decks.config2_deck(os.path.join(d, "c2.in"), solver=solver, n=N, steps=steps)
+            if name == "config3":
+                deck = decks.config3_deck(os.path.join(d, "c2.in"), precision="reference", n=N, steps=steps,
+                                          eps=1e-15, max_iters=8000)
+            else:
+                deck = decks.config2_deck(os.path.join(d, "c2.in"), solver=solver, n=N, steps=steps)
             t0 = time.perf_counter()
             lo = pyoracle.run_deck(deck, os.path.join(d, "c2.log"))
             secs = time.perf_counter() - t0

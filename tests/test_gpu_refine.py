@@ -232,3 +232,35 @@ def test_ppcg_deck_summary(hiplib, oracle, tmp_path, eps, bound):
     so, sg = np.array(lo["summary"]), np.array(lg["summary"])
     rel = np.abs(sg[:, 4:6] - so[:, 4:6]) / np.abs(so[:, 4:6])
     assert np.max(rel) < bound, rel
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_config3_full_size_vs_oracle(hiplib, tmp_path, precision):
+    """The mixed-precision workload at its own size: 4096^2, the two seeded
+    rectangles, conductivity 1/density (contrast 1e3), one timestep, the
+    refined CG to a true residual of 1e-10 (FP32 or FP64 inner vectors),
+    against the oracle's FP64 CG solution of the same deck run past the FP64
+    rounding floor (fixture tests/golden/config3_4096_cg.json, written by
+    tests/golden/make_full_size.py). Volume and mass do not depend on the
+    solve; internal energy and temperature carry its error, printed for each
+    quantity. A has eigenvalues >= 1, so |u - u*|_2 <= |u0 - A u|_2 <= 1e-10
+    |r0|_2, and r0 = u0 - A u0 is of the size of the diffusion term (rx =
+    dt/dx^2 = 671 here, face coefficients up to ~1e3 times that): the stated
+    errors sit near 1e-7 of the temperature, held here to 1e-6."""
+    import json
+    import os
+    from tealeaf_opencl_amd import decks
+    path = os.path.join(os.path.dirname(__file__), "golden", "config3_4096_cg.json")
+    if not os.path.exists(path):
+        pytest.skip("fixture config3_4096_cg.json not generated")
+    with open(path) as f:
+        fx = json.load(f)
+    deck = decks.config3_deck(str(tmp_path / "c3.in"), precision=precision, n=fx["n"], steps=fx["steps"],
+                              eps=1e-10)
+    lg = hiplib.run_deck(deck, tmp_path / "c3.log", mode=1)
+    assert len(lg["refine"]) == fx["steps"] and all(r[3] == 1 for r in lg["refine"])
+    got, want = np.array(lg["summary"][-1][1:]), np.array(fx["summary"][-1][1:])
+    rel = np.abs(got - want) / np.abs(want)
+    print("%s vs oracle, volume mass density ie temperature:" % precision, " ".join("%.2e" % v for v in rel))
+    assert rel[0] <= 1e-14 and rel[1] <= 1e-14
+    assert np.max(rel) <= 1e-6, rel
