@@ -319,8 +319,8 @@ def load_traffic(n):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs one chunk on one GPU; --gpus %d is not supported "
-              "(multi-GPU sharding is design-only)" % args.gpus, file=sys.stderr)
+        print("bench.py: the library runs one chunk on one GPU; --gpus %d is not supported "
+              "(sharding over several GPUs is not implemented)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import tealeaf_opencl_amd as tl
     from tealeaf_opencl_amd import decks

@@ -1,7 +1,7 @@
 # Links the reference's own Fortran driver (tea.f90 and the modules it uses,
 # compiled from the sources where they lie under $(REF)) against
 # libtealeaf_mi355x.so in place of the reference's OpenCL C++ objects: the
-# drop-in test of the C-ABI (the maintainer's link step of INTEGRATION.md).
+# drop-in test of the C-ABI (the link step a maintainer of the reference makes).
 # Test infrastructure only; outputs go to oracle/_ref/ and nothing of the
 # reference is copied into the repository.
 #

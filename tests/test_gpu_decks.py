@@ -3,8 +3,7 @@ residual trajectory of the HIP driver (drop-in and native modes) against the
 CPU oracle, plus the reference's own known-answer tests
 (field_summary.f90:86-106: temp within 0.001 % of the published value).
 
-The north-star bar for FP64 is 1e-11 relative on every field-summary
-quantity; the residual trajectories agree to 1e-6 relative while the residual
+The bar for FP64 is 1e-11 relative on every field-summary quantity; the residual trajectories agree to 1e-6 relative while the residual
 is far above round-off (both are the same algorithm with dot products summed
 in a different order).
 """
