@@ -175,6 +175,18 @@ int tl_print_profiling(tl_chunk *c);
  * copy (kind 0, 16 B/cell) or of a 4-read/3-write stream with the mix of the
  * CG update kernel (kind 1, 56 B/cell), over temporary fields. */
 int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
+/* The deck as tl_run_deck reads it (read_input.f90 keywords and defaults,
+ * parse.f90 line handling, the ocl_init.cpp side channel), without a GPU; no
+ * reference counterpart: the settings come back in fixed arrays, ints[23] = {x_cells, y_cells, end_step, summary_frequency,
+ * visit_frequency, max_iters, coefficient, tl_ch_cg_presteps, tl_check_result,
+ * tl_ppcg_inner_steps, reflective_boundary, solver, fortran_preconditioner,
+ * verbose_on, profiler_on, halo_depth, test_problem, ppcg_fresh_z, nstates,
+ * side-channel preconditioner, halo_depth, solver, profiler_on}, reals[8] =
+ * {xmin, ymin, xmax, ymax, end_time, dtinit, eps, tl_ch_cg_epslim}, and for
+ * each of the first max_states states states[8 s ..] = {density, energy, xmin,
+ * xmax, ymin, ymax, radius, geometry}.
+ * TL_ERR_CONFIG with tl_last_error() set when the deck is rejected. */
+int tl_parse_deck(const char *deck, int *ints, double *reals, double *states, int max_states);
 /* Self-check of the division the iteration kernels use for faces, block
  * factors and diagonals (no reference counterpart): for i < n, quot[i] =
  * num[i] / den[i] by the compiler's IEEE FP64 division and fast[i] by the

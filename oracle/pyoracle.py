@@ -57,11 +57,26 @@ _SIGS = {
     "tlo_calc_eigenvalues": (_I, [_DP, _DP, _I, _DP, _DP]),
     "tlo_calc_ch_coefs": (None, [_DP, _DP, _D, _D, _DP, _I]),
     "tlo_run_deck": (_I, [ctypes.c_char_p, ctypes.c_char_p, _I]),
+    "tlo_parse_deck_values": (_I, [ctypes.c_char_p, _IP, _DP, _DP, _I]),
     "tlo_time_cg_iterations": (_D, [ctypes.c_char_p, _I, _IP]),
     "tlo_num_threads": (_I, []),
 }
 
 _lib = None
+
+
+def parse_deck_values(deck, max_states=64):
+    """tlo_parse_deck_values: the oracle reader's settings in the arrays of the
+    library's tl_parse_deck; None when the reader rejects the deck."""
+    ints = (ctypes.c_int * 23)()
+    reals = np.zeros(8)
+    states = np.zeros((max_states, 8))
+    rc = load().tlo_parse_deck_values(str(deck).encode(), ints, reals.ctypes.data_as(_DP),
+                                      states.ctypes.data_as(_DP), int(max_states))
+    if rc:
+        return None
+    iv = list(ints)
+    return iv, reals, states[:min(iv[18], max_states)]
 
 
 def build():

@@ -1530,6 +1530,29 @@ int tlo_run_config(const tlo_config *cfg, const char *log_path, int max_steps)
     return rc;
 }
 
+int tlo_parse_deck_values(const char *path, int *ints, double *reals, double *states, int max_states)
+{
+    tlo_config cfg;
+    const int rc = tlo_parse_deck(path, &cfg);
+    if (rc) return rc;
+    const int iv[23] = {cfg.x_cells, cfg.y_cells, cfg.end_step, cfg.summary_frequency, cfg.visit_frequency,
+                        cfg.max_iters, cfg.coefficient, cfg.tl_ch_cg_presteps, cfg.tl_check_result,
+                        cfg.tl_ppcg_inner_steps, cfg.reflective_boundary, cfg.solver, cfg.fortran_preconditioner,
+                        cfg.verbose_on, cfg.profiler_on, cfg.halo_depth, cfg.test_problem, cfg.ppcg_fresh_z,
+                        cfg.nstates, cfg.sc_preconditioner, cfg.sc_halo_depth, cfg.sc_solver, cfg.sc_profiler_on};
+    const double rv[8] = {cfg.xmin, cfg.ymin, cfg.xmax, cfg.ymax, cfg.end_time, cfg.dtinit, cfg.eps,
+                          cfg.tl_ch_cg_epslim};
+    memcpy(ints, iv, sizeof iv);
+    memcpy(reals, rv, sizeof rv);
+    for (int s = 0; s < max_states && s < cfg.nstates; s++) {
+        const double sv[8] = {cfg.state_density[s], cfg.state_energy[s], cfg.state_xmin[s], cfg.state_xmax[s],
+                              cfg.state_ymin[s], cfg.state_ymax[s], cfg.state_radius[s],
+                              (double)cfg.state_geometry[s]};
+        memcpy(states + 8 * s, sv, sizeof sv);
+    }
+    return 0;
+}
+
 int tlo_run_deck(const char *deck_path, const char *log_path, int max_steps)
 {
     tlo_config cfg;

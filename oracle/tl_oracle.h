@@ -137,6 +137,16 @@ void tlo_calc_ch_coefs(double *ch_alphas, double *ch_betas, double eigmin, doubl
 /* ---- deck reader and driver (parse.f90, read_input.f90, start.f90, diffuse.f90,
  *      tea_solve.f90, field_summary.f90) ---- */
 int tlo_parse_deck(const char *path, tlo_config *cfg);
+/* The parsed deck in the fixed arrays of the library's tl_parse_deck:
+ * ints[23] = {x_cells, y_cells, end_step, summary_frequency,
+ * visit_frequency, max_iters, coefficient, tl_ch_cg_presteps, tl_check_result,
+ * tl_ppcg_inner_steps, reflective_boundary, solver, fortran_preconditioner,
+ * verbose_on, profiler_on, halo_depth, test_problem, ppcg_fresh_z, nstates,
+ * side-channel preconditioner, halo_depth, solver, profiler_on}, reals[8] =
+ * {xmin, ymin, xmax, ymax, end_time, dtinit, eps, tl_ch_cg_epslim}, and for
+ * each of the first max_states states states[8 s ..] = {density, energy, xmin,
+ * xmax, ymin, ymax, radius, geometry}. */
+int tlo_parse_deck_values(const char *path, int *ints, double *reals, double *states, int max_states);
 /* Runs the whole deck; writes a line-oriented log (SUMMARY/SOLVE/...).
  * max_steps > 0 caps the number of timesteps (for bounded CPU samples).
  * Returns 0 on success, nonzero on a fatal error (message on stderr). */

@@ -115,6 +115,7 @@ _SIGS = {
     "tl_forward_loss": (_I, [_P, _I, _D, _I, _D, _I, _DP, _DP]),
     "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
     "tl_check_division": (_I, [_DP, _DP, _DP, _DP, ctypes.c_long]),
+    "tl_parse_deck": (_I, [ctypes.c_char_p, _IP, _DP, _DP, _I]),
     "tl_set_tuning": (_I, [_P, _I, _I]),
     "tl_ppcg_inner_fused": (_I, [_P, _I, _I]),
     "tl_set_progress": (_I, [_P, _P, _P]),
@@ -442,6 +443,21 @@ def run_deck(deck, log_path, mode=MODE_NATIVE, max_steps=0, tea_out=None):
                                str(tea_out).encode() if tea_out else None, int(mode), int(max_steps)),
            "tl_run_deck")
     return parse_log(log_path)
+
+
+def parse_deck(deck, max_states=64):
+    """tl_parse_deck: the deck's settings as the library reads them, without a
+    GPU: (ints[23], reals[8], states[nstates, 8]) in the order of
+    include/tealeaf_mi355x.h."""
+    import numpy as np
+    lib = load_library()
+    ints = (_I * 23)()
+    reals = np.zeros(8)
+    states = np.zeros((max_states, 8))
+    _check(lib.tl_parse_deck(str(deck).encode(), ints, _dptr(reals), _dptr(states), int(max_states)),
+           "tl_parse_deck")
+    iv = list(ints)
+    return iv, reals, states[:min(iv[18], max_states)]
 
 
 def check_division(num, den):

@@ -848,6 +848,35 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
 
 }  // namespace
 
+extern "C" int tl_parse_deck(const char *deck, int *ints, double *reals, double *states, int max_states)
+{
+    if (!deck || !ints || !reals || (max_states > 0 && !states)) {
+        set_error("tl_parse_deck: null argument");
+        return TL_ERR_ARG;
+    }
+    Config cfg;
+    std::string err;
+    if (parse_deck(deck, &cfg, &err)) {
+        set_error("%s", err.c_str());
+        return TL_ERR_CONFIG;
+    }
+    const int iv[23] = {cfg.x_cells, cfg.y_cells, cfg.end_step, cfg.summary_frequency, cfg.visit_frequency,
+                        cfg.max_iters, cfg.coefficient, cfg.tl_ch_cg_presteps, cfg.tl_check_result,
+                        cfg.tl_ppcg_inner_steps, cfg.reflective_boundary, cfg.solver, cfg.fortran_preconditioner,
+                        cfg.verbose_on, cfg.profiler_on, cfg.halo_depth, cfg.test_problem, cfg.ppcg_fresh_z,
+                        cfg.nstates, cfg.sc.preconditioner, cfg.sc.halo_depth, cfg.sc.solver, cfg.sc.profiler_on};
+    const double rv[8] = {cfg.xmin, cfg.ymin, cfg.xmax, cfg.ymax, cfg.end_time, cfg.dtinit, cfg.eps,
+                          cfg.tl_ch_cg_epslim};
+    std::copy(iv, iv + 23, ints);
+    std::copy(rv, rv + 8, reals);
+    for (int s = 0; s < std::min(max_states, cfg.nstates); ++s) {
+        const double sv[8] = {cfg.density[s], cfg.energy[s], cfg.sxmin[s], cfg.sxmax[s],
+                              cfg.symin[s],   cfg.symax[s],  cfg.radius[s], (double)cfg.geometry[s]};
+        std::copy(sv, sv + 8, states + 8 * s);
+    }
+    return TL_OK;
+}
+
 extern "C" int tl_run_deck_out(const char *deck_path, const char *log_path, const char *tea_out_path, int mode,
                                int max_steps)
 {
