@@ -187,6 +187,14 @@ int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s);
  * xmax, ymin, ymax, radius, geometry}.
  * TL_ERR_CONFIG with tl_last_error() set when the deck is rejected. */
 int tl_parse_deck(const char *deck, int *ints, double *reals, double *states, int max_states);
+/* The host routines the drivers use between the CG presteps and the
+ * Chebyshev / PPCG phase (tea_calc_eigenvalues and tea_calc_ch_coefs of
+ * tea_leaf_cheby.f90:96-172; Fortran routines, no C entry point in the
+ * reference): the eigenvalue bounds of the Lanczos matrix of n CG steps
+ * (alphas/betas 0-based; status TL_ERR_SOLVER when tqli does not converge or
+ * a bound is negative), and n Chebyshev coefficients with theta. No GPU. */
+int tl_calc_eigenvalues(const double *cg_alphas, const double *cg_betas, int n, double *eigmin, double *eigmax);
+int tl_calc_ch_coefs(double eigmin, double eigmax, int n, double *ch_alphas, double *ch_betas, double *theta);
 /* Self-check of the division the iteration kernels use for faces, block
  * factors and diagonals (no reference counterpart): for i < n, quot[i] =
  * num[i] / den[i] by the compiler's IEEE FP64 division and fast[i] by the

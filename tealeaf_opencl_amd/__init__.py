@@ -116,6 +116,8 @@ _SIGS = {
     "tl_stream_bandwidth": (_I, [_P, _I, _I, _DP]),
     "tl_check_division": (_I, [_DP, _DP, _DP, _DP, ctypes.c_long]),
     "tl_parse_deck": (_I, [ctypes.c_char_p, _IP, _DP, _DP, _I]),
+    "tl_calc_eigenvalues": (_I, [_DP, _DP, _I, _DP, _DP]),
+    "tl_calc_ch_coefs": (_I, [_D, _D, _I, _DP, _DP, _DP]),
     "tl_set_tuning": (_I, [_P, _I, _I]),
     "tl_ppcg_inner_fused": (_I, [_P, _I, _I]),
     "tl_set_progress": (_I, [_P, _P, _P]),
@@ -458,6 +460,30 @@ def parse_deck(deck, max_states=64):
            "tl_parse_deck")
     iv = list(ints)
     return iv, reals, states[:min(iv[18], max_states)]
+
+
+def calc_eigenvalues(cg_alphas, cg_betas):
+    """tl_calc_eigenvalues: (eigmin, eigmax) of the Lanczos matrix of the CG
+    steps' alphas and betas (tea_leaf_cheby.f90:96-143)."""
+    import numpy as np
+    lib = load_library()
+    a = np.ascontiguousarray(cg_alphas, dtype=np.float64)
+    b = np.ascontiguousarray(cg_betas, dtype=np.float64)
+    lo, hi = _D(), _D()
+    _check(lib.tl_calc_eigenvalues(_dptr(a), _dptr(b), a.size, ctypes.byref(lo), ctypes.byref(hi)),
+           "tl_calc_eigenvalues")
+    return lo.value, hi.value
+
+
+def calc_ch_coefs(eigmin, eigmax, n):
+    """tl_calc_ch_coefs: (alphas, betas, theta) of n Chebyshev steps
+    (tea_leaf_cheby.f90:145-172)."""
+    import numpy as np
+    lib = load_library()
+    a, b, th = np.zeros(n), np.zeros(n), _D()
+    _check(lib.tl_calc_ch_coefs(float(eigmin), float(eigmax), int(n), _dptr(a), _dptr(b), ctypes.byref(th)),
+           "tl_calc_ch_coefs")
+    return a, b, th.value
 
 
 def check_division(num, den):

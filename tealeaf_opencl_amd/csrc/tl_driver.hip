@@ -848,6 +848,34 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
 
 }  // namespace
 
+extern "C" int tl_calc_eigenvalues(const double *cg_alphas, const double *cg_betas, int n, double *eigmin,
+                                   double *eigmax)
+{
+    if (!cg_alphas || !cg_betas || !eigmin || !eigmax || n < 1) {
+        set_error("tl_calc_eigenvalues: null argument or n %d < 1", n);
+        return TL_ERR_ARG;
+    }
+    if (calc_eigenvalues(cg_alphas, cg_betas, n, eigmin, eigmax) != 0) {
+        set_error("tl_calc_eigenvalues: tqli did not converge or a bound is negative");
+        return TL_ERR_SOLVER;
+    }
+    return TL_OK;
+}
+
+extern "C" int tl_calc_ch_coefs(double eigmin, double eigmax, int n, double *ch_alphas, double *ch_betas,
+                                double *theta)
+{
+    if (!ch_alphas || !ch_betas || !theta || n < 1) {
+        set_error("tl_calc_ch_coefs: null argument or n %d < 1", n);
+        return TL_ERR_ARG;
+    }
+    std::vector<double> ca, cb;
+    calc_ch_coefs(ca, cb, eigmin, eigmax, theta, n);
+    std::copy(ca.begin(), ca.begin() + n, ch_alphas);
+    std::copy(cb.begin(), cb.begin() + n, ch_betas);
+    return TL_OK;
+}
+
 extern "C" int tl_parse_deck(const char *deck, int *ints, double *reals, double *states, int max_states)
 {
     if (!deck || !ints || !reals || (max_states > 0 && !states)) {
