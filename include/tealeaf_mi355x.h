@@ -195,6 +195,13 @@ int tl_parse_deck(const char *deck, int *ints, double *reals, double *states, in
  * a bound is negative), and n Chebyshev coefficients with theta. No GPU. */
 int tl_calc_eigenvalues(const double *cg_alphas, const double *cg_betas, int n, double *eigmin, double *eigmax);
 int tl_calc_ch_coefs(double eigmin, double eigmax, int n, double *ch_alphas, double *ch_betas, double *theta);
+/* The Fortran edit descriptors tea.out is written with (tl_run_deck_out; no
+ * reference counterpart): kind 0 Ew.d, 1 1PEw.d, 2 Fw.d, 3 a list-directed
+ * REAL(8) item (w, d unused); the text of v goes to buf (len bytes, NUL
+ * terminated). Ew.d, 1PEw.d and Fw.d follow the Fortran standard's
+ * editing; a list-directed item's form is compiler-defined (a fixed E25.16
+ * style here). */
+int tl_format_fortran(int kind, double v, int w, int d, char *buf, int len);
 /* Self-check of the division the iteration kernels use for faces, block
  * factors and diagonals (no reference counterpart): for i < n, quot[i] =
  * num[i] / den[i] by the compiler's IEEE FP64 division and fast[i] by the

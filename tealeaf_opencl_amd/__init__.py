@@ -117,6 +117,7 @@ _SIGS = {
     "tl_check_division": (_I, [_DP, _DP, _DP, _DP, ctypes.c_long]),
     "tl_parse_deck": (_I, [ctypes.c_char_p, _IP, _DP, _DP, _I]),
     "tl_calc_eigenvalues": (_I, [_DP, _DP, _I, _DP, _DP]),
+    "tl_format_fortran": (_I, [_I, _D, _I, _I, ctypes.c_char_p, _I]),
     "tl_calc_ch_coefs": (_I, [_D, _D, _I, _DP, _DP, _DP]),
     "tl_set_tuning": (_I, [_P, _I, _I]),
     "tl_ppcg_inner_fused": (_I, [_P, _I, _I]),
@@ -460,6 +461,15 @@ def parse_deck(deck, max_states=64):
            "tl_parse_deck")
     iv = list(ints)
     return iv, reals, states[:min(iv[18], max_states)]
+
+
+def format_fortran(kind, v, w=0, d=0):
+    """tl_format_fortran: v as tea.out writes it (kind 0 Ew.d, 1 1PEw.d, 2
+    Fw.d, 3 list-directed)."""
+    lib = load_library()
+    buf = ctypes.create_string_buffer(128)
+    _check(lib.tl_format_fortran(int(kind), float(v), int(w), int(d), buf, 128), "tl_format_fortran")
+    return buf.value.decode()
 
 
 def calc_eigenvalues(cg_alphas, cg_betas):

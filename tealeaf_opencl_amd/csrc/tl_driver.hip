@@ -90,7 +90,7 @@ std::string fort_E(double v, int w, int d)
     return right(sign + "0." + mant + eb, w);
 }
 
-// 1PEw.d: [-]d.dd..E+ee
+// 1PEw.d: [-]d.dd..E+ee, or [-]d.dd..+eee for |exp| > 99
 std::string fort_1PE(double v, int w, int d)
 {
     char buf[64];
@@ -99,7 +99,8 @@ std::string fort_1PE(double v, int w, int d)
     const size_t epos = t.find('E');
     const int ex = atoi(t.c_str() + epos + 1);
     char eb[16];
-    snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    if (ex >= -99 && ex <= 99) snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
+    else snprintf(eb, sizeof eb, "%c%03d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
     return right(t.substr(0, epos) + eb, w);
 }
 
@@ -113,7 +114,8 @@ std::string fort_F(double v, int w, int d)
 std::string fort_I(long v, int w) { return right(std::to_string(v), w); }
 
 // list-directed REAL(8) item (the leading blank of the record is written by
-// the caller)
+// the caller). Its form is the Fortran compiler's choice (the reference's
+// output differs between compilers); this one is a fixed E25.16 style.
 std::string fort_list(double v)
 {
     char buf[64];
@@ -847,6 +849,22 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
 }
 
 }  // namespace
+
+extern "C" int tl_format_fortran(int kind, double v, int w, int d, char *buf, int len)
+{
+    if (!buf || len < 1 || kind < 0 || kind > 3 || (kind < 3 && (w < 1 || d < 0 || w > 60))) {
+        set_error("tl_format_fortran: bad kind %d, w %d, d %d or buffer", kind, w, d);
+        return TL_ERR_ARG;
+    }
+    const std::string s = kind == 0 ? fort_E(v, w, d) : kind == 1 ? fort_1PE(v, w, d) : kind == 2 ? fort_F(v, w, d)
+                                                                                      : fort_list(v);
+    if ((int)s.size() >= len) {
+        set_error("tl_format_fortran: buffer of %d bytes too small", len);
+        return TL_ERR_ARG;
+    }
+    memcpy(buf, s.c_str(), s.size() + 1);
+    return TL_OK;
+}
 
 extern "C" int tl_calc_eigenvalues(const double *cg_alphas, const double *cg_betas, int n, double *eigmin,
                                    double *eigmax)
