@@ -1010,13 +1010,17 @@ static double fortran_rval(const char *w, int *ok)
     return v;
 }
 
-/* one processed deck line as parse_getline returns it (lowercased, '='/','
- * turned to spaces, comments cut) */
+/* the processed deck lines parse_getline returns (parse.f90:143-247:
+ * lowercased, '='/',' turned to spaces, comments cut), only those read while
+ * the block stack (here, parse.f90:186-224) is exactly "*tea"; the reading
+ * ends at the *endtea that closes it (parse_getline returns -1). -1: cannot
+ * open, -2: unmatched * / *end pair. */
 static int deck_lines(const char *path, char (**out)[LINE_MAX_LEN], int *nout)
 {
     FILE *f = fopen(path, "r");
     if (!f) return -1;
-    int cap = 64, n = 0, inside = 0;
+    int cap = 64, n = 0;
+    char here[LINE_MAX_LEN * 2] = "";  /* the open blocks' names, concatenated */
     char (*lines)[LINE_MAX_LEN] = malloc(sizeof(*lines) * cap);
     char l[LINE_MAX_LEN];
     while (fgets(l, sizeof l, f)) {
@@ -1039,11 +1043,29 @@ static int deck_lines(const char *path, char (**out)[LINE_MAX_LEN], int *nout)
         while (len && s[len - 1] == ' ') s[--len] = 0;
         if (!len) continue;
         if (s[0] == '*') {
-            if (strncmp(s, "*tea", 4) == 0 && (s[4] == 0 || s[4] == ' ')) inside = 1;
-            else if (strncmp(s, "*endtea", 7) == 0) inside = 0;
+            if (strncmp(s, "*select:", 8) == 0) continue;
+            char nugget[LINE_MAX_LEN];
+            size_t w = strcspn(s, " ");
+            memcpy(nugget, s, w);
+            nugget[w] = 0;
+            if (w >= 4 && strncmp(nugget + 1, "end", 3) == 0) {
+                char closes[LINE_MAX_LEN];
+                snprintf(closes, sizeof closes, "*%s", nugget + 4);
+                char *last = strrchr(here, '*');
+                if (!last || strcmp(last, closes) != 0) {
+                    fclose(f);
+                    free(lines);
+                    return -2;
+                }
+                const int at_mask = strcmp(here, "*tea") == 0;
+                *last = 0;
+                if (at_mask) break;
+            } else if (strlen(here) + w < sizeof here) {
+                strcat(here, nugget);
+            }
             continue;
         }
-        if (!inside) continue;
+        if (strcmp(here, "*tea") != 0) continue;
         if (n == cap) { cap *= 2; lines = realloc(lines, sizeof(*lines) * cap); }
         strncpy(lines[n], s, LINE_MAX_LEN - 1);
         lines[n][LINE_MAX_LEN - 1] = 0;
@@ -1175,8 +1197,9 @@ int tlo_parse_deck(const char *path, tlo_config *cfg)
     config_defaults(cfg);
     char (*lines)[LINE_MAX_LEN];
     int nl;
-    if (deck_lines(path, &lines, &nl)) {
-        fprintf(stderr, "Error opening %s\n", path);
+    const int rl = deck_lines(path, &lines, &nl);
+    if (rl) {
+        fprintf(stderr, rl == -2 ? "Unmatched */*end pair in %s\n" : "Error opening %s\n", path);
         return 1;
     }
     int ok, rc = 0;

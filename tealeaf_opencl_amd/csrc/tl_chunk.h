@@ -125,5 +125,8 @@ int read_result(tl_chunk *c, double *out, int nq);
 int precondition_r(tl_chunk *c);
 // frees the refined-solve workspaces (tl_refine.hip)
 void refine_release(tl_chunk *c);
+// destroys every captured iteration batch (native and refined): they hold the
+// face parameters (rx, ry, walls) and buffer addresses by value
+void drop_graphs(tl_chunk *c);
 
 }  // namespace tl

@@ -120,6 +120,18 @@ int ensure_coef_capacity(tl_chunk *c, int n)
     return TL_OK;
 }
 
+void drop_graphs(tl_chunk *c)
+{
+    tl_chunk::GraphSlot *slots[] = {&c->g_cg, &c->g_cheby, &c->g_ppcg, &c->g_jacobi, &c->rf[0].graph,
+                                    &c->rf[0].graph_ppcg, &c->rf[1].graph, &c->rf[1].graph_ppcg};
+    for (auto *gs : slots)
+        if (gs->exec) {
+            hipGraphExecDestroy(gs->exec);
+            gs->exec = nullptr;
+            gs->key = -1;
+        }
+}
+
 int ensure_hist_capacity(tl_chunk *c, int n)
 {
     if (n <= c->hist_len) return TL_OK;
@@ -509,12 +521,23 @@ extern "C" int tl_common_init(tl_chunk *c, int coefficient, double dt, double *r
     c->factors_current = false;
     launch_conductivity(c->s, c->g, c->f2[F_DENSITY], c->f2[F_COND], coefficient);
     TL_CHECK_LAUNCH(c, "conductivity");
-    c->faces.rx = *rx;
-    c->faces.ry = *ry;
-    c->faces.zl = !reflective_boundary && zero_boundary[0];
-    c->faces.zr = !reflective_boundary && zero_boundary[1];
-    c->faces.zb = !reflective_boundary && zero_boundary[2];
-    c->faces.zt = !reflective_boundary && zero_boundary[3];
+    Faces nf;
+    nf.rx = *rx;
+    nf.ry = *ry;
+    nf.zl = !reflective_boundary && zero_boundary[0];
+    nf.zr = !reflective_boundary && zero_boundary[1];
+    nf.zb = !reflective_boundary && zero_boundary[2];
+    nf.zt = !reflective_boundary && zero_boundary[3];
+    // the captured batches were recorded with the previous faces as kernel
+    // arguments: replaying them after a change of dt or walls would apply the
+    // old operator
+    if (memcmp(&nf.rx, &c->faces.rx, sizeof(double)) != 0 || memcmp(&nf.ry, &c->faces.ry, sizeof(double)) != 0 ||
+        nf.zl != c->faces.zl || nf.zr != c->faces.zr || nf.zb != c->faces.zb || nf.zt != c->faces.zt) {
+        hipError_t e = hipStreamSynchronize(c->s);
+        if (e != hipSuccess) return hip_fail("tl_common_init", e);
+        drop_graphs(c);
+    }
+    c->faces = nf;
     if (!reflective_boundary) {
         prof_begin(c);
         launch_zero_boundary(c->s, c->g, ext_range(c, -hd, hd, -hd, hd), c->cap, c->f2[F_KX], c->f2[F_KY],

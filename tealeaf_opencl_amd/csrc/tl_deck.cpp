@@ -16,14 +16,19 @@ namespace tl {
 namespace {
 
 // parse_getline (parse.f90:143-247): blank control characters, cut '!' and
-// ';' comments, lower-case, '=' and ',' become blanks; only lines inside the
-// *tea ... *endtea block are returned
+// ';' comments, lower-case, '=' and ',' become blanks. A line whose first
+// word starts with '*' opens the block of that name, or closes one when its
+// characters 2-4 are "end" ("*endtea" closes "*tea"; the innermost open block
+// must be the one closed). Only lines read while exactly the *tea block is
+// open are returned, and reading stops at the *endtea that closes it.
+// Returns -1 when the file cannot be opened, -2 for an unmatched */*end pair.
 int deck_lines(const std::string &path, std::vector<std::string> *out)
 {
     std::ifstream f(path);
     if (!f.is_open()) return -1;
     std::string raw;
-    bool inside = false;
+    std::vector<std::string> open_blocks;
+    auto at_tea = [&]() { return open_blocks.size() == 1 && open_blocks[0] == "*tea"; };
     while (std::getline(f, raw)) {
         if (raw.size() > 100) raw.resize(100);  // READ(... '(a100)')
         for (auto &ch : raw)
@@ -41,11 +46,20 @@ int deck_lines(const std::string &path, std::vector<std::string> *out)
         std::string l = raw.substr(b);
         while (!l.empty() && l.back() == ' ') l.pop_back();
         if (l[0] == '*') {
-            if (l.compare(0, 4, "*tea") == 0 && (l.size() == 4 || l[4] == ' ')) inside = true;
-            else if (l.compare(0, 7, "*endtea") == 0) inside = false;
+            if (l.compare(0, 8, "*select:") == 0) continue;
+            const std::string name = l.substr(0, l.find(' '));
+            if (name.compare(1, 3, "end") == 0) {
+                const std::string closes = "*" + name.substr(std::min<size_t>(4, name.size()));
+                if (open_blocks.empty() || open_blocks.back() != closes) return -2;
+                const bool last = at_tea();
+                open_blocks.pop_back();
+                if (last) break;
+            } else {
+                open_blocks.push_back(name);
+            }
             continue;
         }
-        if (inside) out->push_back(l);
+        if (at_tea()) out->push_back(l);
     }
     return 0;
 }
@@ -158,8 +172,9 @@ int parse_deck(const std::string &path, Config *cfg, std::string *err)
 {
     *cfg = Config();
     std::vector<std::string> lines;
-    if (deck_lines(path, &lines)) {
-        *err = "Error opening " + path;
+    const int rl = deck_lines(path, &lines);
+    if (rl) {
+        *err = rl == -2 ? "Unmatched */*end pair in " + path : "Error opening " + path;
         return 1;
     }
     int state_max = 0;

@@ -70,9 +70,18 @@ std::string right(const std::string &s, int w)
 }
 
 // Ew.d (no scale factor): [-]0.d1..ddE+ee, or [-]0.d1..dd+eee for |exp| > 99
+// Infinity and NaN in a field of width w, as amdflang writes them under E, 1PE
+// and F editing: "NaN", "Inf", "-Inf", right-justified, asterisks when
+// narrower
+std::string fort_nonfinite(double v, int w)
+{
+    return right(std::isnan(v) ? "NaN" : v < 0 ? "-Inf" : "Inf", w);
+}
+
 std::string fort_E(double v, int w, int d)
 {
-    char buf[64];
+    if (!std::isfinite(v)) return fort_nonfinite(v, w);
+    char buf[128];
     std::string mant, sign = (v < 0 || (v == 0 && std::signbit(v))) ? "-" : "";
     int ex = 0;
     if (v == 0.0) {
@@ -81,6 +90,7 @@ std::string fort_E(double v, int w, int d)
         snprintf(buf, sizeof buf, "%.*e", d - 1, fabs(v));  // m.mmm e+xx
         std::string t(buf);
         const size_t epos = t.find('e');
+        if (epos == std::string::npos || epos < 2) return std::string(w, '*');
         mant = t.substr(0, 1) + t.substr(2, epos - 2);
         ex = atoi(t.c_str() + epos + 1) + 1;
     }
@@ -93,10 +103,12 @@ std::string fort_E(double v, int w, int d)
 // 1PEw.d: [-]d.dd..E+ee, or [-]d.dd..+eee for |exp| > 99
 std::string fort_1PE(double v, int w, int d)
 {
-    char buf[64];
+    if (!std::isfinite(v)) return fort_nonfinite(v, w);
+    char buf[128];
     snprintf(buf, sizeof buf, "%.*E", d, v);
     std::string t(buf);
     const size_t epos = t.find('E');
+    if (epos == std::string::npos) return std::string(w, '*');
     const int ex = atoi(t.c_str() + epos + 1);
     char eb[16];
     if (ex >= -99 && ex <= 99) snprintf(eb, sizeof eb, "E%c%02d", ex < 0 ? '-' : '+', ex < 0 ? -ex : ex);
@@ -106,6 +118,7 @@ std::string fort_1PE(double v, int w, int d)
 
 std::string fort_F(double v, int w, int d)
 {
+    if (!std::isfinite(v)) return fort_nonfinite(v, w);
     char buf[64];
     snprintf(buf, sizeof buf, "%.*f", d, v);
     return right(buf, w);
@@ -852,7 +865,10 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
 
 extern "C" int tl_format_fortran(int kind, double v, int w, int d, char *buf, int len)
 {
-    if (!buf || len < 1 || kind < 0 || kind > 3 || (kind < 3 && (w < 1 || d < 0 || w > 60))) {
+    // d < w bounds every text (at most 60 + exponent characters), and Ew.d
+    // needs at least one mantissa digit
+    if (!buf || len < 1 || kind < 0 || kind > 3 ||
+        (kind < 3 && (w < 1 || d < 0 || w > 60 || d >= w || (kind == 0 && d < 1)))) {
         set_error("tl_format_fortran: bad kind %d, w %d, d %d or buffer", kind, w, d);
         return TL_ERR_ARG;
     }

@@ -120,6 +120,38 @@ def test_cg_and_ppcg_on_one_chunk(hiplib, oracle, precision):
         assert np.array_equal(u, u_ref)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("solver", ["cg", "ppcg"])
+@pytest.mark.parametrize("change", ["dt", "walls"])
+def test_reinit_with_new_operator(hiplib, oracle, precision, solver, change):
+    """A solve, then tl_common_init with another dt (rx, ry) or with other
+    walls (reflective: no zeroed faces, halo densities filled), then a second
+    solve on the same chunk: the captured iteration batches hold the face
+    parameters as kernel arguments, so they must be recaptured. The second
+    solve gives bitwise the solution of the same solve on a fresh chunk."""
+    dt2, refl = (0.008, False) if change == "dt" else (0.004, True)
+
+    def solve(g):
+        if solver == "cg":
+            res = g.solve_refined(precision, tol=1e-10)
+        else:
+            res = g.solve_refined_ppcg(precision, tol=1e-10, max_iters=5000, presteps=20, inner_steps=6)
+        assert res["converged"], res
+        return res["iterations"], g.get_field("u")
+
+    ref = hiplib.Chunk(200, 41, 1, 3)
+    setup(ref, coef=2, dt=dt2, reflective=refl)
+    n_ref, u_ref = solve(ref)
+    g, _ = pair(hiplib, oracle, 200, 41, 1, 3, 2)
+    solve(g)
+    if refl:
+        g.update_halo((1, 1, 1), g.hd)
+    g.common_init(2, dt2, reflective_boundary=refl)
+    n2, u2 = solve(g)
+    assert n2 == n_ref
+    assert np.array_equal(u2, u_ref)
+
+
 def test_iteration_cap(hiplib, oracle):
     g, _ = pair(hiplib, oracle, 200, 41, 1, 2, 2)
     res = g.solve_refined(1, tol=1e-12, max_iters=7)

@@ -4,7 +4,11 @@ What pins what:
 * whole path (deck reader, generate, driver, Jacobi/PPCG solvers, field
   summary): the reference's known-answer temp values, field_summary.f90:90-95,
   tests 1 and 6 (test 2 is run on the GPU and by tests/golden/make_kat2.sh);
-* face coefficients: closed forms on uniform and interface cells;
+* face coefficients: closed forms of both coefficient kinds (conductivity =
+  density and = 1/density) on every face of a random density, the walls
+  exactly zero;
+* reflective update_halo: a mirror index map of each external face's halo band
+  (corners included), its symmetry, and that an off-by-one mirror differs;
 * operator: A 1 = 1 with Neumann walls, symmetry <Ax,y> = <x,Ay>;
 * solvers: dense numpy solves of A u = u0 on small grids (CG none/diag/block,
   Jacobi, and whole-deck Chebyshev/PPCG via the field summary);
@@ -84,6 +88,121 @@ def test_face_coefficients_closed_form(oracle):
             assert abs(Kx[k, j] - want) <= 1e-14 * want
     # walls are zero-flux
     assert np.all(Kx[:, 1] == 0) and np.all(Kx[:, 41] == 0)
+
+
+@pytest.mark.parametrize("coef", [1, 2])
+@pytest.mark.parametrize("hd", [1, 2])
+def test_face_coefficients_every_face(oracle, coef, hd):
+    """tea_leaf_init_common + tea_leaf_zero_boundary
+    (tea_leaf_common_cl.cl:56-125) on a log-uniform random density, both
+    coefficient kinds, against their closed forms: conductivity = density
+    gives K = rx (1/rho_l + 1/rho_c) / 2, conductivity = 1/density gives
+    K = rx (rho_l + rho_c) / 2. With all four walls external, every face that
+    is not between two interior cells is exactly 0."""
+    nx, ny = 23, 17
+    o = oracle.OracleChunk(nx, ny, hd, 1)
+    setup(o, seed=2)
+    rng = np.random.default_rng(11)
+    rho = 10.0 ** rng.uniform(-1, 2, o.shape)
+    rho_in = np.zeros(o.shape)
+    o.interior(rho_in)[:] = o.interior(rho)
+    o.put_field("density", rho_in)
+    rx, ry = o.common_init(coef, 0.004)
+    Kx, Ky = o.get_field("Kx"), o.get_field("Ky")
+
+    def closed(a, b, r):
+        return r * 0.5 * (1.0 / a + 1.0 / b) if coef == 1 else r * 0.5 * (a + b)
+
+    want_x = np.zeros(o.shape)
+    want_y = np.zeros(o.shape)
+    rows, cols = slice(hd, hd + ny), slice(hd, hd + nx)
+    # x faces between interior columns j-1 and j, j = hd+1 .. hd+nx-1
+    want_x[rows, hd + 1:hd + nx] = closed(rho_in[rows, hd:hd + nx - 1], rho_in[rows, hd + 1:hd + nx], rx)
+    want_y[hd + 1:hd + ny, cols] = closed(rho_in[hd:hd + ny - 1, cols], rho_in[hd + 1:hd + ny, cols], ry)
+    mask_x, mask_y = want_x != 0, want_y != 0
+    assert np.all(Kx[~mask_x] == 0.0) and np.all(Ky[~mask_y] == 0.0)
+    assert np.max(np.abs(Kx[mask_x] - want_x[mask_x]) / want_x[mask_x]) <= 2e-15
+    assert np.max(np.abs(Ky[mask_y] - want_y[mask_y]) / want_y[mask_y]) <= 2e-15
+    # the two kinds differ on every interior face of a non-uniform field
+    other = (rx * 0.5 * (rho_in[rows, hd:hd + nx - 1] + rho_in[rows, hd + 1:hd + nx]) if coef == 1 else
+             rx * 0.5 * (1.0 / rho_in[rows, hd:hd + nx - 1] + 1.0 / rho_in[rows, hd + 1:hd + nx]))
+    assert np.all(np.abs(Kx[rows, hd + 1:hd + nx] - other) > 1e-6 * other)
+
+
+def _reflected(a0, hd, nx, ny, depth, ext):
+    """The reflective fill stated as an index map, apart from the kernels'
+    offset arithmetic (update_halo_cl.cl:3-69, extents ocl_kernels.cpp:356-412):
+    a cell of the halo band of an external face, within `depth` of the
+    interior and spanning the corner squares of that depth, takes the value of
+    its mirror image across that wall (and across the other wall too in a
+    corner whose other face is also external); every other cell keeps its
+    value. ext = (left, right, bottom, top)."""
+    ny2, nx2 = a0.shape
+    left, right, bottom, top = ext
+
+    def mirror(i, lo, n, lo_ext, hi_ext):
+        # reflection across the wall between lo-1|lo or lo+n-1|lo+n
+        if i < lo and lo_ext:
+            return 2 * lo - 1 - i
+        if i >= lo + n and hi_ext:
+            return 2 * (lo + n) - 1 - i
+        return i
+
+    out = a0.copy()
+    band_x = lambda j: hd - depth <= j < hd + nx + depth
+    band_y = lambda k: hd - depth <= k < hd + ny + depth
+    for k in range(ny2):
+        for j in range(nx2):
+            in_b = bottom and hd - depth <= k < hd and band_x(j)
+            in_t = top and hd + ny <= k < hd + ny + depth and band_x(j)
+            in_l = left and hd - depth <= j < hd and band_y(k)
+            in_r = right and hd + nx <= j < hd + nx + depth and band_y(k)
+            if in_b or in_t or in_l or in_r:
+                out[k, j] = a0[mirror(k, hd, ny, bottom, top), mirror(j, hd, nx, left, right)]
+    return out
+
+
+@pytest.mark.parametrize("hd,depth", [(1, 1), (2, 1), (2, 2), (3, 1), (3, 2), (3, 3)])
+@pytest.mark.parametrize("ext", [(1, 1, 1, 1), (1, 0, 1, 0), (0, 1, 0, 1), (0, 0, 1, 0)])
+def test_update_halo_reflective_mirror(oracle, hd, depth, ext):
+    """The oracle's update_halo against the mirror index map above on random
+    density, energy0 and energy1 (every cell, halos included, random before
+    the fill), with only the flagged fields touched (fields[] of
+    update_halo_kernel_ocl.cpp:48-62). Plus the mirror's own symmetry: after a
+    full-depth fill with all faces external, reflecting the whole array
+    across each wall reproduces it on every filled cell."""
+    nx, ny = 13, 11
+    o = oracle.OracleChunk(nx, ny, hd, 1)
+    rng = np.random.default_rng(100 * hd + depth)
+    names = ["density", "energy0", "energy1"]
+    a0 = {n: rng.standard_normal(o.shape) for n in names}
+    for n in names:
+        o.put_field(n, a0[n])
+    nb = [-1 if e else 3 for e in ext]
+    o.update_halo((1, 0, 1), depth, chunk_neighbours=nb)
+    for n in names:
+        want = a0[n] if n == "energy0" else _reflected(a0[n], hd, nx, ny, depth, ext)
+        assert np.array_equal(o.get_field(n), want), n
+    if ext == (1, 1, 1, 1) and depth == hd:
+        got = o.get_field("density")
+        core = got[:, :]
+        assert np.array_equal(core[:, hd - 1::-1], core[:, hd:2 * hd])
+        assert np.array_equal(core[:, hd + nx:], core[:, hd + nx - 1:nx - 1:-1][:, :hd])
+        assert np.array_equal(core[hd - 1::-1, :], core[hd:2 * hd, :])
+        assert np.array_equal(core[hd + ny:, :], core[hd + ny - 1:ny - 1:-1, :][:hd, :])
+
+
+def test_update_halo_mirror_catches_wrong_index(oracle):
+    """The index map is strict enough to see a mirror that is one cell off
+    (src = 2hd - j instead of 2hd - 1 - j): such a fill differs from it."""
+    hd, nx, ny = 2, 9, 7
+    rng = np.random.default_rng(5)
+    a0 = rng.standard_normal((ny + 2 * hd, nx + 2 * hd))
+    wrong = a0.copy()
+    for k in range(ny + 2 * hd):
+        for j in range(hd):
+            wrong[k, j] = a0[k, 2 * hd - j]
+    assert not np.array_equal(wrong, _reflected(a0, hd, nx, ny, hd, (1, 0, 0, 0)))
 
 
 def test_operator_invariants(oracle):

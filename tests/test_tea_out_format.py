@@ -23,16 +23,24 @@ DESCRIPTORS = [(0, 14, 7), (0, 15, 7), (0, 14, 6), (0, 26, 17), (0, 16, 7), (2, 
 def values():
     rng = np.random.default_rng(11)
     v = [0.0, 1.0, -1.0, 0.5, 100.0, 9436.127524375916, 124.53439159789798, 15.131590925116196, 0.004,
-         1.0e-15, 9.999999999999999e-1, 9.9999999e-6, 0.999999995, 123456.789, -3.25e-7, 1.0e-100, 2.5e120]
+         1.0e-15, 9.999999999999999e-1, 9.9999999e-6, 0.999999995, 123456.789, -3.25e-7, 1.0e-100, 2.5e120,
+         float("nan"), float("inf"), float("-inf")]
     v += list(rng.uniform(-1, 1, 20) * 10.0 ** rng.integers(-20, 20, 20))
     return v
 
 
 def fortran_records(tmp_path, vals):
-    lines = ["program fmt", "  implicit none", "  real(8) :: v(%d)" % len(vals), "  integer :: i"]
+    lines = ["program fmt", "  use, intrinsic :: ieee_arithmetic", "  implicit none",
+             "  real(8) :: v(%d)" % len(vals), "  integer :: i"]
     # 17 significant digits with a d exponent: each REAL(8) literal is the double itself
     for i, x in enumerate(vals, 1):
-        lines.append("  v(%d) = %s" % (i, ("%.17e" % x).replace("e", "d")))
+        if np.isnan(x):
+            lit = "ieee_value(1d0, ieee_quiet_nan)"
+        elif np.isinf(x):
+            lit = "ieee_value(1d0, ieee_%s_inf)" % ("positive" if x > 0 else "negative")
+        else:
+            lit = ("%.17e" % x).replace("e", "d")
+        lines.append("  v(%d) = %s" % (i, lit))
     lines += ["  do i = 1, %d" % len(vals)]
     for kind, w, d in DESCRIPTORS:
         desc = {0: "E%d.%d", 1: "1PE%d.%d", 2: "F%d.%d"}[kind] % (w, d)
@@ -59,3 +67,14 @@ def test_descriptors_match_fortran(hiplib, tmp_path):
                 bad.append((x, (kind, w, d), recs[k], mine))
             k += 1
     assert not bad, bad[:8]
+
+
+def test_rejects_out_of_range_digits(hiplib):
+    """d must stay below w (the text of every edit then fits its buffers),
+    and Ew.d needs d >= 1."""
+    for kind, w, d in [(0, 10, 10), (0, 60, 59 + 1), (1, 9, 9), (2, 5, 7), (0, 10, 0), (0, 61, 3), (0, 0, 0)]:
+        with pytest.raises(hiplib.TealeafError):
+            hiplib.format_fortran(kind, 1.25, w, d)
+    # the widest accepted edits still format
+    assert len(hiplib.format_fortran(0, -1.0e-300, 60, 59)) == 60
+    assert len(hiplib.format_fortran(1, 1.0e300, 60, 59)) == 60
