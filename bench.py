@@ -19,6 +19,11 @@ import sys
 import tempfile
 import time
 
+# the CPU baseline's OpenMP threads stay on neighbouring cores (read by the
+# OpenMP runtime when the oracle library is loaded)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -72,6 +77,9 @@ def parse():
     ap.add_argument("--max-iters", type=int, default=5000, help="tl_max_iters of the deck (short profiles)")
     ap.add_argument("--no-e10", action="store_true", help="skip the solve-to-1e-10 leg")
     ap.add_argument("--no-solvers", action="store_true", help="skip the Chebyshev and PPCG leg")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the leg in the other driver mode")
+    ap.add_argument("--only-e10", action="store_true",
+                    help="run only the solve-to-1e-10 leg (for profiling it in a process of its own)")
     ap.add_argument("--large", action="store_true",
                     help="run the large combined workload (8192^2 PPCG FP64 vs mixed, adjoint) instead")
     ap.add_argument("--large-n", type=int, default=8192)
@@ -84,15 +92,53 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(deck, iters):
+def host_cpu():
+    """nproc, the CPU model and the socket count of the host (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "model": None, "sockets": None,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        info["nproc"] = int(subprocess.run(["nproc"], stdout=subprocess.PIPE, text=True, timeout=10).stdout)
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() == "Model name":
+                info["model"] = v.strip()
+            elif k.strip() == "Socket(s)":
+                info["sockets"] = int(v.strip())
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(deck, iters, d):
+    """The FP64 oracle on the host cores: a bounded sample of the bench
+    workload (the first `iters` CG iterations of its first timestep), and the
+    512x512 CG + diagonal-Jacobi deck of two seeded rectangles timed in full
+    (10 timesteps) beside the same deck on the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
+    import tealeaf_opencl_amd as tl
+    from tealeaf_opencl_amd import decks
     pyoracle.load()
     secs, done = pyoracle.time_cg_iterations(deck, iters)
+    c1 = decks.config1_deck(os.path.join(d, "config1.in"), seed=1)
+    t0 = time.perf_counter()
+    lo = pyoracle.run_deck(c1, os.path.join(d, "config1_cpu.log"))
+    cpu_s = time.perf_counter() - t0
+    g = tl.bench_deck(c1, mode=tl.MODE_NATIVE, warmup=0, steps=10)
+    its = int(sum(x[1] for x in lo["solve"]))
     return {"value": done / secs if secs > 0 else None, "unit": "iters/s", "cores": pyoracle.num_threads(),
             "kind": "port",
             "sample": "%d FP64 CG + block-Jacobi iterations of the first timestep at %s, oracle/tl_oracle.c "
-                      "(OpenMP), timed after setup" % (done, os.path.basename(deck))}
+                      "(OpenMP), timed after setup" % (done, os.path.basename(deck)),
+            "host": host_cpu(),
+            "config1_512": {"workload": "512x512, 2 rectangles (seed 1), CG + diagonal Jacobi, eps 1e-15, "
+                                        "10 timesteps, whole deck",
+                            "cpu_s": cpu_s, "cpu_iterations": its, "cpu_iters_per_s": its / cpu_s,
+                            "gpu_solve_s": g["seconds"], "gpu_iterations": g["iterations"],
+                            "gpu_iters_per_s": g["iterations"] / g["seconds"]}}
 
 
 def refine_leg(n, tol=1e-10):
@@ -154,6 +200,19 @@ def refine_leg(n, tol=1e-10):
     }
 
 
+def dropin_leg(deck, mode, steps=2):
+    """The same deck in the other driver mode: with the default native mode,
+    the drop-in mode is the reference's own call sequence (tea_solve.f90 with
+    host alpha/beta and two blocking scalars per CG iteration through the
+    *_kernel_ocl entry points), one untimed and `steps` timed timesteps."""
+    import tealeaf_opencl_amd as tl
+    other = tl.MODE_DROPIN if mode == tl.MODE_NATIVE else tl.MODE_NATIVE
+    r = tl.bench_deck(deck, mode=other, warmup=1, steps=steps)
+    return {"mode": "drop-in" if other == tl.MODE_DROPIN else "native", "iterations": r["iterations"],
+            "seconds": r["seconds"], "iters_per_s": r["iterations"] / r["seconds"],
+            "final_summary": r["final_summary"]}
+
+
 def solvers_leg(n, d, steps=1, max_iters=5000):
     """The other two solvers of the 4096^2 deck: Chebyshev after 20 CG
     eigenvalue-estimate steps and PPCG with 10 inner steps, both with strip-4
@@ -162,9 +221,12 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
     the fused Chebyshev iterations, two per launch, stream u, u0, p, Kx, Ky
     in and u, p out (56 B/cell per two iterations); a PPCG outer step is the two fused CG kernels (88 B/cell)
     plus 10 inner steps, two per launch, each launch streaming u, r, sd, Kx,
-    Ky in and u, r, sd out (64 B/cell). `reference_equiv_GBs` counts the
-    reference's bytes for the same iterations instead (Chebyshev 104 B/cell,
-    PPCG outer 208 and inner 112, CG 144)."""
+    Ky in and u, r, sd out (64 B/cell). `vs_reference_roofline` is the
+    measured time against the time the reference's kernels would need for
+    the same iterations at 100 % of peak HBM bandwidth with their own bytes
+    (Chebyshev 104 B/cell, PPCG outer 208 and inner 112, CG 144):
+    dimensionless, above 1 when fewer bytes beat the reference's
+    roofline."""
     import tealeaf_opencl_amd as tl
     from tealeaf_opencl_amd import decks
 
@@ -187,7 +249,7 @@ def solvers_leg(n, d, steps=1, max_iters=5000):
         out[name] = {"s_per_step": secs / steps, "iterations_per_step": its / steps,
                      "inner_iterations_per_step": inner / steps, "iterations_per_s": its / secs,
                      "alg_GBs": alg / secs / 1e9, "alg_frac_of_peak": alg / secs / PEAK_HBM,
-                     "reference_equiv_GBs": ref / secs / 1e9, "final_summary": r["final_summary"]}
+                     "vs_reference_roofline": ref / PEAK_HBM / secs, "final_summary": r["final_summary"]}
         print("solvers leg %s: %.3f s per step, %d iterations" % (name, secs / steps, its // steps),
               file=sys.stderr, flush=True)
     return out
@@ -329,6 +391,9 @@ def main():
     if args.large:
         print(json.dumps({"metric": METRIC, "large": large_leg(args)}))
         return
+    if args.only_e10:
+        print(json.dumps({"metric": METRIC, "solve_to_1e-10": refine_leg(args.n)}))
+        return
     n = args.n
     # attainable streaming bandwidth on the same layout (copy, and the stream
     # mix of the dominant kernel without its stencil)
@@ -341,7 +406,8 @@ def main():
         # solve seconds to 1e-10 (one timestep, true residual reported)
         r10 = None if args.no_e10 else refine_leg(n)
         solvers = None if args.no_solvers else solvers_leg(n, d, max_iters=args.max_iters)
-        cpu = None if args.no_cpu else cpu_baseline(deck, args.cpu_iters)
+        dropin = None if args.no_dropin else dropin_leg(deck, args.mode)
+        cpu = None if args.no_cpu else cpu_baseline(deck, args.cpu_iters, d)
 
     cells = n * n
     its = r["iterations"]
@@ -379,8 +445,9 @@ def main():
         "solve_s_per_step": secs / args.steps,
         "hbm_GBs_per_iteration": per_iter_bytes * it_s / 1e9,
         "hbm_frac_per_iteration": per_iter_bytes * it_s / PEAK_HBM,
-        "reference_equiv_GBs": REF_BYTES_PER_ITER * cells * it_s / 1e9,
+        "vs_reference_roofline": REF_BYTES_PER_ITER * cells * it_s / PEAK_HBM,
         "solve_to_1e-10": r10,
+        "other_mode": dropin,
         "other_solvers": solvers,
         "roofline": roof,
         "cpu_baseline": cpu,

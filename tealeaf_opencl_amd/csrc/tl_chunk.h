@@ -53,6 +53,7 @@ struct tl_chunk {
     // profiler_on: per-kernel device time (ms) and calls (ocl_errors.cpp:64-121)
     std::map<std::string, std::pair<double, long>> prof;
     hipEvent_t pe0 = nullptr, pe1 = nullptr;
+    hipEvent_t pe_mid = nullptr;     // between a reduction kernel and its finisher
 
     // captured iteration batches of the native driver
     struct GraphSlot {
@@ -116,6 +117,10 @@ int hip_fail(const char *what, hipError_t e);
 // profiling wrapper bracket (no-ops unless profiler_on)
 void prof_begin(tl_chunk *c);
 void prof_end(tl_chunk *c, const char *name);
+// ... for a launch that records c->pe_mid between its two kernels: the time
+// up to pe_mid goes to `name`, the rest to `second`
+void prof_end_split(tl_chunk *c, const char *name, const char *second);
+void prof_add(tl_chunk *c, const char *name, double ms);
 int ensure_coef_capacity(tl_chunk *c, int n);
 int ensure_hist_capacity(tl_chunk *c, int n);
 int read_result(tl_chunk *c, double *out, int nq);

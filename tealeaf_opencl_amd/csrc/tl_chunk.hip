@@ -87,6 +87,25 @@ void prof_begin(tl_chunk *c)
     if (c->profiler_on) hipEventRecord(c->pe0, c->s);
 }
 
+void prof_add(tl_chunk *c, const char *name, double ms)
+{
+    auto &e = c->prof[name];
+    e.first += ms;
+    e.second += 1;
+}
+
+void prof_end_split(tl_chunk *c, const char *name, const char *second)
+{
+    if (!c->profiler_on) return;
+    hipEventRecord(c->pe1, c->s);
+    hipEventSynchronize(c->pe1);
+    float a = 0.f, b = 0.f;
+    hipEventElapsedTime(&a, c->pe0, c->pe_mid);
+    hipEventElapsedTime(&b, c->pe_mid, c->pe1);
+    prof_add(c, name, a);
+    prof_add(c, second, b);
+}
+
 void prof_end(tl_chunk *c, const char *name)
 {
     if (!c->profiler_on) return;
@@ -222,7 +241,7 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
     if (c->h_result) hipHostFree(c->h_result);
     if (c->h_flag) hipHostFree(c->h_flag);
     if (c->h_st) hipHostFree(c->h_st);
-    for (hipEvent_t ev : {c->pe0, c->pe1, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[0][3],
+    for (hipEvent_t ev : {c->pe0, c->pe1, c->pe_mid, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[0][3],
                           c->ev_t[1][0], c->ev_t[1][1], c->ev_t[1][2], c->ev_t[1][3], c->ev_batch[0],
                           c->ev_batch[1]})
         if (ev) hipEventDestroy(ev);
@@ -301,7 +320,7 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     }
     hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
     *c->h_flag = 0u;
-    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[0][3],
+    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->pe_mid, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[0][3],
                            &c->ev_t[1][0], &c->ev_t[1][1], &c->ev_t[1][2], &c->ev_t[1][3]})
         if ((e = hipEventCreate(ev)) != hipSuccess) {
             tl_chunk_destroy(c);

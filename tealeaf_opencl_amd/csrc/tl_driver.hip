@@ -63,6 +63,30 @@ const double kat_temp[7] = {0.0, 157.550841832793, 116.067951160930, 95.48651033
 // ---------------------------------------------------------------------------
 FILE *g_tout = nullptr;
 
+// ---------------------------------------------------------------------------
+// profiler_on: the Fortran driver's phase timers (host seconds), accumulated
+// where the reference accumulates them (timestep.f90:55, update_halo.f90:39-83,
+// field_summary.f90:77, set_field.f90:44, tea_solve.f90:143/352/387,
+// global_mpi.f90:48) and written by diffuse.f90:103-157 at the end of the
+// run. One chunk and no MPI: the halo exchange, internal halo update, dot
+// product (MPI_Allreduce) and visit timers stay at zero. The run in
+// progress owns the file-scope pointer (the library is not re-entrant).
+// ---------------------------------------------------------------------------
+struct FortranProfile {
+    double timestep = 0, halo_exchange = 0, halo_update = 0, internal_halo_update = 0, summary = 0, visit = 0,
+           tea_init = 0, dot_product = 0, tea_solve = 0, tea_reset = 0, set_field = 0;
+};
+FortranProfile *g_fprof = nullptr;
+
+double wtime() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+// host time of a device phase: the stream is drained first (profiler_on only)
+double phase_time(tl_chunk *c)
+{
+    if (g_fprof) hipStreamSynchronize(c->s);
+    return wtime();
+}
+
 std::string right(const std::string &s, int w)
 {
     if ((int)s.size() >= w) return (int)s.size() == w ? s : std::string(w, '*');
@@ -158,10 +182,62 @@ void tout_switch(int n, double rro, double eigmin, double eigmax, double cn, dou
          fort_F(cn, 14, 6) + " Error" + fort_E(old_error, 14, 6));
 }
 
+// tea_solve.f90:389-391 (profiler_on): seconds of the whole tea_leaf call and
+// the loop counter n at exit
+void tout_solve_time(double secs, int n)
+{
+    char buf[128];
+    snprintf(buf, sizeof buf, "%16s%16.10f%7s%7d%16s%16.10f", "Solve Time", secs, "Its", n, "Time Per It",
+             n ? secs / n : 0.0);
+    tout(buf);
+}
+
+// diffuse.f90:103-157 (one task: each total is its own maximum)
+void tout_profile(const FortranProfile &p, double wall_clock)
+{
+    const double total = p.timestep + p.halo_exchange + p.summary + p.visit + p.tea_init + p.set_field +
+                         p.tea_solve + p.tea_reset + p.dot_product + p.halo_update + p.internal_halo_update;
+    auto line = [&](const char *label, double t) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "%-23s%s%s", label, fort_F(t, 16, 4).c_str(),
+                 fort_F(100.0 * (t / wall_clock), 16, 4).c_str());
+        tout(buf);
+    };
+    tout("");
+    tout("Profiler Output                 Time            Percentage");
+    line("Timestep              :", p.timestep);
+    line("MPI Halo Exchange     :", p.halo_exchange);
+    line("Self Halo Update      :", p.halo_update);
+    line("Internal Halo Update  :", p.internal_halo_update);
+    line("Summary               :", p.summary);
+    line("Visit                 :", p.visit);
+    line("Tea Init              :", p.tea_init);
+    line("Dot Product           :", p.dot_product);
+    line("Tea Solve             :", p.tea_solve);
+    line("Tea Reset             :", p.tea_reset);
+    line("Set Field             :", p.set_field);
+    line("Total                 :", total);
+    char buf[128];
+    snprintf(buf, sizeof buf, "%-23s%s%s", "The Rest              :", fort_F(wall_clock - total, 16, 4).c_str(),
+             fort_F(100.0 * (wall_clock - total) / wall_clock, 16, 4).c_str());
+    tout(buf);
+}
+
+// update_halo (reflective walls) with its time in the Self Halo Update timer
+int halo_timed(tl_chunk *c, const int *nb, const int *fields, int depth)
+{
+    const double t0 = phase_time(c);
+    const int rc = tl_update_halo(c, nb, fields, depth);
+    if (g_fprof) g_fprof->halo_update += phase_time(c) - t0;
+    return rc;
+}
+
 void emit_summary(FILE *lg, tl_chunk *c, int step, double time, bool complete, const Config &cfg, double *out5)
 {
     double vol, mass, ie, temp;
+    const double t_sum = wtime();
     if (tl_field_summary(c, &vol, &mass, &ie, &temp) != TL_OK) return;
+    if (g_fprof) g_fprof->summary += wtime() - t_sum;
     if (out5) {
         out5[0] = vol; out5[1] = mass; out5[2] = mass / vol; out5[3] = ie; out5[4] = temp;
     }
@@ -184,6 +260,22 @@ void emit_summary(FILE *lg, tl_chunk *c, int step, double time, bool complete, c
     }
 }
 
+// tea_solve.f90:374-391: finalise and the energy halo; with profiler_on the
+// reset timer (halo time left out, as there) and the Solve Time line
+int reset_phase(tl_chunk *c, const Config &cfg, double t_total, int n_exit)
+{
+    const int nb[4] = {-1, -1, -1, -1};
+    const int fields[7] = {0, 0, 1, 0, 0, 0, 0};
+    const double t_reset = phase_time(c), h0 = g_fprof ? g_fprof->halo_update : 0.0;
+    RC(tl_finalise(c));
+    if (cfg.reflective_boundary) RC(halo_timed(c, nb, fields, 1));
+    if (g_fprof) {
+        g_fprof->tea_reset += phase_time(c) - t_reset - (g_fprof->halo_update - h0);
+        tout_solve_time(wtime() - t_total, n_exit);
+    }
+    return TL_OK;
+}
+
 // ---------------------------------------------------------------------------
 // drop-in mode: tea_solve.f90:37-420 over the reference-granularity calls
 // ---------------------------------------------------------------------------
@@ -202,8 +294,10 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
     bool ch_switch_check = false;
     double rx, ry;
 
+    const double t_total = wtime();
+    const double t_init = phase_time(c), h_init = g_fprof ? g_fprof->halo_update : 0.0;
     int fields[7] = {1, 0, 1, 0, 0, 0, 0};
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    if (cfg.reflective_boundary) RC(halo_timed(c, nb, fields, cfg.halo_depth));
     const int zb[4] = {1, 1, 1, 1};
     RC(tl_common_init(c, cfg.coefficient, dt, &rx, &ry, zb, cfg.reflective_boundary));
     RC(tl_calc_residual(c));
@@ -212,6 +306,8 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
     initial_residual = sqrt(initial_residual);
     if (lg && cfg.verbose_on) fprintf(lg, "INITRES %d %.17e\n", step, initial_residual);
     if (use_cg || use_ch || use_ppcg) RC(tl_cg_init(c, &rro));
+    if (g_fprof) g_fprof->tea_init += phase_time(c) - t_init - (g_fprof->halo_update - h_init);
+    const double t_solve = phase_time(c);
 
     int n;
     for (n = 1; n <= max_iters; n++) {
@@ -307,6 +403,7 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(tl_calc_2norm(c, 1, &exact_error));
         exact_error = sqrt(exact_error);
     }
+    if (g_fprof) g_fprof->tea_solve += phase_time(c) - t_solve;
     if (lg)
         fprintf(lg, "SOLVE %d %d %.17e %.17e %d %d %d %d %d\n", step, n - 1, error / initial_residual,
                 cfg.tl_check_result ? exact_error / initial_residual : NAN, ppcg_inner_iters, cg_calc_steps,
@@ -317,11 +414,7 @@ int solve_dropin(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         stats->iterations += n > max_iters ? max_iters : n;
         stats->inner += ppcg_inner_iters;
     }
-    RC(tl_finalise(c));
-    fields[0] = 0;
-    fields[2] = 1;
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, 1));
-    return TL_OK;
+    return reset_phase(c, cfg, t_total, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -370,21 +463,60 @@ int capture(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<v
     return TL_OK;
 }
 
+// One launch of the device-resident driver. With prof (profiler_on: the
+// batches are then launched eagerly instead of replayed) it is bracketed by
+// events and its device time goes to the chunk's per-kernel table under the
+// HIP kernel's name, its finisher's (fin) under k_finish, as the reference
+// times every OpenCL kernel it enqueues (ocl_errors.cpp:64-121).
+template <typename F>
+void launch_timed(tl_chunk *c, bool prof, const char *name, bool fin, F &&launch)
+{
+    if (!prof) {
+        launch(nullptr);
+        return;
+    }
+    prof_begin(c);
+    launch(fin ? c->pe_mid : nullptr);
+    if (fin) prof_end_split(c, name, "k_finish");
+    else prof_end(c, name);
+}
+
 // One CG iteration of the device-resident driver: the fused search-direction
 // update + matvec + p.w, then the fused u/r update + preconditioner + r.z.
 // With `ev`, ev[0..1] bracket the first iteration kernel and ev[2..3] the
 // second (their finishers are outside the brackets).
-void cg_iteration(tl_chunk *c, hipEvent_t *ev)
+void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false)
 {
     const Ext in = ext_interior(c);
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
     if (ev) hipEventRecord(ev[0], c->s);
-    launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
-                      red_args(c, RED_CG_PW, true), ev ? ev[1] : nullptr);
+    launch_timed(c, prof, "k_fused_cg_w", true, [&](hipEvent_t mid) {
+        launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
+                          red_args(c, RED_CG_PW, true), ev ? ev[1] : mid);
+    });
     if (ev) hipEventRecord(ev[2], c->s);
-    launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                       c->f2[F_R], c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_CG_RRN, true),
-                       ev ? ev[3] : nullptr);
+    launch_timed(c, prof, "k_fused_cg_ur", true, [&](hipEvent_t mid) {
+        launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
+                           c->f2[F_R], c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces,
+                           red_args(c, RED_CG_RRN, true), ev ? ev[3] : mid);
+    });
+}
+
+// profiler_on: the iterations of a phase launched one `body` at a time, each
+// followed by a read of the device state, until it halts (so every kernel's
+// call count is the number of iterations that ran)
+int run_eager(tl_chunk *c, const std::function<void(long)> &body, long max_calls)
+{
+    SolveState h;
+    for (long k = 0;; ++k) {
+        body(k);
+        RC(state_download(c, &h));
+        if (h.halt) return TL_OK;
+        if (k > max_calls) {
+            set_error("solver state did not halt after %ld eager iterations", k);
+            return TL_ERR_SOLVER;
+        }
+    }
 }
 
 // Replays `exec` until the device state says halt; batch i+1 is queued before
@@ -450,23 +582,31 @@ int cg_graph(tl_chunk *c, bool timed)
 // norm of tea_solve.f90:223 (iterations divisible by 10) always falls on the
 // second iteration of a pair. Pairs flip u and p between their two buffers
 // at every launch (8 flips per replay).
+// Chebyshev launch k of a phase: one iteration, or (pairs) two per launch
+// with u and p flipping between their two buffers at every launch
+void cheby_launch(tl_chunk *c, bool pairs, long k, bool prof)
+{
+    const Ext in = ext_interior(c);
+    if (pairs) {
+        const bool odd = k & 1;
+        launch_timed(c, prof, "k_fused_cheby2", true, [&](hipEvent_t mid) {
+            launch_fused_cheby2(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[odd ? F_U2 : F_U],
+                                c->f2[odd ? F_U : F_U2], c->f2[odd ? F_P2 : F_P], c->f2[odd ? F_P : F_P2],
+                                c->f2[F_KX], c->f2[F_KY], c->ch_a, c->ch_b, red_args(c, RED_CHEBY2, true), mid);
+        });
+        return;
+    }
+    launch_timed(c, prof, "k_fused_cheby", true, [&](hipEvent_t mid) {
+        launch_fused_cheby(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[F_U], c->f2[F_U2], c->f2[F_P],
+                           c->f2[F_COND], c->faces, c->ch_a, c->ch_b, red_args(c, RED_CHEBY, true), mid);
+    });
+}
+
 int cheby_graph(tl_chunk *c, bool pairs)
 {
     const int B = 16;
-    const Ext in = ext_interior(c);
     return capture(c, c->g_cheby, c->prec * 4 + (pairs ? 2 : 1), [&]() {
-        if (pairs) {
-            for (int b = 0; b < B / 2; ++b) {
-                const bool odd = b & 1;
-                launch_fused_cheby2(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[odd ? F_U2 : F_U],
-                                    c->f2[odd ? F_U : F_U2], c->f2[odd ? F_P2 : F_P], c->f2[odd ? F_P : F_P2],
-                                    c->f2[F_KX], c->f2[F_KY], c->ch_a, c->ch_b, red_args(c, RED_CHEBY2, true));
-            }
-            return;
-        }
-        for (int b = 0; b < B; ++b)
-            launch_fused_cheby(c->s, c->g, in, c->cap, c->prec, c->f2[F_U0], c->f2[F_U], c->f2[F_U2], c->f2[F_P],
-                               c->f2[F_COND], c->faces, c->ch_a, c->ch_b, red_args(c, RED_CHEBY, true));
+        for (int b = 0; b < (pairs ? B / 2 : B); ++b) cheby_launch(c, pairs, b, false);
     });
 }
 
@@ -476,38 +616,53 @@ int cheby_graph(tl_chunk *c, bool pairs)
 // step's update writes r to the second buffer when the number of pairs is
 // odd, so that every outer step starts and ends with r in F_R. An odd inner
 // count ends with one single-step launch (in place for r).
-int ppcg_graph(tl_chunk *c, int inner)
+void ppcg_outer(tl_chunk *c, int inner, bool prof)
 {
-    const int B = std::max(1, 32 / (inner + 2));
     const Ext in = ext_interior(c);
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
     const int per = c->ppcg_steps_per_launch;
     const int pairs = per == 2 ? inner / 2 : 0;
     const int last = c->ppcg_fresh_z ? 2 : 1;  // the last step's r.z: fresh or stale z
+    launch_timed(c, prof, "k_fused_cg_w", true, [&](hipEvent_t mid) {
+        launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
+                          red_args(c, RED_PPCG_PW, true), mid);
+    });
+    double *r_cur = (pairs & 1) ? c->f2[F_R2] : c->f2[F_R];
+    double *r_oth = (pairs & 1) ? c->f2[F_R] : c->f2[F_R2];
+    launch_timed(c, prof, "k_fused_cg_ur_ppcg", false, [&](hipEvent_t) {
+        launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 1, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
+                           r_cur, c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_STORE, true));
+    });
+    double *sd_cur = c->f2[F_SD], *sd_oth = c->f2[F_SD2];
+    int i = 1;
+    for (int k = 0; k < pairs; ++k, i += 2) {
+        const int lst = i + 1 == inner ? last : 0;
+        launch_timed(c, prof, "k_fused_ppcg_inner2", lst != 0, [&](hipEvent_t mid) {
+            launch_fused_ppcg_inner2(c->s, c->g, in, c->cap, c->prec, lst, c->f2[F_U], r_cur, r_oth, sd_cur, sd_oth,
+                                     c->f2[F_KX], c->f2[F_KY], c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
+                                     red_args(c, RED_PPCG_RRN, true), mid);
+        });
+        std::swap(r_cur, r_oth);
+        std::swap(sd_cur, sd_oth);
+    }
+    for (; i <= inner; ++i) {
+        const int lst = i == inner ? last : 0;
+        launch_timed(c, prof, "k_fused_ppcg_inner", lst != 0, [&](hipEvent_t mid) {
+            launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, lst, c->f2[F_U], r_cur, sd_cur, sd_oth,
+                                    c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b, i - 1,
+                                    red_args(c, RED_PPCG_RRN, true), mid);
+        });
+        std::swap(sd_cur, sd_oth);
+    }
+}
+
+int ppcg_graph(tl_chunk *c, int inner)
+{
+    const int B = std::max(1, 32 / (inner + 2));
+    const int per = c->ppcg_steps_per_launch;
+    const int last = c->ppcg_fresh_z ? 2 : 1;
     return capture(c, c->g_ppcg, ((c->prec * 100000 + inner) * 4 + per) * 2 + (last - 1), [&]() {
-        for (int b = 0; b < B; ++b) {
-            launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
-                              red_args(c, RED_PPCG_PW, true));
-            double *r_cur = (pairs & 1) ? c->f2[F_R2] : c->f2[F_R];
-            double *r_oth = (pairs & 1) ? c->f2[F_R] : c->f2[F_R2];
-            launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 1, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
-                               r_cur, c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces, red_args(c, RED_STORE, true));
-            double *sd_cur = c->f2[F_SD], *sd_oth = c->f2[F_SD2];
-            int i = 1;
-            for (int k = 0; k < pairs; ++k, i += 2) {
-                launch_fused_ppcg_inner2(c->s, c->g, in, c->cap, c->prec, i + 1 == inner ? last : 0, c->f2[F_U],
-                                         r_cur, r_oth, sd_cur, sd_oth, c->f2[F_KX], c->f2[F_KY], c->f2[F_Z],
-                                         c->ch_a, c->ch_b, i - 1, red_args(c, RED_PPCG_RRN, true));
-                std::swap(r_cur, r_oth);
-                std::swap(sd_cur, sd_oth);
-            }
-            for (; i <= inner; ++i) {
-                launch_fused_ppcg_inner(c->s, c->g, in, c->cap, c->prec, i == inner ? last : 0, c->f2[F_U], r_cur,
-                                        sd_cur, sd_oth, c->f2[F_COND], c->faces, c->f2[F_Z], c->ch_a, c->ch_b,
-                                        i - 1, red_args(c, RED_PPCG_RRN, true));
-                std::swap(sd_cur, sd_oth);
-            }
-        }
+        for (int b = 0; b < B; ++b) ppcg_outer(c, inner, false);
     });
 }
 
@@ -548,14 +703,19 @@ extern "C" int tl_ppcg_inner_fused(tl_chunk *c, int first_step, int nsteps)
     return e == hipSuccess ? TL_OK : hip_fail("tl_ppcg_inner_fused", e);
 }
 
+void jacobi_launch(tl_chunk *c, bool prof)
+{
+    launch_timed(c, prof, "k_jacobi", true, [&](hipEvent_t mid) {
+        launch_fused_jacobi(c->s, c->g, ext_interior(c), c->cap, c->f2[F_COND], c->faces, c->f2[F_U0], c->f2[F_U],
+                            c->f2[F_U2], red_args(c, RED_JACOBI, true), mid);
+    });
+}
+
 int jacobi_graph(tl_chunk *c)
 {
     const int B = 16;
-    const Ext in = ext_interior(c);
     return capture(c, c->g_jacobi, 1, [&]() {
-        for (int b = 0; b < B; ++b)
-            launch_fused_jacobi(c->s, c->g, in, c->cap, c->f2[F_COND], c->faces, c->f2[F_U0], c->f2[F_U],
-                                c->f2[F_U2], red_args(c, RED_JACOBI, true));
+        for (int b = 0; b < B; ++b) jacobi_launch(c, false);
     });
 }
 
@@ -564,8 +724,12 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
     const int nb[4] = {-1, -1, -1, -1};
     const bool use_ch = cfg.solver == SOLVER_CHEBY, use_ppcg = cfg.solver == SOLVER_PPCG;
     const bool use_jac = cfg.solver == SOLVER_JACOBI;
-    int fields[7] = {1, 0, 1, 0, 0, 0, 0};
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    const double t_total = wtime();
+    const double t_init = phase_time(c), h_init = g_fprof ? g_fprof->halo_update : 0.0;
+    // profiler_on: every kernel of the phase launched eagerly and timed
+    const bool prof = c->profiler_on != 0;
+    const int fields[7] = {1, 0, 1, 0, 0, 0, 0};
+    if (cfg.reflective_boundary) RC(halo_timed(c, nb, fields, cfg.halo_depth));
     const int zb[4] = {1, 1, 1, 1};
     double rx, ry;
     RC(tl_common_init(c, cfg.coefficient, dt, &rx, &ry, zb, cfg.reflective_boundary));
@@ -583,7 +747,9 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
     RC(state_upload(c, h));
 
     RC(tl_calc_residual(c));
-    launch_dot(c->s, c->g, ext_interior(c), c->cap, c->f2[F_R], c->f2[F_R], red_args(c, RED_INIT_RESID, true));
+    launch_timed(c, prof, "k_dot", false, [&](hipEvent_t) {
+        launch_dot(c->s, c->g, ext_interior(c), c->cap, c->f2[F_R], c->f2[F_R], red_args(c, RED_INIT_RESID, true));
+    });
     if (!use_jac) RC(tl_cg_init_device(c));
     if (cfg.max_iters < 1) {
         RC(state_download(c, &h));
@@ -592,21 +758,31 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(state_upload(c, h));
     }
 
-    const bool timed = c->timing;
+    const bool timed = c->timing && !c->profiler_on;
     RC(state_download(c, &h));
+    if (g_fprof) g_fprof->tea_init += phase_time(c) - t_init - (g_fprof->halo_update - h_init);
+    const double t_solve = phase_time(c);
     if (!h.halt) {
-        if (use_jac) {
+        if (use_jac && prof) {
+            RC(run_eager(c, [&](long) { jacobi_launch(c, true); }, cfg.max_iters + 4));
+        } else if (use_jac) {
             RC(jacobi_graph(c));
             RC(run_batches(c, c->g_jacobi.exec, false, cfg.max_iters / 16 + 4));
         } else {
-            RC(cg_graph(c, timed));
-            RC(run_batches(c, c->g_cg.exec, timed, cfg.max_iters / 15 + 4));
+            if (prof) {
+                RC(run_eager(c, [&](long) { cg_iteration(c, nullptr, true); }, cfg.max_iters + 4));
+            } else {
+                RC(cg_graph(c, timed));
+                RC(run_batches(c, c->g_cg.exec, timed, cfg.max_iters / 15 + 4));
+            }
             RC(state_download(c, &h));
             // the fused CG kernels add alpha p into u on even iterations only
             // (two at a time): complete u when the CG phase ended on an odd one
             const int last = h.done ? std::min(h.exit_n, cfg.max_iters) : (h.switch_n > 0 ? h.switch_n - 1 : 0);
             if (last >= 1 && (last & 1))
-                launch_cg_u_fix(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U], c->f2[F_P2], c->st);
+                launch_timed(c, prof, "k_cg_u_fix", false, [&](hipEvent_t) {
+                    launch_cg_u_fix(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U], c->f2[F_P2], c->st);
+                });
         }
         RC(state_download(c, &h));
     }
@@ -675,8 +851,13 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
             RC(state_upload(c, h));
             if (!h.halt) {
                 cheby_pairs = c->cheby_steps_per_launch == 2 && (h.n_fused0 & 1);
-                RC(cheby_graph(c, cheby_pairs));
-                RC(run_batches(c, c->g_cheby.exec, false, cfg.max_iters / 16 + 4));
+                if (prof) {
+                    const bool pairs = cheby_pairs;
+                    RC(run_eager(c, [&](long k) { cheby_launch(c, pairs, k, true); }, cfg.max_iters + 4));
+                } else {
+                    RC(cheby_graph(c, cheby_pairs));
+                    RC(run_batches(c, c->g_cheby.exec, false, cfg.max_iters / 16 + 4));
+                }
             }
         } else {
             RC(tl_ppcg_init(c, ca.data(), cb.data(), sv.theta, cfg.tl_ppcg_inner_steps));
@@ -684,8 +865,12 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
             h.theta = sv.theta;
             h.halt = 0;
             RC(state_upload(c, h));
-            RC(ppcg_graph(c, cfg.tl_ppcg_inner_steps));
-            RC(run_batches(c, c->g_ppcg.exec, false, cfg.max_iters + 4));
+            if (prof) {
+                RC(run_eager(c, [&](long) { ppcg_outer(c, cfg.tl_ppcg_inner_steps, true); }, cfg.max_iters + 4));
+            } else {
+                RC(ppcg_graph(c, cfg.tl_ppcg_inner_steps));
+                RC(run_batches(c, c->g_ppcg.exec, false, cfg.max_iters + 4));
+            }
         }
         RC(state_download(c, &h));
     }
@@ -704,6 +889,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         RC(tl_calc_2norm(c, 1, &exact));
         exact = sqrt(exact) / h.initial_residual;
     }
+    if (g_fprof) g_fprof->tea_solve += phase_time(c) - t_solve;
     const int ppcg_inner = use_ppcg ? h.cheby_calc_steps * cfg.tl_ppcg_inner_steps : 0;
     if (lg && cfg.verbose_on) {
         fprintf(lg, "INITRES %d %.17e\n", step, h.initial_residual);
@@ -720,11 +906,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         stats->iterations += executed;
         stats->inner += ppcg_inner;
     }
-    RC(tl_finalise(c));
-    fields[0] = 0;
-    fields[2] = 1;
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, 1));
-    return TL_OK;
+    return reset_phase(c, cfg, t_total, h.exit_n);
 }
 
 // One timestep with the refined solve (tl_solve_refined, or
@@ -737,11 +919,15 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
 int solve_refined(tl_chunk *c, const Config &cfg, double dt, int step, FILE *lg, Stats *stats)
 {
     const int nb[4] = {-1, -1, -1, -1};
-    int fields[7] = {1, 0, 1, 0, 0, 0, 0};
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    const double t_total = wtime();
+    const double t_init = phase_time(c), h_init = g_fprof ? g_fprof->halo_update : 0.0;
+    const int fields[7] = {1, 0, 1, 0, 0, 0, 0};
+    if (cfg.reflective_boundary) RC(halo_timed(c, nb, fields, cfg.halo_depth));
     const int zb[4] = {1, 1, 1, 1};
     double rx, ry;
     RC(tl_common_init(c, cfg.coefficient, dt, &rx, &ry, zb, cfg.reflective_boundary));
+    if (g_fprof) g_fprof->tea_init += phase_time(c) - t_init - (g_fprof->halo_update - h_init);
+    const double t_solve = phase_time(c);
     const int precision = cfg.refine == 2 ? TL_PRECISION_MIXED : TL_PRECISION_FP64;
     const double delta = cfg.refine_delta >= 0.0 ? cfg.refine_delta : (precision == TL_PRECISION_MIXED ? 0.1 : 0.0);
     tl_refine_result rr;
@@ -750,6 +936,7 @@ int solve_refined(tl_chunk *c, const Config &cfg, double dt, int step, FILE *lg,
                                  cfg.tl_ppcg_inner_steps, &rr));
     else
         RC(tl_solve_refined(c, precision, cfg.eps, cfg.max_iters, delta, &rr));
+    if (g_fprof) g_fprof->tea_solve += phase_time(c) - t_solve;
     if (lg) {
         fprintf(lg, "SOLVE %d %d %.17e %.17e %d %d %d %d %d\n", step, rr.iterations, rr.rel_residual, rr.rel_residual,
                 rr.inner_iterations, rr.iterations, 0, 0, 0);
@@ -761,11 +948,7 @@ int solve_refined(tl_chunk *c, const Config &cfg, double dt, int step, FILE *lg,
         stats->iterations += rr.iterations;
         stats->inner += rr.inner_iterations;
     }
-    RC(tl_finalise(c));
-    fields[0] = 0;
-    fields[2] = 1;
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, 1));
-    return TL_OK;
+    return reset_phase(c, cfg, t_total, rr.iterations);
 }
 
 // start.f90:24-115
@@ -778,8 +961,10 @@ int start(tl_chunk *c, const Config &cfg)
                          cfg.symin.data(), cfg.symax.data(), cfg.radius.data(), cfg.geometry.data(), 1, 2, 3));
     const int nb[4] = {-1, -1, -1, -1};
     const int fields[7] = {1, 1, 1, 0, 0, 0, 0};
-    if (cfg.reflective_boundary) RC(tl_update_halo(c, nb, fields, cfg.halo_depth));
+    if (cfg.reflective_boundary) RC(halo_timed(c, nb, fields, cfg.halo_depth));
+    const double t_sf = phase_time(c);
     RC(tl_set_field(c));
+    if (g_fprof) g_fprof->set_field += phase_time(c) - t_sf;
     return TL_OK;
 }
 
@@ -804,6 +989,8 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
     c->solver = cfg.sc.solver;
     c->ppcg_fresh_z = cfg.ppcg_fresh_z != 0;
     c->timing = (warmup >= 0);
+    FortranProfile fprof;
+    g_fprof = cfg.profiler_on ? &fprof : nullptr;
     int rc = start(c, cfg);
     if (rc == TL_OK) emit_summary(lg, c, 0, 0.0, false, cfg, nullptr);
     Save sv;
@@ -812,6 +999,7 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
     int step = 0;
     std::chrono::steady_clock::time_point t0;
     const auto wall0 = std::chrono::steady_clock::now();
+    double wall_clock = 0.0;
     Stats stats;
     while (rc == TL_OK) {
         step++;
@@ -822,7 +1010,9 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
             stats = Stats();
             t0 = std::chrono::steady_clock::now();
         }
+        const double t_dt = wtime();
         if (cfg.dtinit <= dt) dt = cfg.dtinit;
+        if (g_fprof) g_fprof->timestep += wtime() - t_dt;
         // timestep.f90:58 (the format's trailing i8 has no item)
         tout(" Step " + fort_I(step, 7) + " time " + fort_F(time, 11, 7) + " timestep  " + fort_1PE(dt, 9, 2));
         rc = cfg.refine ? solve_refined(c, cfg, dt, step, lg, &stats)
@@ -830,10 +1020,10 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
                                         : solve_dropin(c, cfg, sv, dt, step, lg, &stats);
         if (rc != TL_OK) break;
         time = time + dt;
+        wall_clock = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
         if (g_tout) {
             // diffuse.f90:66-69 (list-directed)
-            const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
-            tout(" Wall clock " + fort_list(wall));
+            tout(" Wall clock " + fort_list(wall_clock));
             fflush(g_tout);
         }
         if (lg) fflush(lg);
@@ -851,6 +1041,8 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
             break;
         }
     }
+    if (rc == TL_OK && g_fprof) tout_profile(fprof, wall_clock);
+    g_fprof = nullptr;
     if (rc == TL_OK && cfg.sc.profiler_on) tl_print_profiling(c);
     if (out) {
         out->k1_ms = c->t_k1_ms;

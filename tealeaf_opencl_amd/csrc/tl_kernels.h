@@ -96,27 +96,29 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
 void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
                      const SolveState *st);
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
-                         const double *u0, double *uA, double *uB, const RedArgs &ra);
+                         const double *u0, double *uA, double *uB, const RedArgs &ra,
+                         hipEvent_t ev_main = nullptr);
 // last: 0 a smoothing step, 1 the last one forming r.z with the outer step's
 // z (the reference's stale z), 2 the last one storing the fresh z = M^-1 r and
 // forming r.z with it (the fresh-z option)
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
                              const double *W, const Faces &fc, double *z, const double *ch_a,
-                             const double *ch_b, int step, const RedArgs &ra);
+                             const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main = nullptr);
 // two PPCG inner steps (`step`, `step`+1) per launch; r and sd are read from
 // one buffer and written to another
 void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last, double *u,
                               const double *r_in, double *r_out, const double *sd_in, double *sd_out,
                               const double *Kx, const double *Ky, double *z, const double *ch_a,
-                              const double *ch_b, int step, const RedArgs &ra);
+                              const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main = nullptr);
 // two Chebyshev iterations (n odd) per launch; u and p are read from one
 // buffer and written to the other
 void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                          const double *u_in, double *u_out, const double *p_in, double *p_out, const double *Kx,
-                         const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra);
+                         const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra,
+                         hipEvent_t ev_main = nullptr);
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                         double *uA, double *uB, double *p, const double *W, const Faces &fc,
-                        const double *ch_a, const double *ch_b, const RedArgs &ra);
+                        const double *ch_a, const double *ch_b, const RedArgs &ra, hipEvent_t ev_main = nullptr);
 
 }  // namespace tl

@@ -1670,9 +1670,10 @@ void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double
 }
 
 void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *W, const Faces &fc,
-                         const double *u0, double *uA, double *uB, const RedArgs &ra)
+                         const double *u0, double *uA, double *uB, const RedArgs &ra,
+                         hipEvent_t ev_main)
 {
-    TL_LAUNCH_RED(k_jacobi<1>, e, cap, s, 1, ra, g, e, nullptr, nullptr, W, fc, u0, uA, uB, ra);
+    TL_LAUNCH_RED_EV(k_jacobi<1>, e, cap, s, 1, ra, ev_main, g, e, nullptr, nullptr, W, fc, u0, uA, uB, ra);
 }
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
@@ -1700,11 +1701,11 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
 void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last,
                              double *u, double *r, const double *sd_in, double *sd_out,
                              const double *W, const Faces &fc, double *z, const double *ch_a,
-                             const double *ch_b, int step, const RedArgs &ra)
+                             const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main)
 {
 #define TL_FPI(P, L)                                                                                         \
     do {                                                                                                     \
-        if (L) TL_LAUNCH_RED((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, g, e, u, r, sd_in, sd_out, W, fc, z, \
+        if (L) TL_LAUNCH_RED_EV((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, ev_main, g, e, u, r, sd_in, sd_out, W, fc, z, \
                              ch_a, ch_b, step, ra);                                                          \
         else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, W, fc, z, ch_a, ch_b,  \
                        step, ra);                                                                            \
@@ -1728,11 +1729,11 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
 void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int last, double *u,
                               const double *r_in, double *r_out, const double *sd_in, double *sd_out,
                               const double *Kx, const double *Ky, double *z, const double *ch_a,
-                              const double *ch_b, int step, const RedArgs &ra)
+                              const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main)
 {
 #define TL_FPI2(P, L)                                                                                           \
     do {                                                                                                        \
-        if (L) TL_LAUNCH_RED((k_fused_ppcg_inner2<P, L>), e, cap, s, 1, ra, g, e, u, r_in, r_out, sd_in, sd_out, \
+        if (L) TL_LAUNCH_RED_EV((k_fused_ppcg_inner2<P, L>), e, cap, s, 1, ra, ev_main, g, e, u, r_in, r_out, sd_in, sd_out, \
                              Kx, Ky, z, ch_a, ch_b, step, ra);                                                  \
         else TL_LAUNCH((k_fused_ppcg_inner2<P, L>), e, cap, s, g, e, u, r_in, r_out, sd_in, sd_out, Kx, Ky, z,   \
                        ch_a, ch_b, step, ra);                                                                   \
@@ -1755,29 +1756,30 @@ void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int ca
 
 void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                          const double *u_in, double *u_out, const double *p_in, double *p_out, const double *Kx,
-                         const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra)
+                         const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra,
+                         hipEvent_t ev_main)
 {
     if (prec == PREC_BLOCK)
-        TL_LAUNCH_RED(k_fused_cheby2<PREC_BLOCK>, e, cap, s, 1, ra, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
+        TL_LAUNCH_RED_EV(k_fused_cheby2<PREC_BLOCK>, e, cap, s, 1, ra, ev_main, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
                       ch_b, ra);
     else if (prec == PREC_DIAG)
-        TL_LAUNCH_RED(k_fused_cheby2<PREC_DIAG>, e, cap, s, 1, ra, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
+        TL_LAUNCH_RED_EV(k_fused_cheby2<PREC_DIAG>, e, cap, s, 1, ra, ev_main, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
                       ch_b, ra);
     else
-        TL_LAUNCH_RED(k_fused_cheby2<PREC_NONE>, e, cap, s, 1, ra, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
+        TL_LAUNCH_RED_EV(k_fused_cheby2<PREC_NONE>, e, cap, s, 1, ra, ev_main, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
                       ch_b, ra);
 }
 
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                         double *uA, double *uB, double *p, const double *W, const Faces &fc,
-                        const double *ch_a, const double *ch_b, const RedArgs &ra)
+                        const double *ch_a, const double *ch_b, const RedArgs &ra, hipEvent_t ev_main)
 {
     if (prec == PREC_BLOCK)
-        TL_LAUNCH_RED(k_fused_cheby<PREC_BLOCK>, e, cap, s, 1, ra, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
+        TL_LAUNCH_RED_EV(k_fused_cheby<PREC_BLOCK>, e, cap, s, 1, ra, ev_main, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
     else if (prec == PREC_DIAG)
-        TL_LAUNCH_RED(k_fused_cheby<PREC_DIAG>, e, cap, s, 1, ra, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
+        TL_LAUNCH_RED_EV(k_fused_cheby<PREC_DIAG>, e, cap, s, 1, ra, ev_main, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
     else
-        TL_LAUNCH_RED(k_fused_cheby<PREC_NONE>, e, cap, s, 1, ra, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
+        TL_LAUNCH_RED_EV(k_fused_cheby<PREC_NONE>, e, cap, s, 1, ra, ev_main, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
 }
 
 }  // namespace tl

@@ -120,6 +120,40 @@ def test_finite_differences_config4_full_size(hiplib):
     assert max(errs) <= 1e-6, (np.median(errs), max(errs))
 
 
+def test_config4_full_size_vs_cpu_reference(hiplib):
+    """The adjoint workload at 1024x1024 over 5 timesteps against the CPU
+    reference adjoint of the same problem (fixture
+    tests/golden/config4_1024_adjoint.json, written by
+    tests/golden/make_config4_adjoint.py: every solve the oracle's FP64 CG, the
+    face terms scattered in numpy): the loss to 1e-10 relative, dL/drho at the
+    32 sensitive cells and 2000 seeded cells to 1e-8 of the gradient's
+    largest component, and the interior sum of dL/drho. The GPU side forms
+    its own u_obs (refined FP64 solves) from the seed-2 density."""
+    import json
+    import os
+    from tealeaf_opencl_amd import adjoint
+    path = os.path.join(os.path.dirname(__file__), "golden", "config4_1024_adjoint.json")
+    with open(path) as f:
+        fx = json.load(f)
+    n, nsteps, dt, coef = fx["n"], fx["nsteps"], fx["dt"], fx["coefficient"]
+    c = hiplib.Chunk(n, n, 1, fx["preconditioner"])
+    rho, e0, u_obs = adjoint.setup_problem(c, seed=fx["seed"], obs_seed=fx["obs_seed"], dt=dt, nsteps=nsteps,
+                                           tol=1e-14)
+    assert abs(float(np.sum(u_obs[1:-1, 1:-1])) - fx["u_obs_sum"]) <= 1e-12 * abs(fx["u_obs_sum"])
+    L, grad, _ = c.adjoint_density(coef, dt, nsteps, u_obs, tol=1e-14, checkpoint_every=2)
+    scale = fx["grad_max_abs"]
+    assert abs(np.max(np.abs(grad[1:-1, 1:-1])) - scale) <= 1e-8 * scale
+    errs = [abs(grad[k, j] - v) / scale for k, j, v in fx["sensitive"] + fx["sample"]]
+    rel_sens = [abs(grad[k, j] - v) / abs(v) for k, j, v in fx["sensitive"]]
+    sum_err = abs(float(np.sum(grad[1:-1, 1:-1])) - fx["grad_sum"]) / fx["grad_abs_sum"]
+    print("config4 vs CPU reference: L %.2e, components %.2e (of max), sensitive %.2e (own), sum %.2e"
+          % (abs(L - fx["loss"]) / fx["loss"], max(errs), max(rel_sens), sum_err))
+    assert abs(L - fx["loss"]) <= 1e-10 * fx["loss"]
+    assert max(errs) <= 1e-8
+    assert max(rel_sens) <= 1e-6
+    assert sum_err <= 1e-8
+
+
 def test_torch_wrapper(hiplib):
     torch = pytest.importorskip("torch")
     from tealeaf_opencl_amd import adjoint

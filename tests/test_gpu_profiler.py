@@ -1,0 +1,141 @@
+"""Profiler parity (profiler_on): the per-kernel table the reference prints at
+the end of a run (print_opencl_profiling_info_, ocl_errors.cpp:196-232; each
+enqueued kernel timed with events, ocl_errors.cpp:64-121) and the Fortran
+driver's "Profiler Output" block in tea.out (diffuse.f90:103-157, phase timers
+of timestep.f90, update_halo.f90, field_summary.f90, set_field.f90 and
+tea_solve.f90), together with tea_solve.f90:389-391's "Solve Time" line.
+
+Through the reference's own Fortran driver (drop-in: the *_kernel_ocl_ entry
+points time themselves under the reference's kernel names), and through the
+library's device-resident driver, which with profiler_on launches its fused
+kernels eagerly, one iteration at a time, each bracketed by events, under the
+HIP kernels' names. Call counts must equal the iterations that ran, and the
+profiled native run must give bitwise the field summaries of the unprofiled
+one (the same kernels in the same order, launched instead of replayed)."""
+import os
+import re
+
+import pytest
+
+import _fortran_driver as fd
+from tealeaf_opencl_amd import decks
+
+pytestmark = pytest.mark.gpu
+
+ROW = re.compile(r"^\s*(\S+) :\s+([\d.]+) ms \(([\d.]+) \S+ avg\. over (\d+) calls\)$")
+LABELS = ["Timestep", "MPI Halo Exchange", "Self Halo Update", "Internal Halo Update", "Summary", "Visit",
+          "Tea Init", "Dot Product", "Tea Solve", "Tea Reset", "Set Field", "Total", "The Rest"]
+
+
+def kernel_table(text, banner):
+    lines = text.splitlines()
+    i = next(i for i, l in enumerate(lines) if l.strip() == banner)
+    table = {}
+    for l in lines[i + 1:]:
+        m = ROW.match(l)
+        if not m:
+            break
+        table[m.group(1)] = (float(m.group(2)), int(m.group(4)))
+    return table
+
+
+def profile_block(tea_out):
+    """{label: (seconds, percent)} of the Profiler Output block, in order."""
+    lines = tea_out.splitlines()
+    i = next(i for i, l in enumerate(lines) if l.startswith("Profiler Output"))
+    assert lines[i] == "Profiler Output                 Time            Percentage"
+    out = []
+    for l in lines[i + 1:i + 1 + len(LABELS)]:
+        assert len(l) == 55 and l[22] == ":", l
+        out.append((l[:23].rstrip(" :").strip(), float(l[23:39]), float(l[39:55])))
+    return out
+
+
+def solve_time_lines(tea_out):
+    rx = re.compile(r"^ {6}Solve Time[ \d.]{16} {4}Its[ \d]{7} {5}Time Per It[ \d.]{16}$")
+    rows = [l for l in tea_out.splitlines() if "Solve Time" in l]
+    assert rows and all(rx.match(l) for l in rows), rows[:2]
+    return [int(l[39:46]) for l in rows]
+
+
+def deck(path, solver, prec, profiler=True, extra=()):
+    ex = list(extra) + (["profiler_on"] if profiler else [])
+    if solver == "tl_use_chebyshev":
+        ex += ["tl_ch_cg_presteps=20", "tl_ch_cg_epslim=1.0e300"]
+    if solver == "tl_use_ppcg":
+        ex += ["tl_ppcg_inner_steps=7"]
+    return decks.write_deck(path, 200, 148, decks.seeded_states(2, 2, True), solver=solver, preconditioner=prec,
+                            end_step=3, eps=1e-12, max_iters=4000, extra=ex)
+
+
+@pytest.mark.parametrize("solver,prec", [("tl_use_cg", "jac_block"), ("tl_use_chebyshev", "jac_diag"),
+                                         ("tl_use_ppcg", "jac_block"), ("tl_use_jacobi", "none")])
+def test_native_profiler(hiplib, tmp_path, capfd, solver, prec):
+    plain = deck(str(tmp_path / "plain.in"), solver, prec, profiler=False)
+    prof = deck(str(tmp_path / "prof.in"), solver, prec)
+    lg0 = hiplib.run_deck(plain, tmp_path / "plain.log", mode=1)
+    capfd.readouterr()
+    out = tmp_path / "tea.out"
+    lg = hiplib.run_deck(prof, tmp_path / "prof.log", mode=1, tea_out=str(out))
+    printed = capfd.readouterr().out
+    assert lg["summary"] == lg0["summary"]
+    assert [s[1] for s in lg["solve"]] == [s[1] for s in lg0["solve"]]
+    table = kernel_table(printed, "@@@@@ HIP Profiling information @@@@@")
+    steps = len(lg["solve"])
+    n_exit = [s[1] + 1 for s in lg["solve"]]  # the loop counter at exit
+    executed = [min(n, 4000) for n in n_exit]  # iterations run (max_iters 4000)
+    assert table["tea_leaf_init_common"][1] == steps
+    assert table["tea_leaf_calc_residual"][1] == steps
+    assert table["field_summary"][1] == len(lg["summary"])
+    assert table["k_dot"][1] == steps
+    if solver == "tl_use_cg":
+        assert table["k_fused_cg_w"][1] == table["k_fused_cg_ur"][1] == sum(executed)
+        assert table["k_finish"][1] == 2 * sum(executed)
+    elif solver == "tl_use_jacobi":
+        assert table["k_jacobi"][1] == table["k_finish"][1] == sum(executed)
+    elif solver == "tl_use_chebyshev":
+        assert table["k_fused_cg_w"][1] == table["k_fused_cg_ur"][1] > 0
+        assert table["k_fused_cheby2"][1] > 0
+    else:
+        cg = table["k_fused_cg_ur"][1]
+        outer = table["k_fused_cg_ur_ppcg"][1]
+        assert cg + outer == sum(executed)
+        assert table["k_fused_ppcg_inner2"][1] == 3 * outer and table["k_fused_ppcg_inner"][1] == outer
+        inner = sum(s[4] for s in lg["solve"])
+        assert inner == 7 * outer
+    assert all(t[0] >= 0.0 for t in table.values())
+    text = out.read_text()
+    blk = profile_block(text)
+    assert [b[0] for b in blk] == LABELS
+    vals = dict((b[0], b[1]) for b in blk)
+    assert vals["Tea Solve"] > 0 and all(v >= 0 for k, v in vals.items() if k != "The Rest")
+    parts = sum(v for k, v in vals.items() if k not in ("Total", "The Rest"))
+    assert abs(vals["Total"] - parts) <= 6e-4
+    assert solve_time_lines(text) == n_exit
+
+
+@pytest.mark.skipif(not fd.available(), reason="Fortran driver not built (oracle/fortran_driver.mk)")
+def test_fortran_driver_profiler(hiplib, tmp_path):
+    """A profiler_on CG deck through the reference's Fortran driver: its
+    tea.out carries its own Profiler Output block and Solve Time lines, and
+    the library's table (stdout) names the reference's kernels with one
+    calc_w, calc_ur and calc_p per CG iteration; the native driver's block has
+    the same lines."""
+    d = deck(str(tmp_path / "d.in"), "tl_use_cg", "jac_diag")
+    rc, printed, f_out = fd.run(d, str(tmp_path / "fortran"))
+    assert rc == 0, printed[-2000:]
+    its = [int(l[16:]) for l in f_out.splitlines() if l.startswith("Iteration count ")]
+    assert len(its) == 3
+    table = kernel_table(printed, "@@@@@ HIP Profiling information @@@@@")
+    n_loop = sum(i + 1 for i in its)
+    assert table["tea_leaf_cg_solve_calc_w"][1] == n_loop
+    assert table["tea_leaf_cg_solve_calc_ur"][1] == n_loop
+    # calc_p runs before the exit test (tea_solve.f90:270-300)
+    assert table["tea_leaf_cg_solve_calc_p"][1] == n_loop
+    assert table["tea_leaf_init_common"][1] == 3
+    assert solve_time_lines(f_out) == [i + 1 for i in its]
+    ref_blk = profile_block(f_out)
+    out = tmp_path / "native.out"
+    hiplib.run_deck(d, tmp_path / "n.log", mode=1, tea_out=str(out))
+    blk = profile_block(out.read_text())
+    assert [b[0] for b in blk] == [b[0] for b in ref_blk] == LABELS
