@@ -78,9 +78,9 @@ struct tl_chunk {
     bool faces_current = false;
     // Chebyshev iterations per fused launch (1 or 2; TL_TUNE_CHEBY_STEPS_PER_LAUNCH)
     int cheby_steps_per_launch = 2;
-    // events around the eagerly launched first CG iteration of each batch
-    // (one set per pipeline slot)
-    hipEvent_t ev_t[2][4] = {{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr, nullptr}};
+    // events around the eagerly launched first CG iterations of each batch
+    // (an odd and an even one; one set per pipeline slot)
+    hipEvent_t ev_t[2][8] = {};
     hipEvent_t ev_batch[2] = {nullptr, nullptr};
     bool timing = false;
     double t_k1_ms = 0, t_k2_ms = 0;

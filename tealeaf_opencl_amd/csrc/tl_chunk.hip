@@ -241,10 +241,11 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
     if (c->h_result) hipHostFree(c->h_result);
     if (c->h_flag) hipHostFree(c->h_flag);
     if (c->h_st) hipHostFree(c->h_st);
-    for (hipEvent_t ev : {c->pe0, c->pe1, c->pe_mid, c->ev_t[0][0], c->ev_t[0][1], c->ev_t[0][2], c->ev_t[0][3],
-                          c->ev_t[1][0], c->ev_t[1][1], c->ev_t[1][2], c->ev_t[1][3], c->ev_batch[0],
-                          c->ev_batch[1]})
+    for (hipEvent_t ev : {c->pe0, c->pe1, c->pe_mid, c->ev_batch[0], c->ev_batch[1]})
         if (ev) hipEventDestroy(ev);
+    for (auto &set : c->ev_t)
+        for (hipEvent_t ev : set)
+            if (ev) hipEventDestroy(ev);
     if (c->s) hipStreamDestroy(c->s);
     if (g_default == c) g_default = nullptr;
     delete c;
@@ -320,12 +321,17 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     }
     hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
     *c->h_flag = 0u;
-    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->pe_mid, &c->ev_t[0][0], &c->ev_t[0][1], &c->ev_t[0][2], &c->ev_t[0][3],
-                           &c->ev_t[1][0], &c->ev_t[1][1], &c->ev_t[1][2], &c->ev_t[1][3]})
+    for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->pe_mid})
         if ((e = hipEventCreate(ev)) != hipSuccess) {
             tl_chunk_destroy(c);
             return hip_fail("hipEventCreate", e);
         }
+    for (auto &set : c->ev_t)
+        for (hipEvent_t &ev : set)
+            if ((e = hipEventCreate(&ev)) != hipSuccess) {
+                tl_chunk_destroy(c);
+                return hip_fail("hipEventCreate", e);
+            }
     for (hipEvent_t *ev : {&c->ev_batch[0], &c->ev_batch[1]})
         if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) {
             tl_chunk_destroy(c);

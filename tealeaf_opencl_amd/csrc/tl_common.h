@@ -90,7 +90,7 @@ struct SolveState {
     int cheby_calc_steps;
     int cg_calc_steps;
     int ppcg_inner_iters;
-    int pad_;
+    int fault;            // 1: a kernel captured for one iteration parity met the other (all kernels stop)
 };
 
 // reduction finish modes: which scalar algebra the last block runs

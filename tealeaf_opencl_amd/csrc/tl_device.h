@@ -175,8 +175,9 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// Block reduction of NR sums into partials[blockIdx.x * MAX_RED + q]. Must be
-// reached by every thread of the block.
+// Block reduction of NR sums into partials[q * gridDim.x + blockIdx.x]
+// (quantity-major, so that the finisher of a one-quantity reduction reads one
+// contiguous array). Must be reached by every thread of the block.
 template <int NR>
 __device__ __forceinline__ void block_partial(const double (&v)[NR], double *partials)
 {
@@ -190,7 +191,7 @@ __device__ __forceinline__ void block_partial(const double (&v)[NR], double *par
     __syncthreads();
     if (threadIdx.x < NR) {
         const int q = threadIdx.x;
-        partials[blockIdx.x * MAX_RED + q] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
+        partials[q * gridDim.x + blockIdx.x] = ((lds[0][q] + lds[1][q]) + lds[2][q]) + lds[3][q];
     }
 }
 
@@ -212,7 +213,7 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblocks
         for (int u = 0; u < U; ++u) {
             const int b = b0 + u * FINISH_THREADS;
 #pragma unroll
-            for (int q = 0; q < MAX_RED; ++q) v[u][q] = (q < nq && b < nblocks) ? partials[b * MAX_RED + q] : 0.0;
+            for (int q = 0; q < MAX_RED; ++q) v[u][q] = (q < nq && b < nblocks) ? partials[q * nblocks + b] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -467,6 +468,104 @@ __device__ __forceinline__ void strip_precondition(const Geom &g, int ks, int i0
 #pragma unroll
         for (int q = 0; q < STRIP; ++q) zv[q] = rv[q];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Band march (the fused CG kernels of the device-resident driver). A
+// wavefront owns a band of 64 lanes and walks the rows of one segment of M.H
+// rows downwards with the rows above and below the current one in
+// registers: one load per row and field. The west and east neighbours'
+// values come from lanes -1 / +1 by DPP (wave_shr:1 / wave_shl:1), and each
+// face coefficient is formed once, by the lane east of the face, and passed to
+// the lane west of it. Two band layouts: overlapping bands of MARCH_COLS
+// output columns (lanes 1..62 = columns jb .. jb+61; lanes 0 and 63 only
+// supply neighbours), or aligned bands of 64 output columns (lane l = column
+// jb+l, 512-byte aligned rows) whose edge lanes load the column just outside
+// the band themselves, for kernels whose several store streams want whole
+// cache lines. Segments are cut into eight slabs of consecutive rows; blocks
+// b with b % 8 == x (workgroups are dispatched round-robin to the eight
+// XCDs) walk slab x, four bands side by side per block, so the halo rows and
+// columns that neighbouring items share stay in one L2.
+// ---------------------------------------------------------------------------
+constexpr int MARCH_COLS = 62;
+// segments of 16 rows for the one-step kernels (measured at 4096^2: 16 rows
+// 110 us per cg_w launch, 32 rows 117 us, 64 rows 125 us; a taller segment
+// reuses more rows but leaves fewer wavefronts to hide latency)
+constexpr int MARCH_ROWS = 16;
+// two-step kernels: 60 output columns per band (two exchange lanes either
+// side) and taller segments (each segment also runs step 1 on the strip
+// above and below it)
+constexpr int MARCH2_COLS = 60;
+constexpr int MARCH2_ROWS = 32;
+
+struct March {
+    int cols;      // output columns per band: MARCH_COLS or 64
+    int nbands;    // bands across the interior
+    int ngroups;   // blocks of four bands across the interior
+    int H;         // rows per segment
+    int nseg;      // segments
+    int segs_per;  // segments per slab
+};
+
+// lane l gets lane l-1's v (lane 0 keeps `old`)
+__device__ __forceinline__ double dpp_shr1(double v, double old)
+{
+    const int2 a = __builtin_bit_cast(int2, v), o = __builtin_bit_cast(int2, old);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(o.x, a.x, 0x138, 0xF, 0xF, false);
+    r.y = __builtin_amdgcn_update_dpp(o.y, a.y, 0x138, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, r);
+}
+
+// lane l gets lane l+1's v (lane 63 keeps `old`)
+__device__ __forceinline__ double dpp_shl1(double v, double old)
+{
+    const int2 a = __builtin_bit_cast(int2, v), o = __builtin_bit_cast(int2, old);
+    int2 r;
+    r.x = __builtin_amdgcn_update_dpp(o.x, a.x, 0x130, 0xF, 0xF, false);
+    r.y = __builtin_amdgcn_update_dpp(o.y, a.y, 0x130, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, r);
+}
+
+// The items of this block: band (of this wavefront) and segment. Every
+// wavefront of the block runs the same number of loop trips, so a block
+// reduction after the loop is reached by all of them.
+#define TL_MARCH_LOOP(m, e)                                                                    \
+    {                                                                                          \
+        const int xcd_ = blockIdx.x & 7, per_ = gridDim.x >> 3;                                \
+        const int items_ = (m).segs_per * (m).ngroups;                                         \
+        for (int t_ = blockIdx.x >> 3; t_ < items_; t_ += per_) {                              \
+            const int seg_ = xcd_ * (m).segs_per + t_ / (m).ngroups;                           \
+            const int band_ = (t_ % (m).ngroups) * 4 + (int)(threadIdx.x >> 6);                \
+            if (seg_ >= (m).nseg || band_ >= (m).nbands) continue;                             \
+            const int jb = (e).j0 + band_ * (m).cols;                                          \
+            const int k_begin = (e).k0 + seg_ * (m).H;                                         \
+            const int k_end = min((e).k1, k_begin + (m).H - 1);
+#define TL_MARCH_LOOP_END \
+    }                     \
+    }
+
+// aligned bands: lane 0 loads column jb-1, lane 63 column jb+64 (exec-masked);
+// the other lanes get 0
+__device__ __forceinline__ double edge_ld(const double *f, int i, int lane)
+{
+    double h = 0.0;
+    if (lane == 0 || lane == 63) h = f[i + (lane == 0 ? -1 : 1)];
+    return h;
+}
+
+// Kx of the face west of column j in row k from the conductivities either
+// side (bitwise tea_leaf_init_common's value, zeroed as tea_leaf_zero_boundary
+// zeroes it); `inner`: the face is known to lie strictly inside the chunk
+__device__ __forceinline__ double march_kx(const Geom &g, const Faces &f, bool inner, int j, int k, double wl,
+                                           double wc)
+{
+    return inner ? face_coef_nr(wl, wc, f.rx) : kx_at(g, f, j, k, wl, wc);
+}
+__device__ __forceinline__ double march_ky(const Geom &g, const Faces &f, bool inner, int j, int k, double wd,
+                                           double wc)
+{
+    return inner ? face_coef_nr(wd, wc, f.ry) : ky_at(g, f, j, k, wd, wc);
 }
 
 }  // namespace tl

@@ -484,8 +484,10 @@ void launch_timed(tl_chunk *c, bool prof, const char *name, bool fin, F &&launch
 // One CG iteration of the device-resident driver: the fused search-direction
 // update + matvec + p.w, then the fused u/r update + preconditioner + r.z.
 // With `ev`, ev[0..1] bracket the first iteration kernel and ev[2..3] the
-// second (their finishers are outside the brackets).
-void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false)
+// second (their finishers are outside the brackets). parity 1 / 2: the
+// iteration is known to be odd / even (odd ones take the band-march update
+// kernel); 0: unknown.
+void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false, int parity = 0)
 {
     const Ext in = ext_interior(c);
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
@@ -495,10 +497,11 @@ void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false)
                           red_args(c, RED_CG_PW, true), ev ? ev[1] : mid);
     });
     if (ev) hipEventRecord(ev[2], c->s);
-    launch_timed(c, prof, "k_fused_cg_ur", true, [&](hipEvent_t mid) {
+    const char *ur_name = parity == 1 ? "k_fused_cg_r_march" : parity == 2 ? "k_fused_cg_ru_march" : "k_fused_cg_ur";
+    launch_timed(c, prof, ur_name, true, [&](hipEvent_t mid) {
         launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
                            c->f2[F_R], c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces,
-                           red_args(c, RED_CG_RRN, true), ev ? ev[3] : mid);
+                           red_args(c, RED_CG_RRN, true), ev ? ev[3] : mid, parity);
     });
 }
 
@@ -529,7 +532,10 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
     hipError_t e;
     int slot = 0;
     auto enqueue = [&](int sl) -> hipError_t {
-        if (timed) cg_iteration(c, c->ev_t[sl]);
+        if (timed) {
+            cg_iteration(c, c->ev_t[sl], false, 1);
+            cg_iteration(c, c->ev_t[sl] + 4, false, 2);
+        }
         hipError_t ee = hipGraphLaunch(exec, c->s);
         if (ee != hipSuccess) return ee;
         ee = hipMemcpyAsync(&c->h_st[sl], c->st, sizeof(SolveState), hipMemcpyDeviceToHost, c->s);
@@ -541,14 +547,23 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
         if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
         if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
         const SolveState &h = c->h_st[slot];
+        if (h.fault) {
+            hipStreamSynchronize(c->s);
+            set_error("an iteration kernel captured for one iteration parity ran on the other");
+            return TL_ERR_SOLVER;
+        }
         if (timed) {
             // every examined batch follows a batch that did not halt, so its
-            // eager first iteration ran
-            float a = 0.f, b = 0.f;
-            if (hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
-                hipEventElapsedTime(&b, c->ev_t[slot][2], c->ev_t[slot][3]) == hipSuccess) {
-                c->t_k1_ms += a;
-                c->t_k2_ms += b;
+            // eager first iterations ran (unless the pair's first one halted:
+            // then the second one's kernels returned at once and the sample is
+            // left out); the sample is the mean over the odd and the even one
+            float a[4] = {0.f, 0.f, 0.f, 0.f};
+            bool ok = true;
+            for (int q = 0; q < 4; ++q)
+                ok = ok && hipEventElapsedTime(&a[q], c->ev_t[slot][2 * q], c->ev_t[slot][2 * q + 1]) == hipSuccess;
+            if (ok && !h.halt) {
+                c->t_k1_ms += 0.5 * (a[0] + a[2]);
+                c->t_k2_ms += 0.5 * (a[1] + a[3]);
                 c->t_samples += 1;
             }
         }
@@ -565,15 +580,16 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
     return TL_OK;
 }
 
-// 16 CG iterations per replay; with `timed`, 14 plus the eagerly launched one
-// of each batch, so that the timed iteration alternates between odd and even
-// iterations (the update kernel differs by parity)
+// 16 CG iterations per replay; with `timed`, 14 plus the eagerly launched
+// odd and even ones of each batch. Every batch starts on an odd iteration
+// (the CG phase starts at n = 1), so the kernels are captured for known
+// parities: odd iterations take the band-march update kernel.
 int cg_graph(tl_chunk *c, bool timed)
 {
     const int B = timed ? 14 : 16;
     const int key = c->prec * 10 + (timed ? 1 : 0);
     return capture(c, c->g_cg, key, [&]() {
-        for (int b = 0; b < B; ++b) cg_iteration(c, nullptr);
+        for (int b = 0; b < B; ++b) cg_iteration(c, nullptr, false, (b & 1) ? 2 : 1);
     });
 }
 
@@ -770,7 +786,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
             RC(run_batches(c, c->g_jacobi.exec, false, cfg.max_iters / 16 + 4));
         } else {
             if (prof) {
-                RC(run_eager(c, [&](long) { cg_iteration(c, nullptr, true); }, cfg.max_iters + 4));
+                RC(run_eager(c, [&](long k) { cg_iteration(c, nullptr, true, (k & 1) ? 2 : 1); }, cfg.max_iters + 4));
             } else {
                 RC(cg_graph(c, timed));
                 RC(run_batches(c, c->g_cg.exec, timed, cfg.max_iters / 15 + 4));

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "tl_common.h"
+#include "tl_device.h"
 
 namespace tl {
 
@@ -18,6 +19,12 @@ void launch_generate(hipStream_t s, const Geom &g, const Ext &e, int cap, double
                      double *e0, double *u, double *u0, int nstates, const double *const *sarr,
                      const int *sgeom, int g_rect, int g_circ, int g_point);
 void launch_copy(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src, double *dst);
+// band-march geometry of an extent (TL_MARCH_LOOP): bands of `cols` output
+// columns, segments of MARCH_ROWS rows (set_march_rows: another height), and
+// the grid of a launch, capped at the partials' capacity
+March make_march(const Ext &e, int cols);
+void set_march_rows(March &m, const Ext &e, int rows);
+int march_grid(const March &m, int cap);
 // 4 fields read, 3 written (the stream mix of the CG update kernel), no stencil
 void launch_stream_mix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *a, const double *b, double *c,
                        const double *d, double *out);
@@ -88,10 +95,13 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
                        double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
                        hipEvent_t ev_main = nullptr);
+// parity 1 / 2: the launch is known to run an odd / even iteration (CG: the
+// band-march kernels, the odd one leaving u alone); 0: either parity
+// (decided by the tile kernel)
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, const double *r, double *r_out, double *z,
                         double *sd, const double *W, const Faces &fc, const RedArgs &ra,
-                        hipEvent_t ev_main = nullptr);
+                        hipEvent_t ev_main = nullptr, int parity = 0);
 // u += alpha p_L after a fused CG phase that ended on an odd iteration
 void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
                      const SolveState *st);

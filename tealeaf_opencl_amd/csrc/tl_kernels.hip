@@ -151,7 +151,7 @@ __device__ __forceinline__ bool iteration_mode(int mode)
 // solver does nothing, like the kernel before it.
 __global__ void __launch_bounds__(FINISH_THREADS) k_finish(RedArgs ra, int nblocks, int nq)
 {
-    if (ra.st && iteration_mode(ra.mode) && ra.st->halt) return;
+    if (ra.st && iteration_mode(ra.mode) && (ra.st->halt || ra.st->fault)) return;
     double tot[MAX_RED];
     sum_partials(ra.partials, nblocks, nq, tot);
     if (threadIdx.x == 0) finish_scalars(ra, tot);
@@ -864,51 +864,81 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const d
 // fused iteration kernels (device-resident driver)
 // ===========================================================================
 
-// CG / PPCG outer step, first half: p_new = z + beta*p_old (r + beta*p_old
-// without preconditioner; tea_leaf_cg_cl.cl:141-160 of the previous
-// iteration), w = A p_new, p_new.w (tea_leaf_cg_cl.cl:45-69). p_new at the
-// four neighbours is formed from the old buffers, so no cell is read after it
-// is overwritten; iteration n reads buffer (n-1)&1 and writes buffer n&1.
-// Stores p_new only; w is recomputed by the second half.
-__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, const double *src,
-                                                              double *pA, double *pB,
-                                                              const double *W, Faces fc, RedArgs ra)
+// Fused CG search direction and matvec: p = z + beta p_old (the previous
+// iteration's tea_leaf_cg_solve_calc_p, tea_leaf_cg_cl.cl:141-160) folded into
+// w = A p and p.w (calc_w, :45-69); p is double-buffered by the iteration
+// counter and w is not stored. A band march (TL_MARCH_LOOP): each row of
+// z, p_old and the conductivity is loaded once per band, the new p of the
+// rows above and below and the face below are kept in registers.
+// Algorithmic bytes: R z, p_old, cond, W p = 32 B/cell.
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, March m, const double *src,
+                                                              double *pA, double *pB, const double *W, Faces fc,
+                                                              RedArgs ra)
 {
     const SolveState *st = ra.st;
-    if (st->halt) return;
+    if (st->halt || st->fault) return;
     const double beta = st->beta;
     const bool odd = (st->n & 1) != 0;
     const double *p_old = odd ? pA : pB;
     double *p_new = odd ? pB : pA;
-    double acc[1] = {0.0};
-    TL_TILE_LOOP(e)
-    const int i0 = IX(g, j, ks), P = g.pitch;
-    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1];
-    double sc[STRIP + 2], pc[STRIP + 2], sw[STRIP], pw_[STRIP], se[STRIP], pe[STRIP];
-    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col6(src, i0, P, sc);
-    col6(p_old, i0, P, pc);
-    col4(src, i0 - 1, P, sw);
-    col4(p_old, i0 - 1, P, pw_);
-    col4(src, i0 + 1, P, se);
-    col4(p_old, i0 + 1, P, pe);
-    double c6[STRIP + 2], w4[STRIP], e4[STRIP], wv[STRIP];
+    const int lane = threadIdx.x & 63;
+    const int P = g.pitch, hd = g.hd;
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb - 1 + lane;
+    const bool out_lane = lane >= 1 && lane <= MARCH_COLS && j <= e.j1;
+    // every face the band's outputs use lies strictly inside the chunk in x
+    const bool inner_x = jb - 1 > hd && jb + MARCH_COLS < hd + g.x_max;
+    // rows k-1 (pa) and k (pb) of the new p, conductivity of row k, the face
+    // between them (kyb)
+    int i = IX(g, j, k_begin - 1);
+    double pa = src[i] + beta * p_old[i];
+    const double ca = W[i];
+    i += P;
+    double pb = src[i] + beta * p_old[i];
+    double cb = W[i];
+    double kyb = ky_at(g, fc, j, k_begin, ca, cb);
+    // loads of the next two rows in flight
+    constexpr int PF = 2;
+    double zr[PF], pr[PF], cr[PF];
 #pragma unroll
-    for (int q = 0; q < STRIP + 2; ++q) c6[q] = sc[q] + beta * pc[q];
-#pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        w4[q] = sw[q] + beta * pw_[q];
-        e4[q] = se[q] + beta * pe[q];
+    for (int q = 0; q < PF; ++q) {
+        const int ii = i + (q + 1) * P;
+        zr[q] = src[ii];
+        pr[q] = p_old[ii];
+        cr[q] = W[ii];
     }
-    strip_apply(c6, w4, e4, kx0, kx1, ky, wv);
+    for (int k = k_begin; k <= k_end; k += PF) {
 #pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        if (ks + q > e.k1) continue;
-        store_nt(p_new + i0 + q * P, c6[q + 1]);
-        acc[0] += c6[q + 1] * wv[q];
+        for (int q = 0; q < PF; ++q) {
+            const int kk = k + q;
+            const double pc = zr[q] + beta * pr[q];  // row kk+1
+            const double cc = cr[q];
+            {
+                const int ii = IX(g, j, kk + 1 + PF);
+                zr[q] = src[ii];
+                pr[q] = p_old[ii];
+                cr[q] = W[ii];
+            }
+            if (kk > k_end) continue;
+            const bool inner = inner_x && kk > hd && kk + 1 < hd + g.y_max;
+            const double kyn = march_ky(g, fc, inner, j, kk + 1, cb, cc);
+            const double kx0 = march_kx(g, fc, inner, j, kk, dpp_shr1(cb, 0.0), cb);
+            const double kx1 = dpp_shl1(kx0, 0.0);
+            const double wv = opA(kx0, kx1, kyb, kyn, pb, dpp_shr1(pb, 0.0), dpp_shl1(pb, 0.0), pa, pc);
+            if (out_lane) {
+                store_nt(p_new + IX(g, j, kk), pb);
+                acc += pb * wv;
+            }
+            pa = pb;
+            pb = pc;
+            cb = cc;
+            kyb = kyn;
+        }
     }
-    TL_TILE_LOOP_END
-    block_partial<1>(acc, ra.partials);
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    block_partial<1>(a1, ra.partials);
 }
 
 // CG / PPCG outer step, second half (tea_leaf_cg_cl.cl:73-137): w = A p
@@ -932,7 +962,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
                                                                RedArgs ra)
 {
     const SolveState *st = ra.st;
-    if (st->halt) return;
+    if (st->halt || st->fault) return;
     const bool odd = (st->n & 1) != 0;
     const double *p = odd ? pB : pA;
     const double *p_prev = odd ? pA : pB;
@@ -975,6 +1005,187 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     }
     TL_TILE_LOOP_END
     if (!PPCG) block_partial<1>(acc, ra.partials);
+}
+
+// k_fused_cg_ur<PREC, 0> on odd iterations (those that leave u alone) as a
+// band march (TL_MARCH_LOOP): r -= alpha A p, z = M^-1 r, r.z, with one load
+// per row of p and the conductivity, the neighbours by lane exchange and the
+// faces formed once. The driver captures it only for odd iterations; on an
+// even one it sets the state's fault flag instead, which stops every later
+// kernel and the solve. Algorithmic bytes: R p, r, cond, W r, z = 40 B/cell.
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_r_march(Geom g, Ext e, March m, const double *pB,
+                                                                    const double *r, double *r_out, double *z,
+                                                                    const double *W, Faces fc, RedArgs ra)
+{
+    SolveState *st = ra.st;
+    if (st->halt || st->fault) return;
+    if (!(st->n & 1)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->fault = 1;
+        return;
+    }
+    const double *p = pB;  // iteration n (odd) wrote p into buffer n & 1
+    const double alpha = st->alpha;
+    const int lane = threadIdx.x & 63;
+    const int P = g.pitch, hd = g.hd;
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb - 1 + lane;
+    const bool out_lane = lane >= 1 && lane <= MARCH_COLS && j <= e.j1;
+    const bool inner_x = jb - 1 > hd && jb + MARCH_COLS < hd + g.x_max;
+    // rows ks-1 .. ks+4 of p and the conductivity; the first two carried from
+    // the strip above
+    double p6[STRIP + 2], c6[STRIP + 2];
+    const int ib = IX(g, j, k_begin - 1);
+    p6[0] = p[ib];
+    p6[1] = p[ib + P];
+    c6[0] = W[ib];
+    c6[1] = W[ib + P];
+    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
+    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
+        const int i0 = IX(g, j, ks);
+        double rv[STRIP];
+        col4_nt(r, i0, P, rv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            p6[q + 2] = p[i0 + (q + 1) * P];
+            c6[q + 2] = W[i0 + (q + 1) * P];
+        }
+        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], wv[STRIP], zv[STRIP];
+        ky[0] = ky0;
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
+            kx1[q] = dpp_shl1(kx0[q], 0.0);
+            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
+            w4[q] = dpp_shr1(p6[q + 1], 0.0);
+            e4[q] = dpp_shl1(p6[q + 1], 0.0);
+        }
+        strip_apply(p6, w4, e4, kx0, kx1, ky, wv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) rv[q] -= alpha * wv[q];
+        mask_rows(ks, e.k1, rv);
+        if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1 || !out_lane) continue;
+            const int ii = i0 + q * P;
+            store_nt(r_out + ii, rv[q]);
+            double zz = rv[q];
+            if (PREC == PREC_BLOCK) zz = zv[q];
+            if (PREC == PREC_DIAG) zz = rv[q] * div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+            if (PREC != PREC_NONE) store_nt(z + ii, zz);
+            acc += rv[q] * zz;
+        }
+        p6[0] = p6[4];
+        p6[1] = p6[5];
+        c6[0] = c6[4];
+        c6[1] = c6[5];
+        ky0 = ky[STRIP];
+    }
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    block_partial<1>(a1, ra.partials);
+}
+
+// k_fused_cg_ur<PREC, 0> on even iterations as a band march on aligned bands
+// (three store streams: whole cache lines): r -= alpha A p, z = M^-1 r, r.z,
+// and u += alpha_{n-1} p_{n-1} + alpha_n p_n (the two iterations' updates of
+// u, each one FMA, in the order of per-iteration updates). Captured for even
+// iterations only; on an odd one it sets the fault flag. Algorithmic bytes:
+// R p, r, cond, u, p_{n-1}, W r, z, u = 64 B/cell.
+template <int PREC>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ru_march(Geom g, Ext e, March m, double *u,
+                                                                     const double *pA, const double *pB,
+                                                                     const double *r, double *r_out, double *z,
+                                                                     const double *W, Faces fc, RedArgs ra)
+{
+    SolveState *st = ra.st;
+    if (st->halt || st->fault) return;
+    if (st->n & 1) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->fault = 1;
+        return;
+    }
+    const double *p = pA, *p_prev = pB;  // even n: p_n in buffer 0, p_{n-1} in buffer 1
+    const double alpha = st->alpha, alpha_prev = st->alpha_prev;
+    const int lane = threadIdx.x & 63;
+    const int P = g.pitch, hd = g.hd;
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb + lane;
+    const bool out_lane = j <= e.j1;
+    const bool inner_x = jb - 1 > hd && jb + 64 < hd + g.x_max;
+    // rows ks-1 .. ks+4 of p and the conductivity (and the conductivity of
+    // the edge lanes' outside columns); the first two carried from the strip above
+    double p6[STRIP + 2], c6[STRIP + 2], h6[STRIP + 2];
+    const int ib = IX(g, j, k_begin - 1);
+    p6[0] = p[ib];
+    p6[1] = p[ib + P];
+    c6[0] = W[ib];
+    c6[1] = W[ib + P];
+    h6[0] = edge_ld(W, ib, lane);
+    h6[1] = edge_ld(W, ib + P, lane);
+    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
+    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
+        const int i0 = IX(g, j, ks);
+        double uv[STRIP], rv[STRIP], pp[STRIP], ph[STRIP];
+        col4_nt(u, i0, P, uv);
+        col4_nt(r, i0, P, rv);
+        col4_nt(p_prev, i0, P, pp);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            p6[q + 2] = p[i0 + (q + 1) * P];
+            c6[q + 2] = W[i0 + (q + 1) * P];
+            h6[q + 2] = edge_ld(W, i0 + (q + 1) * P, lane);
+            ph[q] = edge_ld(p, i0 + q * P, lane);
+        }
+        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], wv[STRIP], zv[STRIP];
+        ky[0] = ky0;
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], h6[q + 1]), c6[q + 1]);
+            // the face east of the band, formed by lane 63 from its outside column
+            double ke = 0.0;
+            if (lane == 63) ke = march_kx(g, fc, inner, j + 1, ks + q, c6[q + 1], h6[q + 1]);
+            kx1[q] = dpp_shl1(kx0[q], ke);
+            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
+            w4[q] = dpp_shr1(p6[q + 1], ph[q]);
+            e4[q] = dpp_shl1(p6[q + 1], ph[q]);
+        }
+        strip_apply(p6, w4, e4, kx0, kx1, ky, wv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            uv[q] = fma(alpha_prev, pp[q], uv[q]);
+            uv[q] = fma(alpha, p6[q + 1], uv[q]);
+            rv[q] -= alpha * wv[q];
+        }
+        mask_rows(ks, e.k1, rv);
+        if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1 || !out_lane) continue;
+            const int ii = i0 + q * P;
+            store_nt(u + ii, uv[q]);
+            store_nt(r_out + ii, rv[q]);
+            double zz = rv[q];
+            if (PREC == PREC_BLOCK) zz = zv[q];
+            if (PREC == PREC_DIAG) zz = rv[q] * div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
+            if (PREC != PREC_NONE) store_nt(z + ii, zz);
+            acc += rv[q] * zz;
+        }
+        p6[0] = p6[4];
+        p6[1] = p6[5];
+        c6[0] = c6[4];
+        c6[1] = c6[5];
+        h6[0] = h6[4];
+        h6[1] = h6[5];
+        ky0 = ky[STRIP];
+    }
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    block_partial<1>(a1, ra.partials);
 }
 
 // Completes u after a CG phase of the fused kernels that ended on an odd
@@ -1035,38 +1246,76 @@ __device__ __forceinline__ double ppcg_rz(double rv, double zo, double zf)
 
 // PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
 // sd is double-buffered. LAST = 1, 2 also form r.z (ppcg_rz); the finisher
-// runs the tail.
+// runs the tail. A band march with the geometry of the two-step kernel
+// (bands of MARCH2_COLS columns, segments of MARCH2_ROWS rows), so that a
+// step's r.z is summed in the same order whether it ends a one-step or a
+// two-step launch; faces from the conductivity (bitwise the stored Kx, Ky).
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
-    Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *W, Faces fc,
+    Geom g, Ext e, March m, double *u, double *r, const double *sd_in, double *sd_out, const double *W, Faces fc,
     double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
     const double ca = ch_a[step], cb = ch_b[step];
-    double acc[1] = {0.0};
-    TL_TILE_LOOP(e)
-    const int i0 = IX(g, j, ks), P = g.pitch;
-    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], sdn[STRIP],
-        zo[STRIP] = {}, zf[STRIP];
-    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col4_nt(u, i0, P, uv);
-    col4_nt(r, i0, P, rv);
-    if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
-    strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
-    ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn, zf);
+    const int lane = threadIdx.x & 63;
+    const int P = g.pitch, hd = g.hd;
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb - 2 + lane;
+    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
+    const bool inner_x = jb - 1 > hd && jb + MARCH2_COLS + 1 < hd + g.x_max;
+    double s6[STRIP + 2], c6[STRIP + 2];
+    const int ib = IX(g, j, k_begin - 1);
+    s6[0] = sd_in[ib];
+    s6[1] = sd_in[ib + P];
+    c6[0] = W[ib];
+    c6[1] = W[ib + P];
+    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
+    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
+        const int i0 = IX(g, j, ks);
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], uv[STRIP],
+            rv[STRIP], sdn[STRIP], zo[STRIP] = {}, zf[STRIP];
 #pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        if (ks + q > e.k1) continue;
-        const int i = i0 + q * P;
-        store_nt(u + i, uv[q] + sc[q]);
-        store_nt(r + i, rv[q]);
-        store_nt(sd_out + i, sdn[q]);
-        if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
-        if (LAST) acc[0] += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
+        for (int q = 0; q < STRIP; ++q) {
+            s6[q + 2] = sd_in[i0 + (q + 1) * P];
+            c6[q + 2] = W[i0 + (q + 1) * P];
+        }
+        col4_nt(u, i0, P, uv);
+        col4_nt(r, i0, P, rv);
+        if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
+        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
+        ky[0] = ky0;
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
+            kx1[q] = dpp_shl1(kx0[q], 0.0);
+            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
+            w4[q] = dpp_shr1(s6[q + 1], 0.0);
+            e4[q] = dpp_shl1(s6[q + 1], 0.0);
+            sc[q] = s6[q + 1];
+        }
+        strip_apply(s6, w4, e4, kx0, kx1, ky, res);
+        ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn, zf);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1 || !out_lane) continue;
+            const int i = i0 + q * P;
+            store_nt(u + i, uv[q] + sc[q]);
+            store_nt(r + i, rv[q]);
+            store_nt(sd_out + i, sdn[q]);
+            if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
+            if (LAST) acc += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
+        }
+        s6[0] = s6[STRIP];
+        s6[1] = s6[STRIP + 1];
+        c6[0] = c6[STRIP];
+        c6[1] = c6[STRIP + 1];
+        ky0 = ky[STRIP];
     }
-    TL_TILE_LOOP_END
-    if (LAST) block_partial<1>(acc, ra.partials);
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    if (LAST) block_partial<1>(a1, ra.partials);
 }
 
 // sd at ring cell (j, k) when it is not computed in the block: the value the
@@ -1080,126 +1329,131 @@ __device__ __forceinline__ double ring_outside(const Geom &g, const Ext &e, cons
 }
 
 // Two PPCG inner steps (steps `step` and `step`+1 of the smoother) in one
-// pass over the fields. Step 1 is computed on the block's 64x16 tile and on
-// the ring around it that step 2 reads: the 4-row strips directly above and
-// below the tile (their block-Jacobi solve needs the whole strip) and the
-// tile's strips in the columns left and right of it. The ring's new sd goes
-// through LDS; the tile keeps r, u and the faces in registers. Every value is
-// formed by the same expressions as the one-step kernel, so two one-step
-// launches and one two-step launch give bitwise the same u, r and sd.
-// The face coefficients are read from Kx, Ky (bitwise what the one-step
-// kernels recompute from the conductivity): with the ring's extra strips this
-// kernel is bound by arithmetic as much as by memory, and reading the faces
-// instead of dividing measured 5 % faster per PPCG outer step at 4096^2.
-// Algorithmic bytes per two steps: R u, r, sd, Kx, Ky  W u, r, sd (64 B/cell)
-// instead of 112. Neighbouring blocks read r and sd of this block's tile in
-// their step 1, so r and sd are both read from one buffer and written to
-// another.
+// pass over the fields, as a band march with a one-strip lag: a wavefront
+// owns a band of 64 lanes (columns jb-2 .. jb+61; lanes 2..61 are its outputs,
+// since step 2 needs step-1 values one lane beyond step 1's own neighbours)
+// and walks 4-row strips down a segment: step 1 on strip S (u, r, sd of S,
+// sd of the rows either side, the stored faces), then step 2 on strip S-1
+// from step 1's sd of the rows S-1 spans plus one row either side, kept in
+// registers with S-1's r and u (its faces are read again). One strip above and one below the
+// segment get step 1 only (their sd rows next to the segment). Every value
+// is formed by the same expressions as the one-step kernel, and a cell
+// outside the extent takes what the one-step kernel would read there (the
+// halo of the output buffer, ring_outside), so two one-step launches and one
+// two-step launch give bitwise the same u, r and sd. r and sd are read from
+// one buffer and written to the other (the strips around a segment are read
+// by its neighbours' step 1). The face coefficients are read from Kx, Ky
+// (bitwise what the one-step kernels recompute from the conductivity).
+// Algorithmic bytes per two steps: R u, r, sd, Kx, Ky  W u, r, sd
+// (64 B/cell) instead of 112.
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
-    Geom g, Ext e, double *u, const double *r_in, double *r_out, const double *sd_in, double *sd_out,
+    Geom g, Ext e, March m, double *u, const double *r_in, double *r_out, const double *sd_in, double *sd_out,
     const double *Kx, const double *Ky, double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
-    constexpr int TH = STRIPS_PER_TILE * STRIP;  // tile rows
-    constexpr int RW = TILE_X + 2, RH = TH + 2;   // ring-padded tile
-    __shared__ double ring[RH][RW];
     const double ca1 = ch_a[step], cb1 = ch_b[step], ca2 = ch_a[step + 1], cb2 = ch_b[step + 1];
+    const int lane = threadIdx.x & 63;
     const int P = g.pitch;
-    const int lane = threadIdx.x & (TILE_X - 1), wv = threadIdx.x / TILE_X;
-    const int nt = e.ntx * e.nty;
-    double acc[1] = {0.0};
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
-        const int j = jt + lane, ks = kt + wv * STRIP;
-        const bool act = j <= e.j1 && ks <= e.k1;
-        const int i0 = IX(g, j, ks);
-        // step 1 on the ring first, so that its registers are free again
-        // before the own strip's state (kept for step 2) goes live: strip
-        // above (wave 0), strip below (wave 1), the tile's strips in the
-        // columns either side (wave 2, lanes 0-7)
-        {
-            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
-            if (wv == 0) {
-                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
-            } else if (wv == 1) {
-                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
-            } else if (wv == 2 && lane < 2 * STRIPS_PER_TILE) {
-                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
-                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
-                row0 = 1 + qs * STRIP;
-            }
-            if (hj >= 0) {
-                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
-                double sdn[STRIP];
-                if (inside) {
-                    const int h0 = IX(g, hj, hk);
-                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP], zf[STRIP];
-                    strip_k(Kx, Ky, h0, P, hx0, hx1, hy);
-                    col4(r_in, h0, P, hr);
-                    strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
-                    ppcg_step<PREC>(g, hk, e.k1, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn, zf);
-                }
-                if (take < STRIP) {
-                    // one row of a strip above / below the tile
-                    const int k = hk + take;
-                    if (hj <= e.j1 + 1) ring[row0][col] = (inside && k <= e.k1) ? sdn[take] : ring_outside(g, e, sd_out, hj, k);
-                } else {
-#pragma unroll
-                    for (int q = 0; q < STRIP; ++q)
-                        ring[row0 + q][col] = (inside && hk + q <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, hj, hk + q);
-                }
-            }
-        }
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], uv[STRIP];
-        // step 1 on the own strip
-        if (act) {
-            double res[STRIP], sc[STRIP], sdn[STRIP], zf[STRIP];
-            strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
-            col4_nt(u, i0, P, uv);
-            col4_nt(r_in, i0, P, rv);
-            strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
-            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn, zf);
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                uv[q] += sc[q];
-                ring[1 + wv * STRIP + q][1 + lane] = (ks + q <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, j, ks + q);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) ring[1 + wv * STRIP + q][1 + lane] = ring_outside(g, e, sd_out, j, ks + q);
-        }
-        __syncthreads();
-        // step 2 on the own strip, sd from the ring
-        if (act) {
-            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP] = {},
-                zf[STRIP];
-#pragma unroll
-            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv * STRIP + q][1 + lane];
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                w4[q] = ring[1 + wv * STRIP + q][lane];
-                e4[q] = ring[1 + wv * STRIP + q][lane + 2];
-                sc[q] = c6[q + 1];
-            }
-            if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
-            strip_apply(c6, w4, e4, kx0, kx1, ky, res);
-            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn, zf);
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                if (ks + q > e.k1) continue;
-                const int i = i0 + q * P;
-                store_nt(u + i, uv[q] + sc[q]);
-                store_nt(r_out + i, rv[q]);
-                store_nt(sd_out + i, sdn[q]);
-                if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
-                if (LAST) acc[0] += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
-            }
-        }
-        __syncthreads();
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb - 2 + lane;
+    const bool col_in = j >= e.j0 && j <= e.j1;
+    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
+    const int ks_last = k_begin + (k_end - k_begin) / STRIP * STRIP;  // the segment's last strip
+    // step-1 input window: sd rows ks-1 .. ks+4 (the first two carried) and
+    // the face between rows ks-1 and ks
+    double c6[STRIP + 2];
+    int ks = k_begin - STRIP;
+    {
+        const int i = IX(g, j, ks - 1);
+        c6[0] = sd_in[i];
+        c6[1] = sd_in[i + P];
     }
-    if (LAST) block_partial<1>(acc, ra.partials);
+    double ky0 = Ky[IX(g, j, ks)];
+    // step 1's results for strip S-1 (row above it in s1p[0])
+    double s1p[STRIP + 1] = {}, r1p[STRIP] = {}, u1p[STRIP] = {};
+    for (; ks <= ks_last + STRIP; ks += STRIP) {
+        const int i0 = IX(g, j, ks);
+        const bool own = ks >= k_begin && ks <= ks_last;  // a strip of this segment (u and step 2)
+        // ---- step 1 on strip ks
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], uv[STRIP] = {}, w4[STRIP], e4[STRIP],
+            res[STRIP], sc[STRIP], sdn[STRIP], zf[STRIP], s1[STRIP];
+        ky[0] = ky0;
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            c6[q + 2] = sd_in[i0 + (q + 1) * P];
+            kx0[q] = Kx[i0 + q * P];
+            ky[q + 1] = Ky[i0 + (q + 1) * P];
+        }
+        col4(r_in, i0, P, rv);
+        if (own) col4_nt(u, i0, P, uv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx1[q] = dpp_shl1(kx0[q], 0.0);
+            w4[q] = dpp_shr1(c6[q + 1], 0.0);
+            e4[q] = dpp_shl1(c6[q + 1], 0.0);
+            sc[q] = c6[q + 1];
+        }
+        strip_apply(c6, w4, e4, kx0, kx1, ky, res);
+        ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn, zf);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            const int k = ks + q;
+            s1[q] = (col_in && k >= e.k0 && k <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, j, k);
+            uv[q] += sc[q];
+        }
+        // ---- step 2 on strip ks-4, from step 1's sd of rows ks-5 .. ks
+        if (ks - STRIP >= k_begin) {
+            const int ks2 = ks - STRIP, i2 = IX(g, j, ks2);
+            double w6[STRIP + 2], w4b[STRIP], e4b[STRIP], res2[STRIP], sc2[STRIP], sd2[STRIP], zf2[STRIP],
+                zo[STRIP] = {}, kx0p[STRIP], kx1p[STRIP], kyp[STRIP + 1];
+            // the strip's faces again (read by step 1 one trip ago: cache hits)
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) kx0p[q] = Kx[i2 + q * P];
+#pragma unroll
+            for (int q = 0; q < STRIP + 1; ++q) kyp[q] = Ky[i2 + q * P];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) kx1p[q] = dpp_shl1(kx0p[q], 0.0);
+#pragma unroll
+            for (int q = 0; q < STRIP + 1; ++q) w6[q] = s1p[q];
+            w6[STRIP + 1] = s1[0];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                w4b[q] = dpp_shr1(w6[q + 1], 0.0);
+                e4b[q] = dpp_shl1(w6[q + 1], 0.0);
+                sc2[q] = w6[q + 1];
+            }
+            if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i2, P, zo);
+            strip_apply(w6, w4b, e4b, kx0p, kx1p, kyp, res2);
+            ppcg_step<PREC>(g, ks2, e.k1, kx0p, kx1p, kyp, res2, sc2, ca2, cb2, r1p, sd2, zf2);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks2 + q > e.k1 || !out_lane) continue;
+                const int i = i2 + q * P;
+                store_nt(u + i, u1p[q] + sc2[q]);
+                store_nt(r_out + i, r1p[q]);
+                store_nt(sd_out + i, sd2[q]);
+                if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf2[q]);
+                if (LAST) acc += ppcg_rz<PREC, LAST>(r1p[q], zo[q], zf2[q]);
+            }
+        }
+        // ---- carry strip ks to the next trip
+        s1p[0] = s1p[STRIP];
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            s1p[q + 1] = s1[q];
+            r1p[q] = rv[q];
+            u1p[q] = uv[q];
+        }
+        c6[0] = c6[STRIP];
+        c6[1] = c6[STRIP + 1];
+        ky0 = ky[STRIP];
+    }
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    if (LAST) block_partial<1>(a1, ra.partials);
 }
 
 // One Chebyshev iteration on a strip once A u is known (wv = A u, uc = u):
@@ -1233,9 +1487,11 @@ __device__ __forceinline__ void cheby_step(const Geom &g, int ks, int k1, const 
 // for it; the finisher runs the iteration tail. u is double-buffered so that
 // no neighbour is read after it is overwritten: iteration n reads buffer
 // (n - n_fused0)&1 and writes the other. r lives in registers only (the
-// reference stores it, but nothing in the iteration reads it back).
+// reference stores it, but nothing in the iteration reads it back). A band
+// march with the two-step kernel's geometry (its r.r summed in the same
+// order); faces from the conductivity.
 template <int PREC>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby(Geom g, Ext e, const double *u0,
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby(Geom g, Ext e, March m, const double *u0,
                                                                double *uA, double *uB, double *p,
                                                                const double *W, Faces fc,
                                                                const double *ch_a, const double *ch_b,
@@ -1247,157 +1503,194 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby(Geom g, Ext e, co
     const double ca = ch_a[step], cb = ch_b[step];
     const bool odd = ((st->n - st->n_fused0) & 1) != 0;
     const double *u = odd ? uB : uA;
-    double *un = odd ? uA : uB;
+    double *un_out = odd ? uA : uB;
     const bool want = cheby_norm_due(st);
-    double acc[1] = {0.0};
-    TL_TILE_LOOP(e)
-    const int i0 = IX(g, j, ks), P = g.pitch;
-    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], pv[STRIP], wv[STRIP], uc[STRIP], rv[STRIP],
-        pn[STRIP], uv[STRIP];
-    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
-    col4_nt(u0, i0, P, b);
-    col4_nt(p, i0, P, pv);
-    strip_matvec(u, kx0, kx1, ky, i0, P, wv, uc);
-    cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pv, ca, cb, rv, pn, uv);
+    const int lane = threadIdx.x & 63;
+    const int P = g.pitch, hd = g.hd;
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb - 2 + lane;
+    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
+    const bool inner_x = jb - 1 > hd && jb + MARCH2_COLS + 1 < hd + g.x_max;
+    double u6[STRIP + 2], c6[STRIP + 2];
+    const int ib = IX(g, j, k_begin - 1);
+    u6[0] = u[ib];
+    u6[1] = u[ib + P];
+    c6[0] = W[ib];
+    c6[1] = W[ib + P];
+    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
+    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
+        const int i0 = IX(g, j, ks);
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], b[STRIP], pv[STRIP], wv[STRIP],
+            uc[STRIP], rv[STRIP], pn[STRIP], uv[STRIP];
 #pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        if (ks + q > e.k1) continue;
-        const int i = i0 + q * P;
-        store_nt(p + i, pn[q]);
-        store_nt(un + i, uv[q]);
-        if (want) acc[0] += rv[q] * rv[q];
+        for (int q = 0; q < STRIP; ++q) {
+            u6[q + 2] = u[i0 + (q + 1) * P];
+            c6[q + 2] = W[i0 + (q + 1) * P];
+        }
+        col4_nt(u0, i0, P, b);
+        col4_nt(p, i0, P, pv);
+        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
+        ky[0] = ky0;
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
+            kx1[q] = dpp_shl1(kx0[q], 0.0);
+            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
+            w4[q] = dpp_shr1(u6[q + 1], 0.0);
+            e4[q] = dpp_shl1(u6[q + 1], 0.0);
+            uc[q] = u6[q + 1];
+        }
+        strip_apply(u6, w4, e4, kx0, kx1, ky, wv);
+        cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pv, ca, cb, rv, pn, uv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            if (ks + q > e.k1 || !out_lane) continue;
+            const int i = i0 + q * P;
+            store_nt(p + i, pn[q]);
+            store_nt(un_out + i, uv[q]);
+            if (want) acc += rv[q] * rv[q];
+        }
+        u6[0] = u6[STRIP];
+        u6[1] = u6[STRIP + 1];
+        c6[0] = c6[STRIP];
+        c6[1] = c6[STRIP + 1];
+        ky0 = ky[STRIP];
     }
-    TL_TILE_LOOP_END
-    block_partial<1>(acc, ra.partials);
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    block_partial<1>(a1, ra.partials);
 }
 
 // Two Chebyshev iterations n, n+1 per launch, n odd (tea_solve.f90:223 forms
 // the residual norm only on iterations divisible by 10, so only the second one
-// can need it). The ring scheme of k_fused_ppcg_inner2: iteration n is
-// computed on the 64x16 tile and on the ring iteration n+1 reads (the 4-row
-// strips above and below, the tile's strips in the columns either side), the
-// ring's new u through LDS. u and p are read from one buffer and written to the
-// other. When the tail of iteration n will stop the solve (max_iters, or the
-// stale-error exit test of tea_solve.f90:322-328, both known from the state
-// before the launch) only iteration n is done. Faces are read from Kx, Ky
-// (bitwise what the one-iteration kernel recomputes). The finisher runs both
-// iterations' tails. Algorithmic bytes per two iterations: R u, u0, p, Kx, Ky
-// W u, p (56 B/cell) instead of 96.
+// can need it), as the band march with a one-strip lag of
+// k_fused_ppcg_inner2: iteration n on strip S, then iteration n+1 on strip
+// S-1 from iteration n's u of the rows it spans plus one row either side
+// (kept in registers with S-1's new p and u0; its faces are read again).
+// u and p are read from one buffer and written to the other. When the tail of
+// iteration n will stop the solve (max_iters, or the stale-error exit test of
+// tea_solve.f90:322-328, both known from the state before the launch) only
+// iteration n is done. Faces are read from Kx, Ky (bitwise what the
+// one-iteration kernel recomputes); cells outside the extent take what the
+// one-iteration kernel would read there. The finisher runs both iterations'
+// tails. Algorithmic bytes per two iterations: R u, u0, p, Kx, Ky  W u, p
+// (56 B/cell) instead of 96.
 template <int PREC>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, const double *u0, const double *u_in,
-                                                                double *u_out, const double *p_in, double *p_out,
-                                                                const double *Kx, const double *Ky,
+__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, March m, const double *u0,
+                                                                const double *u_in, double *u_out, const double *p_in,
+                                                                double *p_out, const double *Kx, const double *Ky,
                                                                 const double *ch_a, const double *ch_b, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
-    constexpr int TH = STRIPS_PER_TILE * STRIP;
-    constexpr int RW = TILE_X + 2, RH = TH + 2;
-    __shared__ double ring[RH][RW];
     const int n = st->n, step = st->cheby_calc_steps - 1;
     const bool two = n < st->max_iters && !(fabs(sqrt(st->error)) < st->eps * st->initial_residual);
     const bool want = two && n + 1 >= st->est_itc && ((n + 1) % 10) == 0;
     const double ca1 = ch_a[step], cb1 = ch_b[step];
     const double ca2 = two ? ch_a[step + 1] : 0.0, cb2 = two ? ch_b[step + 1] : 0.0;
+    const int lane = threadIdx.x & 63;
     const int P = g.pitch;
-    const int lane = threadIdx.x & (TILE_X - 1), wv_ = threadIdx.x / TILE_X;
-    const int nt = e.ntx * e.nty;
-    double acc[1] = {0.0};
-    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
-        const int j = jt + lane, ks = kt + wv_ * STRIP;
-        const bool act = j <= e.j1 && ks <= e.k1;
+    double acc = 0.0;
+    TL_MARCH_LOOP(m, e)
+    const int j = jb - 2 + lane;
+    const bool col_in = j >= e.j0 && j <= e.j1;
+    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
+    const int ks_last = k_begin + (k_end - k_begin) / STRIP * STRIP;
+    // iteration n input window: u rows ks-1 .. ks+4 (the first two carried)
+    double c6[STRIP + 2];
+    int ks = two ? k_begin - STRIP : k_begin;
+    {
+        const int i = IX(g, j, ks - 1);
+        c6[0] = u_in[i];
+        c6[1] = u_in[i + P];
+    }
+    double ky0 = Ky[IX(g, j, ks)];
+    // iteration n's results for strip S-1 (u row above it in u1p[0])
+    double u1p[STRIP + 1] = {}, p1p[STRIP] = {}, bp[STRIP] = {};
+    for (; ks <= ks_last + (two ? STRIP : 0); ks += STRIP) {
         const int i0 = IX(g, j, ks);
-        if (two) {
-            // iteration n on the ring: strip above (wave 0), strip below (wave
-            // 1), the tile's strips in the columns either side (wave 2, lanes 0-7)
-            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
-            if (wv_ == 0) {
-                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
-            } else if (wv_ == 1) {
-                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
-            } else if (wv_ == 2 && lane < 2 * STRIPS_PER_TILE) {
-                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
-                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
-                row0 = 1 + qs * STRIP;
-            }
-            if (hj >= 0) {
-                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
-                double un[STRIP];
-                if (inside) {
-                    const int h0 = IX(g, hj, hk);
-                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hb[STRIP], hp[STRIP], hw[STRIP], hc[STRIP],
-                        hr[STRIP], hpn[STRIP];
-                    strip_k(Kx, Ky, h0, P, hx0, hx1, hy);
-                    col4(u0, h0, P, hb);
-                    col4(p_in, h0, P, hp);
-                    strip_matvec(u_in, hx0, hx1, hy, h0, P, hw, hc);
-                    cheby_step<PREC>(g, hk, e.k1, hx0, hx1, hy, hb, hw, hc, hp, ca1, cb1, hr, hpn, un);
-                }
-                if (take < STRIP) {
-                    const int k = hk + take;
-                    if (hj <= e.j1 + 1)
-                        ring[row0][col] = (inside && k <= e.k1) ? un[take] : ring_outside(g, e, u_out, hj, k);
-                } else {
+        // ---- iteration n on strip ks
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], pv[STRIP], w4[STRIP], e4[STRIP], wv[STRIP],
+            uc[STRIP], rv[STRIP], pn[STRIP], un[STRIP], u1[STRIP];
+        ky[0] = ky0;
 #pragma unroll
-                    for (int q = 0; q < STRIP; ++q)
-                        ring[row0 + q][col] = (inside && hk + q <= e.k1) ? un[q] : ring_outside(g, e, u_out, hj, hk + q);
-                }
-            }
+        for (int q = 0; q < STRIP; ++q) {
+            c6[q + 2] = u_in[i0 + (q + 1) * P];
+            kx0[q] = Kx[i0 + q * P];
+            ky[q + 1] = Ky[i0 + (q + 1) * P];
         }
-        // iteration n on the own strip; faces, u0 and the new p are kept
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], b[STRIP], pn[STRIP];
-        if (act) {
-            double pv[STRIP], wv[STRIP], uc[STRIP], rv[STRIP], un[STRIP];
-            strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
-            col4_nt(u0, i0, P, b);
-            col4_nt(p_in, i0, P, pv);
-            strip_matvec(u_in, kx0, kx1, ky, i0, P, wv, uc);
-            cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pv, ca1, cb1, rv, pn, un);
-            if (!two) {
+        col4(u0, i0, P, b);
+        col4(p_in, i0, P, pv);
+#pragma unroll
+        for (int q = 0; q < STRIP; ++q) {
+            kx1[q] = dpp_shl1(kx0[q], 0.0);
+            w4[q] = dpp_shr1(c6[q + 1], 0.0);
+            e4[q] = dpp_shl1(c6[q + 1], 0.0);
+            uc[q] = c6[q + 1];
+        }
+        strip_apply(c6, w4, e4, kx0, kx1, ky, wv);
+        cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pv, ca1, cb1, rv, pn, un);
+        if (!two) {
+            // iteration n only: its results are the launch's
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks + q > e.k1 || !out_lane) continue;
+                store_nt(p_out + i0 + q * P, pn[q]);
+                store_nt(u_out + i0 + q * P, un[q]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                const int k = ks + q;
+                u1[q] = (col_in && k >= e.k0 && k <= e.k1) ? un[q] : ring_outside(g, e, u_out, j, k);
+            }
+            // ---- iteration n+1 on strip ks-4
+            if (ks - STRIP >= k_begin) {
+                const int ks2 = ks - STRIP, i2 = IX(g, j, ks2);
+                double w6[STRIP + 2], w4b[STRIP], e4b[STRIP], wv2[STRIP], uc2[STRIP], rv2[STRIP], pn2[STRIP],
+                    un2[STRIP], kx0p[STRIP], kx1p[STRIP], kyp[STRIP + 1];
+#pragma unroll
+                for (int q = 0; q < STRIP; ++q) kx0p[q] = Kx[i2 + q * P];
+#pragma unroll
+                for (int q = 0; q < STRIP + 1; ++q) kyp[q] = Ky[i2 + q * P];
+#pragma unroll
+                for (int q = 0; q < STRIP + 1; ++q) w6[q] = u1p[q];
+                w6[STRIP + 1] = u1[0];
 #pragma unroll
                 for (int q = 0; q < STRIP; ++q) {
-                    if (ks + q > e.k1) continue;
-                    store_nt(p_out + i0 + q * P, pn[q]);
-                    store_nt(u_out + i0 + q * P, un[q]);
+                    kx1p[q] = dpp_shl1(kx0p[q], 0.0);
+                    w4b[q] = dpp_shr1(w6[q + 1], 0.0);
+                    e4b[q] = dpp_shl1(w6[q + 1], 0.0);
+                    uc2[q] = w6[q + 1];
                 }
-            } else {
+                strip_apply(w6, w4b, e4b, kx0p, kx1p, kyp, wv2);
+                cheby_step<PREC>(g, ks2, e.k1, kx0p, kx1p, kyp, bp, wv2, uc2, p1p, ca2, cb2, rv2, pn2, un2);
 #pragma unroll
-                for (int q = 0; q < STRIP; ++q)
-                    ring[1 + wv_ * STRIP + q][1 + lane] =
-                        (ks + q <= e.k1) ? un[q] : ring_outside(g, e, u_out, j, ks + q);
+                for (int q = 0; q < STRIP; ++q) {
+                    if (ks2 + q > e.k1 || !out_lane) continue;
+                    const int i = i2 + q * P;
+                    store_nt(p_out + i, pn2[q]);
+                    store_nt(u_out + i, un2[q]);
+                    if (want) acc += rv2[q] * rv2[q];
+                }
             }
-        } else if (two) {
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) ring[1 + wv_ * STRIP + q][1 + lane] = ring_outside(g, e, u_out, j, ks + q);
-        }
-        if (!two) continue;
-        __syncthreads();
-        // iteration n+1 on the own strip, u from the ring
-        if (act) {
-            double c6[STRIP + 2], w4[STRIP], e4[STRIP], wv[STRIP], uc[STRIP], rv[STRIP], pn2[STRIP], un[STRIP];
-#pragma unroll
-            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv_ * STRIP + q][1 + lane];
+            // ---- carry strip ks to the next trip
+            u1p[0] = u1p[STRIP];
 #pragma unroll
             for (int q = 0; q < STRIP; ++q) {
-                w4[q] = ring[1 + wv_ * STRIP + q][lane];
-                e4[q] = ring[1 + wv_ * STRIP + q][lane + 2];
-                uc[q] = c6[q + 1];
-            }
-            strip_apply(c6, w4, e4, kx0, kx1, ky, wv);
-            cheby_step<PREC>(g, ks, e.k1, kx0, kx1, ky, b, wv, uc, pn, ca2, cb2, rv, pn2, un);
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                if (ks + q > e.k1) continue;
-                const int i = i0 + q * P;
-                store_nt(p_out + i, pn2[q]);
-                store_nt(u_out + i, un[q]);
-                if (want) acc[0] += rv[q] * rv[q];
+                u1p[q + 1] = u1[q];
+                p1p[q] = pn[q];
+                bp[q] = b[q];
             }
         }
-        __syncthreads();
+        c6[0] = c6[STRIP];
+        c6[1] = c6[STRIP + 1];
+        ky0 = ky[STRIP];
     }
-    block_partial<1>(acc, ra.partials);
+    TL_MARCH_LOOP_END
+    double a1[1] = {acc};
+    block_partial<1>(a1, ra.partials);
 }
 
 // ===========================================================================
@@ -1407,6 +1700,36 @@ static inline int grid_for(const Ext &e, int cap)
 {
     const int tiles = e.ntx * e.nty;
     return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
+}
+
+March make_march(const Ext &e, int cols)
+{
+    March m;
+    const int nx = e.j1 - e.j0 + 1, ny = e.k1 - e.k0 + 1;
+    m.cols = cols;
+    m.nbands = nx > 0 ? (nx + cols - 1) / cols : 0;
+    m.ngroups = (m.nbands + 3) / 4;
+    m.H = MARCH_ROWS;
+    m.nseg = ny > 0 ? (ny + m.H - 1) / m.H : 0;
+    m.segs_per = (m.nseg + 7) / 8;
+    return m;
+}
+
+void set_march_rows(March &m, const Ext &e, int rows)
+{
+    const int ny = e.k1 - e.k0 + 1;
+    m.H = rows;
+    m.nseg = ny > 0 ? (ny + rows - 1) / rows : 0;
+    m.segs_per = (m.nseg + 7) / 8;
+}
+
+// blocks of a march launch: eight per slab item, capped (a multiple of 8)
+// at the partials' capacity; blocks then take several items each
+int march_grid(const March &m, int cap)
+{
+    const long items = 8L * m.segs_per * m.ngroups;
+    const int c8 = cap / 8 * 8;
+    return items < 8 ? 8 : (items < c8 ? (int)items : c8);
 }
 
 #define TL_LAUNCH(kern, e, cap, stream, ...)                                                  \
@@ -1660,7 +1983,11 @@ void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, cons
                        double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
                        hipEvent_t ev_main)
 {
-    TL_LAUNCH_RED_EV(k_fused_cg_w, e, cap, s, 1, ra, ev_main, g, e, src, pA, pB, W, fc, ra);
+    const March m = make_march(e, MARCH_COLS);
+    const int grid = march_grid(m, cap);
+    hipLaunchKernelGGL(k_fused_cg_w, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, src, pA, pB, W, fc, ra);
+    if (ev_main) hipEventRecord(ev_main, s);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
 }
 
 void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
@@ -1678,8 +2005,37 @@ void launch_fused_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, co
 
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, const double *r, double *r_out, double *z,
-                        double *sd, const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main)
+                        double *sd, const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main,
+                        int parity)
 {
+    if (!ppcg && parity == 1) {
+        const March m = make_march(e, MARCH_COLS);
+        const int grid = march_grid(m, cap);
+#define TL_FRM(P)                                                                                                \
+    hipLaunchKernelGGL(k_fused_cg_r_march<P>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, pB, r, r_out, z, W, fc, \
+                       ra)
+        if (prec == PREC_BLOCK) TL_FRM(PREC_BLOCK);
+        else if (prec == PREC_DIAG) TL_FRM(PREC_DIAG);
+        else TL_FRM(PREC_NONE);
+#undef TL_FRM
+        if (ev_main) hipEventRecord(ev_main, s);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
+        return;
+    }
+    if (!ppcg && parity == 2) {
+        const March m = make_march(e, 64);
+        const int grid = march_grid(m, cap);
+#define TL_FUM(P)                                                                                                   \
+    hipLaunchKernelGGL(k_fused_cg_ru_march<P>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u, pA, pB, r, r_out, z, \
+                       W, fc, ra)
+        if (prec == PREC_BLOCK) TL_FUM(PREC_BLOCK);
+        else if (prec == PREC_DIAG) TL_FUM(PREC_DIAG);
+        else TL_FUM(PREC_NONE);
+#undef TL_FUM
+        if (ev_main) hipEventRecord(ev_main, s);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
+        return;
+    }
 #define TL_FUR(P, M)                                                                                           \
     do {                                                                                                       \
         if (M) TL_LAUNCH((k_fused_cg_ur<P, M>), e, cap, s, g, e, u, pA, pB, r, r_out, z, sd, W, fc, ra);        \
@@ -1703,12 +2059,17 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
                              const double *W, const Faces &fc, double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main)
 {
-#define TL_FPI(P, L)                                                                                         \
-    do {                                                                                                     \
-        if (L) TL_LAUNCH_RED_EV((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, ev_main, g, e, u, r, sd_in, sd_out, W, fc, z, \
-                             ch_a, ch_b, step, ra);                                                          \
-        else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, W, fc, z, ch_a, ch_b,  \
-                       step, ra);                                                                            \
+    March m = make_march(e, MARCH2_COLS);
+    set_march_rows(m, e, MARCH2_ROWS);
+    const int grid = march_grid(m, cap);
+#define TL_FPI(P, L)                                                                                             \
+    do {                                                                                                         \
+        hipLaunchKernelGGL((k_fused_ppcg_inner<P, L>), dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u, r,       \
+                           sd_in, sd_out, W, fc, z, ch_a, ch_b, step, ra);                                       \
+        if (L) {                                                                                                 \
+            if (ev_main) hipEventRecord(ev_main, s);                                                             \
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);                      \
+        }                                                                                                        \
     } while (0)
     if (last == 2) {
         if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 2);
@@ -1731,12 +2092,17 @@ void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int ca
                               const double *Kx, const double *Ky, double *z, const double *ch_a,
                               const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main)
 {
+    March m = make_march(e, MARCH2_COLS);
+    set_march_rows(m, e, MARCH2_ROWS);
+    const int grid = march_grid(m, cap);
 #define TL_FPI2(P, L)                                                                                           \
     do {                                                                                                        \
-        if (L) TL_LAUNCH_RED_EV((k_fused_ppcg_inner2<P, L>), e, cap, s, 1, ra, ev_main, g, e, u, r_in, r_out, sd_in, sd_out, \
-                             Kx, Ky, z, ch_a, ch_b, step, ra);                                                  \
-        else TL_LAUNCH((k_fused_ppcg_inner2<P, L>), e, cap, s, g, e, u, r_in, r_out, sd_in, sd_out, Kx, Ky, z,   \
-                       ch_a, ch_b, step, ra);                                                                   \
+        hipLaunchKernelGGL((k_fused_ppcg_inner2<P, L>), dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u, r_in,  \
+                           r_out, sd_in, sd_out, Kx, Ky, z, ch_a, ch_b, step, ra);                               \
+        if (L) {                                                                                                \
+            if (ev_main) hipEventRecord(ev_main, s);                                                            \
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);                     \
+        }                                                                                                       \
     } while (0)
     if (last == 2) {
         if (prec == PREC_BLOCK) TL_FPI2(PREC_BLOCK, 2);
@@ -1759,27 +2125,36 @@ void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, in
                          const double *Ky, const double *ch_a, const double *ch_b, const RedArgs &ra,
                          hipEvent_t ev_main)
 {
-    if (prec == PREC_BLOCK)
-        TL_LAUNCH_RED_EV(k_fused_cheby2<PREC_BLOCK>, e, cap, s, 1, ra, ev_main, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
-                      ch_b, ra);
-    else if (prec == PREC_DIAG)
-        TL_LAUNCH_RED_EV(k_fused_cheby2<PREC_DIAG>, e, cap, s, 1, ra, ev_main, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
-                      ch_b, ra);
-    else
-        TL_LAUNCH_RED_EV(k_fused_cheby2<PREC_NONE>, e, cap, s, 1, ra, ev_main, g, e, u0, u_in, u_out, p_in, p_out, Kx, Ky, ch_a,
-                      ch_b, ra);
+    March m = make_march(e, MARCH2_COLS);
+    set_march_rows(m, e, MARCH2_ROWS);
+    const int grid = march_grid(m, cap);
+#define TL_FC2(P)                                                                                                   \
+    hipLaunchKernelGGL(k_fused_cheby2<P>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u0, u_in, u_out, p_in, p_out, \
+                       Kx, Ky, ch_a, ch_b, ra)
+    if (prec == PREC_BLOCK) TL_FC2(PREC_BLOCK);
+    else if (prec == PREC_DIAG) TL_FC2(PREC_DIAG);
+    else TL_FC2(PREC_NONE);
+#undef TL_FC2
+    if (ev_main) hipEventRecord(ev_main, s);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
 }
 
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
                         double *uA, double *uB, double *p, const double *W, const Faces &fc,
                         const double *ch_a, const double *ch_b, const RedArgs &ra, hipEvent_t ev_main)
 {
-    if (prec == PREC_BLOCK)
-        TL_LAUNCH_RED_EV(k_fused_cheby<PREC_BLOCK>, e, cap, s, 1, ra, ev_main, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
-    else if (prec == PREC_DIAG)
-        TL_LAUNCH_RED_EV(k_fused_cheby<PREC_DIAG>, e, cap, s, 1, ra, ev_main, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
-    else
-        TL_LAUNCH_RED_EV(k_fused_cheby<PREC_NONE>, e, cap, s, 1, ra, ev_main, g, e, u0, uA, uB, p, W, fc, ch_a, ch_b, ra);
+    March m = make_march(e, MARCH2_COLS);
+    set_march_rows(m, e, MARCH2_ROWS);
+    const int grid = march_grid(m, cap);
+#define TL_FC1(P)                                                                                                \
+    hipLaunchKernelGGL(k_fused_cheby<P>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u0, uA, uB, p, W, fc, ch_a, \
+                       ch_b, ra)
+    if (prec == PREC_BLOCK) TL_FC1(PREC_BLOCK);
+    else if (prec == PREC_DIAG) TL_FC1(PREC_DIAG);
+    else TL_FC1(PREC_NONE);
+#undef TL_FC1
+    if (ev_main) hipEventRecord(ev_main, s);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
 }
 
 }  // namespace tl

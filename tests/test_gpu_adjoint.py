@@ -94,10 +94,15 @@ def test_finite_differences_config4(hiplib):
 def test_finite_differences_config4_full_size(hiplib):
     """The adjoint workload at its own size, 1024x1024, 5 timesteps: 32
     seeded sensitive cells, central differences with h = 1e-4 relative. At
-    this size a central difference with h = 1e-6 resolves the loss only to
-    the FP64 forward solve's rounding floor (median 6e-6, worst 8e-5 against
-    the adjoint); the error falls as 1/h down to h = 1e-4 (median 8e-8), where
-    the O(h^2) truncation is still far below 1e-6."""
+    this size a central difference resolves the loss only to the FP64
+    forward solves' rounding floor: with h = 1e-6 the error against the
+    adjoint is that floor over h (median 6e-6, worst 8e-5 of the 32 cells),
+    and it falls as 1/h (h = 1e-5: 9e-7 / 7e-6); at h = 1e-4 the floor's
+    share is ~1e-7 median / ~1e-6 worst, and the O(h^2) truncation is below
+    1e-8 (h = 1e-3 gives 3e-8 / 7e-7). Which rounding the floor takes moves
+    with the order of the solves' sums, so the bounds are three times those
+    figures. (The exact check of the adjoint at this size is the comparison
+    with the CPU reference adjoint below.)"""
     from tealeaf_opencl_amd import adjoint
     n, nsteps, dt, coef = 1024, 5, 0.004, 1
     c = hiplib.Chunk(n, n, 1, 3)
@@ -117,7 +122,8 @@ def test_finite_differences_config4_full_size(hiplib):
             vals.append(c.forward_loss(coef, dt, nsteps, u_obs, tol=1e-14))
         fd = (vals[0] - vals[1]) / (2.0 * h)
         errs.append(abs(fd - grad[k, j]) / abs(grad[k, j]))
-    assert max(errs) <= 1e-6, (np.median(errs), max(errs))
+    print("h = 1e-4: median %.2e, worst %.2e" % (np.median(errs), max(errs)))
+    assert np.median(errs) <= 3e-7 and max(errs) <= 3e-6, (np.median(errs), max(errs))
 
 
 def test_config4_full_size_vs_cpu_reference(hiplib):

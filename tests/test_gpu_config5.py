@@ -16,9 +16,8 @@ of A sum to 1, so sum(u) = sum(u0) - sum(r) and the internal energy
 sum(vol rho e1) moves only by the solve's residual; the refined solves'
 true residuals, formed by the library's own residual and norm kernels, are
 below 1e-10 of the initial ones; the mixed and FP64 refined summaries agree
-as closely as the measured run of all 20 timesteps did (2.8e-14 on the
-internal energy, 1.1e-14 on the temperature); and the adjoint's directional
-derivative matches a central difference of the GPU forward model."""
+within what their residuals allow; and the adjoint's directional derivative matches a central
+difference of the GPU forward model."""
 import numpy as np
 import pytest
 
@@ -31,7 +30,7 @@ DT = 0.004
 
 
 def painted_density_sum(n=N):
-    """Sum over the interior of the density generate_chunk paints (numpy
+    """Sum and 2-norm over the interior of the density generate_chunk paints (numpy
     restatement of generate_chunk_cl.cl:3-65 for rectangles, with the dx/100
     shrink of read_input.f90:320-327)."""
     dx = 10.0 / float(np.float32(n))
@@ -43,11 +42,11 @@ def painted_density_sum(n=N):
         cx = (v[1:] >= x0) & (v[:-1] < x1)
         cy = (v[1:] >= y0) & (v[:-1] < y1)
         rho[np.ix_(cy, cx)] = s["density"]
-    return float(np.sum(rho)), dx
+    return float(np.sum(rho)), float(np.sqrt(np.sum(rho * rho))), dx
 
 
 def test_config5_ppcg_one_timestep(hiplib, tmp_path):
-    rho_sum, dx = painted_density_sum()
+    rho_sum, rho_norm, dx = painted_density_sum()
     mass_want = dx * dx * rho_sum
     # reference FP64 PPCG, device-resident driver
     deck = decks.config5_deck(str(tmp_path / "c5.in"), precision="reference", n=N, steps=1, summary_frequency=1)
@@ -63,7 +62,7 @@ def test_config5_ppcg_one_timestep(hiplib, tmp_path):
     print("reference PPCG: outer %d, inner %d, ie drift %.2e" % (lg["solve"][0][1], lg["solve"][0][4],
                                                                  abs(s1[4] - s0[4]) / s0[4]))
 
-    out = {}
+    out, r_norm = {}, {}
     for precision in (0, 1):
         c = hiplib.Chunk(N, N, 1, 3)
         try:
@@ -88,12 +87,24 @@ def test_config5_ppcg_one_timestep(hiplib, tmp_path):
         assert rel <= 1e-10 and abs(rel - res["rel_residual"]) <= 1e-2 * 1e-10
         assert before[0] == 100.0 and after[0] == 100.0
         assert before[1] == after[1] and abs(after[1] - mass_want) <= 1e-13 * mass_want
-        assert abs(after[2] - before[2]) <= 1e-9 * before[2]
+        # sum(u) moves by -sum(r), |sum(r)| <= sqrt(N) |r|_2: the internal
+        # energy sum(vol u) moves by at most vol * sqrt(N) * |r|_2 (plus the
+        # rounding of u / rho * rho and of the sums)
+        r_norm[precision] = rel * np.sqrt(rr0)
+        drift = abs(after[2] - before[2])
+        print("  internal energy drift %.2e, bound %.2e" % (drift, dx * dx * N * r_norm[precision]))
+        assert drift <= dx * dx * N * r_norm[precision] + 1e-12 * before[2]
         out[precision] = np.array(after)
-    d = np.abs(out[1] - out[0]) / np.abs(out[0])
-    print("mixed vs FP64 refined: volume %.1e mass %.1e ie %.2e temperature %.2e" % tuple(d))
+    d = np.abs(out[1] - out[0])
+    print("mixed vs FP64 refined (relative): volume %.1e mass %.1e ie %.2e temperature %.2e"
+          % tuple(d / np.abs(out[0])))
     assert d[0] == 0 and d[1] == 0
-    assert d[2] <= 2.8e-13 and d[3] <= 1.1e-12
+    # both drifts bound the internal energies' difference; the eigenvalues of
+    # A are >= 1, so |u_a - u_b|_2 <= |r_a|_2 + |r_b|_2 and the temperatures
+    # sum(vol rho u) differ by at most vol |rho|_2 (|r_a|_2 + |r_b|_2)
+    r_sum = r_norm[0] + r_norm[1]
+    assert d[2] <= dx * dx * N * r_sum + 1e-12 * out[0][2]
+    assert d[3] <= dx * dx * rho_norm * r_sum + 1e-12 * out[0][3]
     # the refined solves and the reference PPCG solve one timestep: the
     # internal energies agree to what the reference's exit test leaves
     assert abs(out[0][2] - ie_ref) <= 1e-6 * ie_ref
