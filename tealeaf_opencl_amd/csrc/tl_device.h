@@ -471,18 +471,17 @@ __device__ __forceinline__ void strip_precondition(const Geom &g, int ks, int i0
 }
 
 // ---------------------------------------------------------------------------
-// Band march (the fused CG kernels of the device-resident driver). A
+// Band march (k_fused_cg_w, the fused Chebyshev kernels and the refined
+// solve's k_rf_w). A
 // wavefront owns a band of 64 lanes and walks the rows of one segment of M.H
 // rows downwards with the rows above and below the current one in
 // registers: one load per row and field. The west and east neighbours'
 // values come from lanes -1 / +1 by DPP (wave_shr:1 / wave_shl:1), and each
 // face coefficient is formed once, by the lane east of the face, and passed to
-// the lane west of it. Two band layouts: overlapping bands of MARCH_COLS
-// output columns (lanes 1..62 = columns jb .. jb+61; lanes 0 and 63 only
-// supply neighbours), or aligned bands of 64 output columns (lane l = column
-// jb+l, 512-byte aligned rows) whose edge lanes load the column just outside
-// the band themselves, for kernels whose several store streams want whole
-// cache lines. Segments are cut into eight slabs of consecutive rows; blocks
+// the lane west of it. Bands overlap: MARCH_COLS output columns (lanes
+// 1..62 = columns jb .. jb+61; lanes 0 and 63 only supply neighbours), or
+// MARCH2_COLS for the two-step kernels (lanes 2..61).
+// Segments are cut into eight slabs of consecutive rows; blocks
 // b with b % 8 == x (workgroups are dispatched round-robin to the eight
 // XCDs) walk slab x, four bands side by side per block, so the halo rows and
 // columns that neighbouring items share stay in one L2.
@@ -499,7 +498,7 @@ constexpr int MARCH2_COLS = 60;
 constexpr int MARCH2_ROWS = 32;
 
 struct March {
-    int cols;      // output columns per band: MARCH_COLS or 64
+    int cols;      // output columns per band (MARCH_COLS, or MARCH2_COLS for two steps)
     int nbands;    // bands across the interior
     int ngroups;   // blocks of four bands across the interior
     int H;         // rows per segment
@@ -544,15 +543,6 @@ __device__ __forceinline__ double dpp_shl1(double v, double old)
 #define TL_MARCH_LOOP_END \
     }                     \
     }
-
-// aligned bands: lane 0 loads column jb-1, lane 63 column jb+64 (exec-masked);
-// the other lanes get 0
-__device__ __forceinline__ double edge_ld(const double *f, int i, int lane)
-{
-    double h = 0.0;
-    if (lane == 0 || lane == 63) h = f[i + (lane == 0 ? -1 : 1)];
-    return h;
-}
 
 // Kx of the face west of column j in row k from the conductivities either
 // side (bitwise tea_leaf_init_common's value, zeroed as tea_leaf_zero_boundary

@@ -497,8 +497,7 @@ void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false, int parity = 0
                           red_args(c, RED_CG_PW, true), ev ? ev[1] : mid);
     });
     if (ev) hipEventRecord(ev[2], c->s);
-    const char *ur_name = parity == 1 ? "k_fused_cg_r_march" : parity == 2 ? "k_fused_cg_ru_march" : "k_fused_cg_ur";
-    launch_timed(c, prof, ur_name, true, [&](hipEvent_t mid) {
+    launch_timed(c, prof, "k_fused_cg_ur", true, [&](hipEvent_t mid) {
         launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
                            c->f2[F_R], c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces,
                            red_args(c, RED_CG_RRN, true), ev ? ev[3] : mid, parity);

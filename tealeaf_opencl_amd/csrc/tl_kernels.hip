@@ -954,16 +954,24 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, Mar
 // fused update per iteration gives; a CG phase that ends on an odd iteration
 // is completed by k_cg_u_fix. This saves u's read and write on every other
 // iteration (8 of 88 B/cell per iteration on average).
-template <int PREC, int PPCG>
+// PAR: 0 takes the parity of n from the state; 1 / 2 compile the odd / even
+// form only (the driver captures each for iterations of that parity; run on
+// the other parity it sets the state's fault flag instead, which stops every
+// later kernel and the solve).
+template <int PREC, int PPCG, int PAR = 0>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, double *u,
                                                                const double *pA, const double *pB,
                                                                const double *r, double *r_out, double *z,
                                                                double *sd, const double *W, Faces fc,
                                                                RedArgs ra)
 {
-    const SolveState *st = ra.st;
+    SolveState *st = ra.st;
     if (st->halt || st->fault) return;
-    const bool odd = (st->n & 1) != 0;
+    const bool odd = PAR ? PAR == 1 : (st->n & 1) != 0;
+    if (PAR && odd != ((st->n & 1) != 0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->fault = 1;
+        return;
+    }
     const double *p = odd ? pB : pA;
     const double *p_prev = odd ? pA : pB;
     const double alpha = st->alpha, alpha_prev = st->alpha_prev;
@@ -1005,187 +1013,6 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ur(Geom g, Ext e, do
     }
     TL_TILE_LOOP_END
     if (!PPCG) block_partial<1>(acc, ra.partials);
-}
-
-// k_fused_cg_ur<PREC, 0> on odd iterations (those that leave u alone) as a
-// band march (TL_MARCH_LOOP): r -= alpha A p, z = M^-1 r, r.z, with one load
-// per row of p and the conductivity, the neighbours by lane exchange and the
-// faces formed once. The driver captures it only for odd iterations; on an
-// even one it sets the state's fault flag instead, which stops every later
-// kernel and the solve. Algorithmic bytes: R p, r, cond, W r, z = 40 B/cell.
-template <int PREC>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_r_march(Geom g, Ext e, March m, const double *pB,
-                                                                    const double *r, double *r_out, double *z,
-                                                                    const double *W, Faces fc, RedArgs ra)
-{
-    SolveState *st = ra.st;
-    if (st->halt || st->fault) return;
-    if (!(st->n & 1)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->fault = 1;
-        return;
-    }
-    const double *p = pB;  // iteration n (odd) wrote p into buffer n & 1
-    const double alpha = st->alpha;
-    const int lane = threadIdx.x & 63;
-    const int P = g.pitch, hd = g.hd;
-    double acc = 0.0;
-    TL_MARCH_LOOP(m, e)
-    const int j = jb - 1 + lane;
-    const bool out_lane = lane >= 1 && lane <= MARCH_COLS && j <= e.j1;
-    const bool inner_x = jb - 1 > hd && jb + MARCH_COLS < hd + g.x_max;
-    // rows ks-1 .. ks+4 of p and the conductivity; the first two carried from
-    // the strip above
-    double p6[STRIP + 2], c6[STRIP + 2];
-    const int ib = IX(g, j, k_begin - 1);
-    p6[0] = p[ib];
-    p6[1] = p[ib + P];
-    c6[0] = W[ib];
-    c6[1] = W[ib + P];
-    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
-    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
-        const int i0 = IX(g, j, ks);
-        double rv[STRIP];
-        col4_nt(r, i0, P, rv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            p6[q + 2] = p[i0 + (q + 1) * P];
-            c6[q + 2] = W[i0 + (q + 1) * P];
-        }
-        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], wv[STRIP], zv[STRIP];
-        ky[0] = ky0;
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
-            kx1[q] = dpp_shl1(kx0[q], 0.0);
-            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
-            w4[q] = dpp_shr1(p6[q + 1], 0.0);
-            e4[q] = dpp_shl1(p6[q + 1], 0.0);
-        }
-        strip_apply(p6, w4, e4, kx0, kx1, ky, wv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) rv[q] -= alpha * wv[q];
-        mask_rows(ks, e.k1, rv);
-        if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1 || !out_lane) continue;
-            const int ii = i0 + q * P;
-            store_nt(r_out + ii, rv[q]);
-            double zz = rv[q];
-            if (PREC == PREC_BLOCK) zz = zv[q];
-            if (PREC == PREC_DIAG) zz = rv[q] * div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
-            if (PREC != PREC_NONE) store_nt(z + ii, zz);
-            acc += rv[q] * zz;
-        }
-        p6[0] = p6[4];
-        p6[1] = p6[5];
-        c6[0] = c6[4];
-        c6[1] = c6[5];
-        ky0 = ky[STRIP];
-    }
-    TL_MARCH_LOOP_END
-    double a1[1] = {acc};
-    block_partial<1>(a1, ra.partials);
-}
-
-// k_fused_cg_ur<PREC, 0> on even iterations as a band march on aligned bands
-// (three store streams: whole cache lines): r -= alpha A p, z = M^-1 r, r.z,
-// and u += alpha_{n-1} p_{n-1} + alpha_n p_n (the two iterations' updates of
-// u, each one FMA, in the order of per-iteration updates). Captured for even
-// iterations only; on an odd one it sets the fault flag. Algorithmic bytes:
-// R p, r, cond, u, p_{n-1}, W r, z, u = 64 B/cell.
-template <int PREC>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_ru_march(Geom g, Ext e, March m, double *u,
-                                                                     const double *pA, const double *pB,
-                                                                     const double *r, double *r_out, double *z,
-                                                                     const double *W, Faces fc, RedArgs ra)
-{
-    SolveState *st = ra.st;
-    if (st->halt || st->fault) return;
-    if (st->n & 1) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->fault = 1;
-        return;
-    }
-    const double *p = pA, *p_prev = pB;  // even n: p_n in buffer 0, p_{n-1} in buffer 1
-    const double alpha = st->alpha, alpha_prev = st->alpha_prev;
-    const int lane = threadIdx.x & 63;
-    const int P = g.pitch, hd = g.hd;
-    double acc = 0.0;
-    TL_MARCH_LOOP(m, e)
-    const int j = jb + lane;
-    const bool out_lane = j <= e.j1;
-    const bool inner_x = jb - 1 > hd && jb + 64 < hd + g.x_max;
-    // rows ks-1 .. ks+4 of p and the conductivity (and the conductivity of
-    // the edge lanes' outside columns); the first two carried from the strip above
-    double p6[STRIP + 2], c6[STRIP + 2], h6[STRIP + 2];
-    const int ib = IX(g, j, k_begin - 1);
-    p6[0] = p[ib];
-    p6[1] = p[ib + P];
-    c6[0] = W[ib];
-    c6[1] = W[ib + P];
-    h6[0] = edge_ld(W, ib, lane);
-    h6[1] = edge_ld(W, ib + P, lane);
-    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
-    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
-        const int i0 = IX(g, j, ks);
-        double uv[STRIP], rv[STRIP], pp[STRIP], ph[STRIP];
-        col4_nt(u, i0, P, uv);
-        col4_nt(r, i0, P, rv);
-        col4_nt(p_prev, i0, P, pp);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            p6[q + 2] = p[i0 + (q + 1) * P];
-            c6[q + 2] = W[i0 + (q + 1) * P];
-            h6[q + 2] = edge_ld(W, i0 + (q + 1) * P, lane);
-            ph[q] = edge_ld(p, i0 + q * P, lane);
-        }
-        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], wv[STRIP], zv[STRIP];
-        ky[0] = ky0;
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], h6[q + 1]), c6[q + 1]);
-            // the face east of the band, formed by lane 63 from its outside column
-            double ke = 0.0;
-            if (lane == 63) ke = march_kx(g, fc, inner, j + 1, ks + q, c6[q + 1], h6[q + 1]);
-            kx1[q] = dpp_shl1(kx0[q], ke);
-            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
-            w4[q] = dpp_shr1(p6[q + 1], ph[q]);
-            e4[q] = dpp_shl1(p6[q + 1], ph[q]);
-        }
-        strip_apply(p6, w4, e4, kx0, kx1, ky, wv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            uv[q] = fma(alpha_prev, pp[q], uv[q]);
-            uv[q] = fma(alpha, p6[q + 1], uv[q]);
-            rv[q] -= alpha * wv[q];
-        }
-        mask_rows(ks, e.k1, rv);
-        if (PREC == PREC_BLOCK) strip_block(g, ks, i0, kx0, kx1, ky, nullptr, nullptr, rv, zv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1 || !out_lane) continue;
-            const int ii = i0 + q * P;
-            store_nt(u + ii, uv[q]);
-            store_nt(r_out + ii, rv[q]);
-            double zz = rv[q];
-            if (PREC == PREC_BLOCK) zz = zv[q];
-            if (PREC == PREC_DIAG) zz = rv[q] * div_nr(1.0, diag_of(kx0[q], kx1[q], ky[q], ky[q + 1]));
-            if (PREC != PREC_NONE) store_nt(z + ii, zz);
-            acc += rv[q] * zz;
-        }
-        p6[0] = p6[4];
-        p6[1] = p6[5];
-        c6[0] = c6[4];
-        c6[1] = c6[5];
-        h6[0] = h6[4];
-        h6[1] = h6[5];
-        ky0 = ky[STRIP];
-    }
-    TL_MARCH_LOOP_END
-    double a1[1] = {acc};
-    block_partial<1>(a1, ra.partials);
 }
 
 // Completes u after a CG phase of the fused kernels that ended on an odd
@@ -1246,76 +1073,38 @@ __device__ __forceinline__ double ppcg_rz(double rv, double zo, double zf)
 
 // PPCG inner step: update_r then calc_sd (tea_leaf_ppcg_cl.cl:62-155) fused;
 // sd is double-buffered. LAST = 1, 2 also form r.z (ppcg_rz); the finisher
-// runs the tail. A band march with the geometry of the two-step kernel
-// (bands of MARCH2_COLS columns, segments of MARCH2_ROWS rows), so that a
-// step's r.z is summed in the same order whether it ends a one-step or a
-// two-step launch; faces from the conductivity (bitwise the stored Kx, Ky).
+// runs the tail.
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner(
-    Geom g, Ext e, March m, double *u, double *r, const double *sd_in, double *sd_out, const double *W, Faces fc,
+    Geom g, Ext e, double *u, double *r, const double *sd_in, double *sd_out, const double *W, Faces fc,
     double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
     const double ca = ch_a[step], cb = ch_b[step];
-    const int lane = threadIdx.x & 63;
-    const int P = g.pitch, hd = g.hd;
-    double acc = 0.0;
-    TL_MARCH_LOOP(m, e)
-    const int j = jb - 2 + lane;
-    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
-    const bool inner_x = jb - 1 > hd && jb + MARCH2_COLS + 1 < hd + g.x_max;
-    double s6[STRIP + 2], c6[STRIP + 2];
-    const int ib = IX(g, j, k_begin - 1);
-    s6[0] = sd_in[ib];
-    s6[1] = sd_in[ib + P];
-    c6[0] = W[ib];
-    c6[1] = W[ib + P];
-    double ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
-    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
-        const int i0 = IX(g, j, ks);
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], uv[STRIP],
-            rv[STRIP], sdn[STRIP], zo[STRIP] = {}, zf[STRIP];
+    double acc[1] = {0.0};
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], uv[STRIP], rv[STRIP], sdn[STRIP],
+        zo[STRIP] = {}, zf[STRIP];
+    strip_k_cond(g, fc, W, i0, P, j, ks, kx0, kx1, ky);
+    col4_nt(u, i0, P, uv);
+    col4_nt(r, i0, P, rv);
+    if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
+    strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+    ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn, zf);
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            s6[q + 2] = sd_in[i0 + (q + 1) * P];
-            c6[q + 2] = W[i0 + (q + 1) * P];
-        }
-        col4_nt(u, i0, P, uv);
-        col4_nt(r, i0, P, rv);
-        if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
-        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
-        ky[0] = ky0;
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
-            kx1[q] = dpp_shl1(kx0[q], 0.0);
-            ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
-            w4[q] = dpp_shr1(s6[q + 1], 0.0);
-            e4[q] = dpp_shl1(s6[q + 1], 0.0);
-            sc[q] = s6[q + 1];
-        }
-        strip_apply(s6, w4, e4, kx0, kx1, ky, res);
-        ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca, cb, rv, sdn, zf);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1 || !out_lane) continue;
-            const int i = i0 + q * P;
-            store_nt(u + i, uv[q] + sc[q]);
-            store_nt(r + i, rv[q]);
-            store_nt(sd_out + i, sdn[q]);
-            if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
-            if (LAST) acc += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
-        }
-        s6[0] = s6[STRIP];
-        s6[1] = s6[STRIP + 1];
-        c6[0] = c6[STRIP];
-        c6[1] = c6[STRIP + 1];
-        ky0 = ky[STRIP];
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        store_nt(u + i, uv[q] + sc[q]);
+        store_nt(r + i, rv[q]);
+        store_nt(sd_out + i, sdn[q]);
+        if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
+        if (LAST) acc[0] += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
     }
-    TL_MARCH_LOOP_END
-    double a1[1] = {acc};
-    if (LAST) block_partial<1>(a1, ra.partials);
+    TL_TILE_LOOP_END
+    if (LAST) block_partial<1>(acc, ra.partials);
 }
 
 // sd at ring cell (j, k) when it is not computed in the block: the value the
@@ -1329,131 +1118,126 @@ __device__ __forceinline__ double ring_outside(const Geom &g, const Ext &e, cons
 }
 
 // Two PPCG inner steps (steps `step` and `step`+1 of the smoother) in one
-// pass over the fields, as a band march with a one-strip lag: a wavefront
-// owns a band of 64 lanes (columns jb-2 .. jb+61; lanes 2..61 are its outputs,
-// since step 2 needs step-1 values one lane beyond step 1's own neighbours)
-// and walks 4-row strips down a segment: step 1 on strip S (u, r, sd of S,
-// sd of the rows either side, the stored faces), then step 2 on strip S-1
-// from step 1's sd of the rows S-1 spans plus one row either side, kept in
-// registers with S-1's r and u (its faces are read again). One strip above and one below the
-// segment get step 1 only (their sd rows next to the segment). Every value
-// is formed by the same expressions as the one-step kernel, and a cell
-// outside the extent takes what the one-step kernel would read there (the
-// halo of the output buffer, ring_outside), so two one-step launches and one
-// two-step launch give bitwise the same u, r and sd. r and sd are read from
-// one buffer and written to the other (the strips around a segment are read
-// by its neighbours' step 1). The face coefficients are read from Kx, Ky
-// (bitwise what the one-step kernels recompute from the conductivity).
-// Algorithmic bytes per two steps: R u, r, sd, Kx, Ky  W u, r, sd
-// (64 B/cell) instead of 112.
+// pass over the fields. Step 1 is computed on the block's 64x16 tile and on
+// the ring around it that step 2 reads: the 4-row strips directly above and
+// below the tile (their block-Jacobi solve needs the whole strip) and the
+// tile's strips in the columns left and right of it. The ring's new sd goes
+// through LDS; the tile keeps r, u and the faces in registers. Every value is
+// formed by the same expressions as the one-step kernel, so two one-step
+// launches and one two-step launch give bitwise the same u, r and sd.
+// The face coefficients are read from Kx, Ky (bitwise what the one-step
+// kernels recompute from the conductivity): with the ring's extra strips this
+// kernel is bound by arithmetic as much as by memory, and reading the faces
+// instead of dividing measured 5 % faster per PPCG outer step at 4096^2.
+// Algorithmic bytes per two steps: R u, r, sd, Kx, Ky  W u, r, sd (64 B/cell)
+// instead of 112. Neighbouring blocks read r and sd of this block's tile in
+// their step 1, so r and sd are both read from one buffer and written to
+// another.
 template <int PREC, int LAST>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_ppcg_inner2(
-    Geom g, Ext e, March m, double *u, const double *r_in, double *r_out, const double *sd_in, double *sd_out,
+    Geom g, Ext e, double *u, const double *r_in, double *r_out, const double *sd_in, double *sd_out,
     const double *Kx, const double *Ky, double *z, const double *ch_a, const double *ch_b, int step, RedArgs ra)
 {
     const SolveState *st = ra.st;
     if (st->halt) return;
+    constexpr int TH = STRIPS_PER_TILE * STRIP;  // tile rows
+    constexpr int RW = TILE_X + 2, RH = TH + 2;   // ring-padded tile
+    __shared__ double ring[RH][RW];
     const double ca1 = ch_a[step], cb1 = ch_b[step], ca2 = ch_a[step + 1], cb2 = ch_b[step + 1];
-    const int lane = threadIdx.x & 63;
     const int P = g.pitch;
-    double acc = 0.0;
-    TL_MARCH_LOOP(m, e)
-    const int j = jb - 2 + lane;
-    const bool col_in = j >= e.j0 && j <= e.j1;
-    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
-    const int ks_last = k_begin + (k_end - k_begin) / STRIP * STRIP;  // the segment's last strip
-    // step-1 input window: sd rows ks-1 .. ks+4 (the first two carried) and
-    // the face between rows ks-1 and ks
-    double c6[STRIP + 2];
-    int ks = k_begin - STRIP;
-    {
-        const int i = IX(g, j, ks - 1);
-        c6[0] = sd_in[i];
-        c6[1] = sd_in[i + P];
-    }
-    double ky0 = Ky[IX(g, j, ks)];
-    // step 1's results for strip S-1 (row above it in s1p[0])
-    double s1p[STRIP + 1] = {}, r1p[STRIP] = {}, u1p[STRIP] = {};
-    for (; ks <= ks_last + STRIP; ks += STRIP) {
+    const int lane = threadIdx.x & (TILE_X - 1), wv = threadIdx.x / TILE_X;
+    const int nt = e.ntx * e.nty;
+    double acc[1] = {0.0};
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
+        const int j = jt + lane, ks = kt + wv * STRIP;
+        const bool act = j <= e.j1 && ks <= e.k1;
         const int i0 = IX(g, j, ks);
-        const bool own = ks >= k_begin && ks <= ks_last;  // a strip of this segment (u and step 2)
-        // ---- step 1 on strip ks
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], uv[STRIP] = {}, w4[STRIP], e4[STRIP],
-            res[STRIP], sc[STRIP], sdn[STRIP], zf[STRIP], s1[STRIP];
-        ky[0] = ky0;
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            c6[q + 2] = sd_in[i0 + (q + 1) * P];
-            kx0[q] = Kx[i0 + q * P];
-            ky[q + 1] = Ky[i0 + (q + 1) * P];
-        }
-        col4(r_in, i0, P, rv);
-        if (own) col4_nt(u, i0, P, uv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx1[q] = dpp_shl1(kx0[q], 0.0);
-            w4[q] = dpp_shr1(c6[q + 1], 0.0);
-            e4[q] = dpp_shl1(c6[q + 1], 0.0);
-            sc[q] = c6[q + 1];
-        }
-        strip_apply(c6, w4, e4, kx0, kx1, ky, res);
-        ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn, zf);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            const int k = ks + q;
-            s1[q] = (col_in && k >= e.k0 && k <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, j, k);
-            uv[q] += sc[q];
-        }
-        // ---- step 2 on strip ks-4, from step 1's sd of rows ks-5 .. ks
-        if (ks - STRIP >= k_begin) {
-            const int ks2 = ks - STRIP, i2 = IX(g, j, ks2);
-            double w6[STRIP + 2], w4b[STRIP], e4b[STRIP], res2[STRIP], sc2[STRIP], sd2[STRIP], zf2[STRIP],
-                zo[STRIP] = {}, kx0p[STRIP], kx1p[STRIP], kyp[STRIP + 1];
-            // the strip's faces again (read by step 1 one trip ago: cache hits)
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) kx0p[q] = Kx[i2 + q * P];
-#pragma unroll
-            for (int q = 0; q < STRIP + 1; ++q) kyp[q] = Ky[i2 + q * P];
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) kx1p[q] = dpp_shl1(kx0p[q], 0.0);
-#pragma unroll
-            for (int q = 0; q < STRIP + 1; ++q) w6[q] = s1p[q];
-            w6[STRIP + 1] = s1[0];
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                w4b[q] = dpp_shr1(w6[q + 1], 0.0);
-                e4b[q] = dpp_shl1(w6[q + 1], 0.0);
-                sc2[q] = w6[q + 1];
+        // step 1 on the ring first, so that its registers are free again
+        // before the own strip's state (kept for step 2) goes live: strip
+        // above (wave 0), strip below (wave 1), the tile's strips in the
+        // columns either side (wave 2, lanes 0-7)
+        {
+            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
+            if (wv == 0) {
+                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
+            } else if (wv == 1) {
+                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
+            } else if (wv == 2 && lane < 2 * STRIPS_PER_TILE) {
+                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
+                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
+                row0 = 1 + qs * STRIP;
             }
-            if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i2, P, zo);
-            strip_apply(w6, w4b, e4b, kx0p, kx1p, kyp, res2);
-            ppcg_step<PREC>(g, ks2, e.k1, kx0p, kx1p, kyp, res2, sc2, ca2, cb2, r1p, sd2, zf2);
+            if (hj >= 0) {
+                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
+                double sdn[STRIP];
+                if (inside) {
+                    const int h0 = IX(g, hj, hk);
+                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP], zf[STRIP];
+                    strip_k(Kx, Ky, h0, P, hx0, hx1, hy);
+                    col4(r_in, h0, P, hr);
+                    strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
+                    ppcg_step<PREC>(g, hk, e.k1, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn, zf);
+                }
+                if (take < STRIP) {
+                    // one row of a strip above / below the tile
+                    const int k = hk + take;
+                    if (hj <= e.j1 + 1) ring[row0][col] = (inside && k <= e.k1) ? sdn[take] : ring_outside(g, e, sd_out, hj, k);
+                } else {
 #pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                if (ks2 + q > e.k1 || !out_lane) continue;
-                const int i = i2 + q * P;
-                store_nt(u + i, u1p[q] + sc2[q]);
-                store_nt(r_out + i, r1p[q]);
-                store_nt(sd_out + i, sd2[q]);
-                if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf2[q]);
-                if (LAST) acc += ppcg_rz<PREC, LAST>(r1p[q], zo[q], zf2[q]);
+                    for (int q = 0; q < STRIP; ++q)
+                        ring[row0 + q][col] = (inside && hk + q <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, hj, hk + q);
+                }
             }
         }
-        // ---- carry strip ks to the next trip
-        s1p[0] = s1p[STRIP];
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], uv[STRIP];
+        // step 1 on the own strip
+        if (act) {
+            double res[STRIP], sc[STRIP], sdn[STRIP], zf[STRIP];
+            strip_k(Kx, Ky, i0, P, kx0, kx1, ky);
+            col4_nt(u, i0, P, uv);
+            col4_nt(r_in, i0, P, rv);
+            strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn, zf);
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            s1p[q + 1] = s1[q];
-            r1p[q] = rv[q];
-            u1p[q] = uv[q];
+            for (int q = 0; q < STRIP; ++q) {
+                uv[q] += sc[q];
+                ring[1 + wv * STRIP + q][1 + lane] = (ks + q <= e.k1) ? sdn[q] : ring_outside(g, e, sd_out, j, ks + q);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) ring[1 + wv * STRIP + q][1 + lane] = ring_outside(g, e, sd_out, j, ks + q);
         }
-        c6[0] = c6[STRIP];
-        c6[1] = c6[STRIP + 1];
-        ky0 = ky[STRIP];
+        __syncthreads();
+        // step 2 on the own strip, sd from the ring
+        if (act) {
+            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP] = {},
+                zf[STRIP];
+#pragma unroll
+            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv * STRIP + q][1 + lane];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                w4[q] = ring[1 + wv * STRIP + q][lane];
+                e4[q] = ring[1 + wv * STRIP + q][lane + 2];
+                sc[q] = c6[q + 1];
+            }
+            if (LAST == 1 && PREC != PREC_NONE) col4_nt(z, i0, P, zo);
+            strip_apply(c6, w4, e4, kx0, kx1, ky, res);
+            ppcg_step<PREC>(g, ks, e.k1, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn, zf);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks + q > e.k1) continue;
+                const int i = i0 + q * P;
+                store_nt(u + i, uv[q] + sc[q]);
+                store_nt(r_out + i, rv[q]);
+                store_nt(sd_out + i, sdn[q]);
+                if (LAST == 2 && PREC != PREC_NONE) store_nt(z + i, zf[q]);
+                if (LAST) acc[0] += ppcg_rz<PREC, LAST>(rv[q], zo[q], zf[q]);
+            }
+        }
+        __syncthreads();
     }
-    TL_MARCH_LOOP_END
-    double a1[1] = {acc};
-    if (LAST) block_partial<1>(a1, ra.partials);
+    if (LAST) block_partial<1>(acc, ra.partials);
 }
 
 // One Chebyshev iteration on a strip once A u is known (wv = A u, uc = u):
@@ -2008,32 +1792,19 @@ void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int
                         double *sd, const double *W, const Faces &fc, const RedArgs &ra, hipEvent_t ev_main,
                         int parity)
 {
-    if (!ppcg && parity == 1) {
-        const March m = make_march(e, MARCH_COLS);
-        const int grid = march_grid(m, cap);
-#define TL_FRM(P)                                                                                                \
-    hipLaunchKernelGGL(k_fused_cg_r_march<P>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, pB, r, r_out, z, W, fc, \
-                       ra)
-        if (prec == PREC_BLOCK) TL_FRM(PREC_BLOCK);
-        else if (prec == PREC_DIAG) TL_FRM(PREC_DIAG);
-        else TL_FRM(PREC_NONE);
-#undef TL_FRM
-        if (ev_main) hipEventRecord(ev_main, s);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
-        return;
-    }
-    if (!ppcg && parity == 2) {
-        const March m = make_march(e, 64);
-        const int grid = march_grid(m, cap);
-#define TL_FUM(P)                                                                                                   \
-    hipLaunchKernelGGL(k_fused_cg_ru_march<P>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u, pA, pB, r, r_out, z, \
-                       W, fc, ra)
-        if (prec == PREC_BLOCK) TL_FUM(PREC_BLOCK);
-        else if (prec == PREC_DIAG) TL_FUM(PREC_DIAG);
-        else TL_FUM(PREC_NONE);
-#undef TL_FUM
-        if (ev_main) hipEventRecord(ev_main, s);
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
+    if (!ppcg && (parity == 1 || parity == 2)) {
+#define TL_FUP(P, PAR) TL_LAUNCH_RED_EV((k_fused_cg_ur<P, 0, PAR>), e, cap, s, 1, ra, ev_main, g, e, u, pA, pB, r, \
+                                        r_out, z, sd, W, fc, ra)
+        if (parity == 1) {
+            if (prec == PREC_BLOCK) TL_FUP(PREC_BLOCK, 1);
+            else if (prec == PREC_DIAG) TL_FUP(PREC_DIAG, 1);
+            else TL_FUP(PREC_NONE, 1);
+        } else {
+            if (prec == PREC_BLOCK) TL_FUP(PREC_BLOCK, 2);
+            else if (prec == PREC_DIAG) TL_FUP(PREC_DIAG, 2);
+            else TL_FUP(PREC_NONE, 2);
+        }
+#undef TL_FUP
         return;
     }
 #define TL_FUR(P, M)                                                                                           \
@@ -2059,17 +1830,12 @@ void launch_fused_ppcg_inner(hipStream_t s, const Geom &g, const Ext &e, int cap
                              const double *W, const Faces &fc, double *z, const double *ch_a,
                              const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main)
 {
-    March m = make_march(e, MARCH2_COLS);
-    set_march_rows(m, e, MARCH2_ROWS);
-    const int grid = march_grid(m, cap);
-#define TL_FPI(P, L)                                                                                             \
-    do {                                                                                                         \
-        hipLaunchKernelGGL((k_fused_ppcg_inner<P, L>), dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u, r,       \
-                           sd_in, sd_out, W, fc, z, ch_a, ch_b, step, ra);                                       \
-        if (L) {                                                                                                 \
-            if (ev_main) hipEventRecord(ev_main, s);                                                             \
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);                      \
-        }                                                                                                        \
+#define TL_FPI(P, L)                                                                                         \
+    do {                                                                                                     \
+        if (L) TL_LAUNCH_RED_EV((k_fused_ppcg_inner<P, L>), e, cap, s, 1, ra, ev_main, g, e, u, r, sd_in, sd_out, W, fc, z, \
+                             ch_a, ch_b, step, ra);                                                          \
+        else TL_LAUNCH((k_fused_ppcg_inner<P, L>), e, cap, s, g, e, u, r, sd_in, sd_out, W, fc, z, ch_a, ch_b,  \
+                       step, ra);                                                                            \
     } while (0)
     if (last == 2) {
         if (prec == PREC_BLOCK) TL_FPI(PREC_BLOCK, 2);
@@ -2092,17 +1858,12 @@ void launch_fused_ppcg_inner2(hipStream_t s, const Geom &g, const Ext &e, int ca
                               const double *Kx, const double *Ky, double *z, const double *ch_a,
                               const double *ch_b, int step, const RedArgs &ra, hipEvent_t ev_main)
 {
-    March m = make_march(e, MARCH2_COLS);
-    set_march_rows(m, e, MARCH2_ROWS);
-    const int grid = march_grid(m, cap);
 #define TL_FPI2(P, L)                                                                                           \
     do {                                                                                                        \
-        hipLaunchKernelGGL((k_fused_ppcg_inner2<P, L>), dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, u, r_in,  \
-                           r_out, sd_in, sd_out, Kx, Ky, z, ch_a, ch_b, step, ra);                               \
-        if (L) {                                                                                                \
-            if (ev_main) hipEventRecord(ev_main, s);                                                            \
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);                     \
-        }                                                                                                       \
+        if (L) TL_LAUNCH_RED_EV((k_fused_ppcg_inner2<P, L>), e, cap, s, 1, ra, ev_main, g, e, u, r_in, r_out, sd_in, sd_out, \
+                             Kx, Ky, z, ch_a, ch_b, step, ra);                                                  \
+        else TL_LAUNCH((k_fused_ppcg_inner2<P, L>), e, cap, s, g, e, u, r_in, r_out, sd_in, sd_out, Kx, Ky, z,   \
+                       ch_a, ch_b, step, ra);                                                                   \
     } while (0)
     if (last == 2) {
         if (prec == PREC_BLOCK) TL_FPI2(PREC_BLOCK, 2);

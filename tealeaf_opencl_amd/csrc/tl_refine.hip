@@ -358,79 +358,42 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
 
 // d += alpha p, r -= alpha A p (A p recomputed from the stored p),
 // z = M^-1 r with the preconditioner recomputed from the stored
-// coefficients; r.z and r.r of the stored values. A band march over 4-row
-// strips (the block-Jacobi strips), as k_fused_cg_r_march.
+// coefficients; r.z and r.r of the stored values.
 template <typename T, int PREC, int FACES>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, March m, T *d, const T *pA, const T *pB,
-                                                         T *r, T *z, const T *W, const T *Wy, Faces fc, RfArgs ra)
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
+                                                         T *z, const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
     const T *p = (st->n & 1) ? pB : pA;
     const double alpha = st->alpha;
-    constexpr bool K32 = FACES == RF_FACES_K32;
-    const int lane = threadIdx.x & 63;
-    const int P = g.pitch, hd = g.hd;
     double acc[2] = {0.0, 0.0};
-    TL_MARCH_LOOP(m, e)
-    const int j = jb - 1 + lane;
-    const bool out_lane = lane >= 1 && lane <= MARCH_COLS && j <= e.j1;
-    const bool inner_x = jb - 1 > hd && jb + MARCH_COLS < hd + g.x_max;
-    double p6[STRIP + 2], c6[STRIP + 2];
-    const int ib = IX(g, j, k_begin - 1);
-    p6[0] = (double)p[ib];
-    p6[1] = (double)p[ib + P];
-    c6[0] = (double)W[ib];
-    c6[1] = (double)W[ib + P];
-    double ky0 = K32 ? (double)Wy[ib + P] : rf_march_ky<FACES>(g, fc, false, j, k_begin, c6[0], c6[1]);
-    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
-        const int i0 = IX(g, j, ks);
-        double dv[STRIP], rv[STRIP], kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], wv[STRIP],
-            zv[STRIP];
-        col4_nt(d, i0, P, dv);
-        col4_nt(r, i0, P, rv);
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
+    rf_strip_faces<T, FACES>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
+    col4_nt(d, i0, P, dv);
+    col4_nt(r, i0, P, rv);
+    strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            p6[q + 2] = (double)p[i0 + (q + 1) * P];
-            c6[q + 2] = (double)W[i0 + (q + 1) * P];
-        }
-        const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
-        ky[0] = ky0;
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx0[q] = K32 ? c6[q + 1] : rf_march_kx<FACES>(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
-            kx1[q] = dpp_shl1(kx0[q], 0.0);
-            ky[q + 1] = K32 ? (double)Wy[i0 + (q + 1) * P]
-                            : rf_march_ky<FACES>(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
-            w4[q] = dpp_shr1(p6[q + 1], 0.0);
-            e4[q] = dpp_shl1(p6[q + 1], 0.0);
-        }
-        strip_apply(p6, w4, e4, kx0, kx1, ky, wv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            dv[q] += alpha * p6[q + 1];
-            rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
-        }
-        mask_rows(ks, e.k1, rv);
-        strip_precondition<PREC, inner_fast_rcp<T>()>(g, ks, i0, kx0, kx1, ky, rv, zv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1 || !out_lane) continue;
-            const int i = i0 + q * P;
-            const T zt = (T)zv[q];
-            store_nt(d + i, dv[q]);
-            store_nt(r + i, rv[q]);
-            if (PREC != PREC_NONE) store_nt(z + i, (double)zt);
-            acc[0] += rv[q] * (double)zt;
-            acc[1] += rv[q] * rv[q];
-        }
-        p6[0] = p6[STRIP];
-        p6[1] = p6[STRIP + 1];
-        c6[0] = c6[STRIP];
-        c6[1] = c6[STRIP + 1];
-        ky0 = ky[STRIP];
+    for (int q = 0; q < STRIP; ++q) {
+        dv[q] += alpha * pc[q];
+        rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
     }
-    TL_MARCH_LOOP_END
+    mask_rows(ks, e.k1, rv);
+    strip_precondition<PREC, inner_fast_rcp<T>()>(g, ks, i0, kx0, kx1, ky, rv, zv);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        const T zt = (T)zv[q];
+        store_nt(d + i, dv[q]);
+        store_nt(r + i, rv[q]);
+        if (PREC != PREC_NONE) store_nt(z + i, (double)zt);
+        acc[0] += rv[q] * (double)zt;
+        acc[1] += rv[q] * rv[q];
+    }
+    TL_TILE_LOOP_END
     block_partial<2>(acc, ra.partials);
 }
 
@@ -494,93 +457,40 @@ __device__ __forceinline__ void rf_ppcg_step(const Geom &g, int ks, int k1, int 
 
 // PPCG inner step (tea_leaf_ppcg_cl.cl:62-155, fused update_r + calc_sd):
 // d += sd, r -= A sd, sd' = a_i sd + b_i M^-1 r; sd double-buffered. LAST
-// also sums r.z with the z of the outer step and r.r. A band march with the
-// two-step kernel's geometry, so that the sums come out in the same order
-// whichever kernel forms them; faces from the conductivity (FP64) or read
-// from the FP32 copies of Kx, Ky.
+// also sums r.z with the z of the outer step and r.r.
 template <typename T, int PREC, int LAST>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, March m, T *d, T *r, const T *sd_in,
-                                                            T *sd_out, const T *W, const T *Wy, Faces fc, const T *z,
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner(Geom g, Ext e, T *d, T *r, const T *sd_in, T *sd_out,
+                                                            const T *W, const T *Wy, Faces fc, const T *z,
                                                             const double *ch_a, const double *ch_b, int step,
                                                             RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
     const double ca = ch_a[step], cb = ch_b[step];
-    constexpr bool K32 = ppcg_faces<T>() == RF_FACES_K32;
-    const int lane = threadIdx.x & 63;
-    const int P = g.pitch, hd = g.hd;
     double acc[2] = {0.0, 0.0};
-    TL_MARCH_LOOP(m, e)
-    const int j = jb - 2 + lane;
-    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
-    const bool inner_x = jb - 1 > hd && jb + MARCH2_COLS + 1 < hd + g.x_max;
-    double s6[STRIP + 2], c6[STRIP + 2] = {};
-    const int ib = IX(g, j, k_begin - 1);
-    s6[0] = (double)sd_in[ib];
-    s6[1] = (double)sd_in[ib + P];
-    double ky0;
-    if (K32) {
-        ky0 = (double)Wy[ib + P];
-    } else {
-        c6[0] = (double)W[ib];
-        c6[1] = (double)W[ib + P];
-        ky0 = ky_at(g, fc, j, k_begin, c6[0], c6[1]);
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], res[STRIP], sc[STRIP], dv[STRIP], rv[STRIP], sdn[STRIP],
+        zo[STRIP];
+    rf_strip_faces<T, ppcg_faces<T>()>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
+    col4_nt(d, i0, P, dv);
+    col4_nt(r, i0, P, rv);
+    if (LAST) col4_nt(z, i0, P, zo);
+    strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+    rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca, cb, rv, sdn);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        const int i = i0 + q * P;
+        store_nt(d + i, dv[q] + sc[q]);
+        store_nt(r + i, rv[q]);
+        store_nt(sd_out + i, sdn[q]);
+        if (LAST) {
+            acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+            acc[1] += rv[q] * rv[q];
+        }
     }
-    for (int ks = k_begin; ks <= k_end; ks += STRIP) {
-        const int i0 = IX(g, j, ks);
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], dv[STRIP],
-            rv[STRIP], sdn[STRIP], zo[STRIP] = {};
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) s6[q + 2] = (double)sd_in[i0 + (q + 1) * P];
-        col4_nt(d, i0, P, dv);
-        col4_nt(r, i0, P, rv);
-        if (LAST) col4_nt(z, i0, P, zo);
-        ky[0] = ky0;
-        if (K32) {
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                kx0[q] = (double)W[i0 + q * P];
-                ky[q + 1] = (double)Wy[i0 + (q + 1) * P];
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) c6[q + 2] = (double)W[i0 + (q + 1) * P];
-            const bool inner = inner_x && ks > hd && ks + STRIP < hd + g.y_max;
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                kx0[q] = march_kx(g, fc, inner, j, ks + q, dpp_shr1(c6[q + 1], 0.0), c6[q + 1]);
-                ky[q + 1] = march_ky(g, fc, inner, j, ks + q + 1, c6[q + 1], c6[q + 2]);
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx1[q] = dpp_shl1(kx0[q], 0.0);
-            w4[q] = dpp_shr1(s6[q + 1], 0.0);
-            e4[q] = dpp_shl1(s6[q + 1], 0.0);
-            sc[q] = s6[q + 1];
-        }
-        strip_apply(s6, w4, e4, kx0, kx1, ky, res);
-        rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca, cb, rv, sdn);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            if (ks + q > e.k1 || !out_lane) continue;
-            const int i = i0 + q * P;
-            store_nt(d + i, dv[q] + sc[q]);
-            store_nt(r + i, rv[q]);
-            store_nt(sd_out + i, sdn[q]);
-            if (LAST) {
-                acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
-                acc[1] += rv[q] * rv[q];
-            }
-        }
-        s6[0] = s6[STRIP];
-        s6[1] = s6[STRIP + 1];
-        c6[0] = c6[STRIP];
-        c6[1] = c6[STRIP + 1];
-        ky0 = ky[STRIP];
-    }
-    TL_MARCH_LOOP_END
+    TL_TILE_LOOP_END
     if (LAST) block_partial<2>(acc, ra.partials);
 }
 
@@ -606,23 +516,15 @@ __device__ __forceinline__ double rf_ring_outside(const Geom &g, const Ext &e, c
 }
 
 // Two PPCG inner steps per launch, as k_fused_ppcg_inner2 does for the FP64
-// driver: a band march with a one-strip lag (step 1 on strip S, step 2 on
-// strip S-1 from step 1's sd of the rows it spans plus one row either side,
-// kept in registers with S-1's r and d; one strip above and below each
-// segment get step 1 only). Values are rounded to the storage exactly where
-// the one-step kernel stores and reloads them, and cells outside the extent
-// take what the one-step kernel reads there, so two one-step launches and
-// one two-step launch give bitwise the same d, r, sd. r and sd are read from
-// one buffer and written to another (the strips around a segment are read
-// by its neighbours' step 1).
-template <typename T>
-__device__ __forceinline__ double rf_face(const T *W, const double *K, int i)
-{
-    return sizeof(T) == 8 ? K[i] : (double)W[i];
-}
-
+// driver: step 1 on the 64x16 tile and on the ring step 2 reads (the 4-row
+// strips above and below, the tile's strips in the columns either side), the
+// ring's sd through LDS, step 2 on the tile. Values are rounded to the
+// storage exactly where the one-step kernel stores and reloads them, so two
+// one-step launches and one two-step launch give bitwise the same d, r, sd.
+// r and sd are read from one buffer and written to another (neighbouring
+// blocks read this tile's r and sd in their step 1).
 template <typename T, int PREC, int LAST>
-__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, March m, T *d, const T *r_in, T *r_out,
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, T *d, const T *r_in, T *r_out,
                                                              const T *sd_in, T *sd_out, const T *W, const T *Wy,
                                                              const double *Kx, const double *Ky, const T *z,
                                                              const double *ch_a, const double *ch_b, int step,
@@ -630,101 +532,105 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, Marc
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
+    constexpr int TH = STRIPS_PER_TILE * STRIP;
+    constexpr int RW = TILE_X + 2, RH = TH + 2;
+    __shared__ double ring[RH][RW];
     const double ca1 = ch_a[step], cb1 = ch_b[step], ca2 = ch_a[step + 1], cb2 = ch_b[step + 1];
-    const int lane = threadIdx.x & 63;
     const int P = g.pitch;
+    const int lane = threadIdx.x & (TILE_X - 1), wv = threadIdx.x / TILE_X;
+    const int nt = e.ntx * e.nty;
     double acc[2] = {0.0, 0.0};
-    TL_MARCH_LOOP(m, e)
-    const int j = jb - 2 + lane;
-    const bool col_in = j >= e.j0 && j <= e.j1;
-    const bool out_lane = lane >= 2 && lane <= 61 && j <= e.j1;
-    const int ks_last = k_begin + (k_end - k_begin) / STRIP * STRIP;
-    double c6[STRIP + 2];
-    int ks = k_begin - STRIP;
-    {
-        const int i = IX(g, j, ks - 1);
-        c6[0] = (double)sd_in[i];
-        c6[1] = (double)sd_in[i + P];
-    }
-    double ky0 = rf_face<T>(Wy, Ky, IX(g, j, ks));
-    double s1p[STRIP + 1] = {}, r1p[STRIP] = {}, d1p[STRIP] = {};
-    for (; ks <= ks_last + STRIP; ks += STRIP) {
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int jt = e.j0 + (t % e.ntx) * TILE_X, kt = e.k0 + (t / e.ntx) * TH;
+        const int j = jt + lane, ks = kt + wv * STRIP;
+        const bool act = j <= e.j1 && ks <= e.k1;
         const int i0 = IX(g, j, ks);
-        const bool own = ks >= k_begin && ks <= ks_last;
-        // ---- step 1 on strip ks
-        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], dv[STRIP] = {}, w4[STRIP], e4[STRIP],
-            res[STRIP], sc[STRIP], sdn[STRIP], s1[STRIP];
-        ky[0] = ky0;
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            c6[q + 2] = (double)sd_in[i0 + (q + 1) * P];
-            kx0[q] = rf_face<T>(W, Kx, i0 + q * P);
-            ky[q + 1] = rf_face<T>(Wy, Ky, i0 + (q + 1) * P);
-        }
-        col4(r_in, i0, P, rv);
-        if (own) col4_nt(d, i0, P, dv);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            kx1[q] = dpp_shl1(kx0[q], 0.0);
-            w4[q] = dpp_shr1(c6[q + 1], 0.0);
-            e4[q] = dpp_shl1(c6[q + 1], 0.0);
-            sc[q] = c6[q + 1];
-        }
-        strip_apply(c6, w4, e4, kx0, kx1, ky, res);
-        rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn);
-#pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            const int k = ks + q;
-            s1[q] = (col_in && k >= e.k0 && k <= e.k1) ? sdn[q] : rf_ring_outside(g, e, sd_out, j, k);
-            dv[q] = rnd<T>(dv[q] + sc[q]);
-        }
-        // ---- step 2 on strip ks-4
-        if (ks - STRIP >= k_begin) {
-            const int ks2 = ks - STRIP, i2 = IX(g, j, ks2);
-            double w6[STRIP + 2], w4b[STRIP], e4b[STRIP], res2[STRIP], sc2[STRIP], sd2[STRIP], zo[STRIP] = {},
-                kx0p[STRIP], kx1p[STRIP], kyp[STRIP + 1];
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) kx0p[q] = rf_face<T>(W, Kx, i2 + q * P);
-#pragma unroll
-            for (int q = 0; q < STRIP + 1; ++q) kyp[q] = rf_face<T>(Wy, Ky, i2 + q * P);
-#pragma unroll
-            for (int q = 0; q < STRIP + 1; ++q) w6[q] = s1p[q];
-            w6[STRIP + 1] = s1[0];
-#pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                kx1p[q] = dpp_shl1(kx0p[q], 0.0);
-                w4b[q] = dpp_shr1(w6[q + 1], 0.0);
-                e4b[q] = dpp_shl1(w6[q + 1], 0.0);
-                sc2[q] = w6[q + 1];
+        // step 1 on the ring: strip above (wave 0), strip below (wave 1), the
+        // tile's strips in the columns either side (wave 2, lanes 0-7)
+        {
+            int hj = -1, hk = 0, take = -1, col = 0, row0 = 0;
+            if (wv == 0) {
+                hj = j; hk = kt - STRIP; take = STRIP - 1; col = 1 + lane; row0 = 0;
+            } else if (wv == 1) {
+                hj = j; hk = kt + TH; take = 0; col = 1 + lane; row0 = RH - 1;
+            } else if (wv == 2 && lane < 2 * STRIPS_PER_TILE) {
+                const int side = lane / STRIPS_PER_TILE, qs = lane % STRIPS_PER_TILE;
+                hj = side ? jt + TILE_X : jt - 1; hk = kt + qs * STRIP; take = STRIP; col = side ? RW - 1 : 0;
+                row0 = 1 + qs * STRIP;
             }
-            if (LAST) col4_nt(z, i2, P, zo);
-            strip_apply(w6, w4b, e4b, kx0p, kx1p, kyp, res2);
-            rf_ppcg_step<T, PREC>(g, ks2, e.k1, i2, kx0p, kx1p, kyp, res2, sc2, ca2, cb2, r1p, sd2);
+            if (hj >= 0) {
+                const bool inside = hj >= e.j0 && hj <= e.j1 && hk >= e.k0 && hk <= e.k1;
+                double sdn[STRIP];
+                if (inside) {
+                    const int h0 = IX(g, hj, hk);
+                    double hx0[STRIP], hx1[STRIP], hy[STRIP + 1], hr[STRIP], res[STRIP], sc[STRIP];
+                    rf_faces<T>(W, Wy, Kx, Ky, h0, P, hx0, hx1, hy);
+                    col4(r_in, h0, P, hr);
+                    strip_matvec(sd_in, hx0, hx1, hy, h0, P, res, sc);
+                    rf_ppcg_step<T, PREC>(g, hk, e.k1, h0, hx0, hx1, hy, res, sc, ca1, cb1, hr, sdn);
+                }
+                if (take < STRIP) {
+                    const int k = hk + take;
+                    if (hj <= e.j1 + 1)
+                        ring[row0][col] = (inside && k <= e.k1) ? sdn[take] : rf_ring_outside(g, e, sd_out, hj, k);
+                } else {
 #pragma unroll
-            for (int q = 0; q < STRIP; ++q) {
-                if (ks2 + q > e.k1 || !out_lane) continue;
-                const int i = i2 + q * P;
-                store_nt(d + i, d1p[q] + sc2[q]);
-                store_nt(r_out + i, r1p[q]);
-                store_nt(sd_out + i, sd2[q]);
-                if (LAST) {
-                    acc[0] += r1p[q] * (PREC != PREC_NONE ? zo[q] : r1p[q]);
-                    acc[1] += r1p[q] * r1p[q];
+                    for (int q = 0; q < STRIP; ++q)
+                        ring[row0 + q][col] =
+                            (inside && hk + q <= e.k1) ? sdn[q] : rf_ring_outside(g, e, sd_out, hj, hk + q);
                 }
             }
         }
-        s1p[0] = s1p[STRIP];
+        // step 1 on the own strip; its faces, r and d are kept for step 2
+        double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], rv[STRIP], dv[STRIP];
+        if (act) {
+            double res[STRIP], sc[STRIP], sdn[STRIP];
+            rf_faces<T>(W, Wy, Kx, Ky, i0, P, kx0, kx1, ky);
+            col4_nt(d, i0, P, dv);
+            col4_nt(r_in, i0, P, rv);
+            strip_matvec(sd_in, kx0, kx1, ky, i0, P, res, sc);
+            rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca1, cb1, rv, sdn);
 #pragma unroll
-        for (int q = 0; q < STRIP; ++q) {
-            s1p[q + 1] = s1[q];
-            r1p[q] = rv[q];
-            d1p[q] = dv[q];
+            for (int q = 0; q < STRIP; ++q) {
+                dv[q] = rnd<T>(dv[q] + sc[q]);
+                ring[1 + wv * STRIP + q][1 + lane] =
+                    (ks + q <= e.k1) ? sdn[q] : rf_ring_outside(g, e, sd_out, j, ks + q);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q)
+                ring[1 + wv * STRIP + q][1 + lane] = rf_ring_outside(g, e, sd_out, j, ks + q);
         }
-        c6[0] = c6[STRIP];
-        c6[1] = c6[STRIP + 1];
-        ky0 = ky[STRIP];
+        __syncthreads();
+        // step 2 on the own strip, sd from the ring
+        if (act) {
+            double c6[STRIP + 2], w4[STRIP], e4[STRIP], res[STRIP], sc[STRIP], sdn[STRIP], zo[STRIP];
+#pragma unroll
+            for (int q = 0; q < STRIP + 2; ++q) c6[q] = ring[wv * STRIP + q][1 + lane];
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                w4[q] = ring[1 + wv * STRIP + q][lane];
+                e4[q] = ring[1 + wv * STRIP + q][lane + 2];
+                sc[q] = c6[q + 1];
+            }
+            if (LAST) col4_nt(z, i0, P, zo);
+            strip_apply(c6, w4, e4, kx0, kx1, ky, res);
+            rf_ppcg_step<T, PREC>(g, ks, e.k1, i0, kx0, kx1, ky, res, sc, ca2, cb2, rv, sdn);
+#pragma unroll
+            for (int q = 0; q < STRIP; ++q) {
+                if (ks + q > e.k1) continue;
+                const int i = i0 + q * P;
+                store_nt(d + i, dv[q] + sc[q]);
+                store_nt(r_out + i, rv[q]);
+                store_nt(sd_out + i, sdn[q]);
+                if (LAST) {
+                    acc[0] += rv[q] * (PREC != PREC_NONE ? zo[q] : rv[q]);
+                    acc[1] += rv[q] * rv[q];
+                }
+            }
+        }
+        __syncthreads();
     }
-    TL_MARCH_LOOP_END
     if (LAST) block_partial<2>(acc, ra.partials);
 }
 
@@ -773,21 +679,22 @@ template <typename T, int FACES>
 void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
 {
     const Ext e = ext_interior(c);
+    const int grid = grid_of(e, c->cap);
     const March m = make_march(e, MARCH_COLS);
-    const int grid = march_grid(m, c->cap);
+    const int grid_m = march_grid(m, c->cap);
     T *const *f = reinterpret_cast<T *const *>(w.f);
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
         ra_u{c->partials, w.st, RF_RRN, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) hipEventRecord(ev[0], c->s);
-    hipLaunchKernelGGL((k_rf_w<T, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
+    hipLaunchKernelGGL((k_rf_w<T, FACES>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
                        f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
     if (ev) hipEventRecord(ev[1], c->s);
-    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid);
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid_m);
     if (ev) hipEventRecord(ev[2], c->s);
 #define TL_RFI(P)                                                                                       \
-    hipLaunchKernelGGL((k_rf_ur<T, P, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, f[RW_D],       \
-                       f[RW_P], f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], f[RW_KY], c->faces, ra_u)
+    hipLaunchKernelGGL((k_rf_ur<T, P, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
+                       f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], f[RW_KY], c->faces, ra_u)
     if (c->prec == PREC_BLOCK) TL_RFI(PREC_BLOCK);
     else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
     else TL_RFI(PREC_NONE);
@@ -891,23 +798,20 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
 {
     const Ext e = ext_interior(c);
     const int grid = grid_of(e, c->cap);
-    const March m1 = make_march(e, MARCH_COLS);
-    const int grid1 = march_grid(m1, c->cap);
+    const March m = make_march(e, MARCH_COLS);
+    const int grid_m = march_grid(m, c->cap);
     T *const *f = reinterpret_cast<T *const *>(w.f);
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW, nullptr, nullptr, 0},
         ra_u{c->partials, w.st, RF_RRN, nullptr, nullptr, 0};
-    hipLaunchKernelGGL((k_rf_w<T, ppcg_faces<T>()>), dim3(grid1), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m1, src,
+    hipLaunchKernelGGL((k_rf_w<T, ppcg_faces<T>()>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src,
                        f[RW_P], f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
-    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid1);
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid_m);
     const double *ca = w.coef, *cb = w.coef + RF_MAX_INNER;
     // two inner steps per launch: r and sd flip between their two buffers at
     // every pair; the outer kernel writes r to the second buffer when the
     // number of pairs is odd, so that every outer step ends with r in RW_R
     const int pairs = c->ppcg_steps_per_launch == 2 ? inner / 2 : 0;
-    March m2 = make_march(e, MARCH2_COLS);
-    set_march_rows(m2, e, MARCH2_ROWS);
-    const int grid2 = march_grid(m2, c->cap);
     T *r_cur = (pairs & 1) ? f[RW_R2] : f[RW_R], *r_oth = (pairs & 1) ? f[RW_R] : f[RW_R2];
     T *sd_cur = f[RW_SD], *sd_oth = f[RW_SD2];
     const double *Kx = c->f2[F_KX], *Ky = c->f2[F_KY];
@@ -919,25 +823,25 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
         int i = 0;                                                                                             \
         for (int k = 0; k < pairs; ++k, i += 2) {                                                              \
             if (i + 2 == inner)                                                                                \
-                hipLaunchKernelGGL((k_rf_inner2<T, P, 1>), dim3(grid2), dim3(BLOCK_THREADS), 0, c->s, w.g, e,   \
-                                   m2, f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], f[RW_KY], Kx, Ky,        \
-                                   f[RW_Z], ca, cb, i, ra_u);                                                  \
+                hipLaunchKernelGGL((k_rf_inner2<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
+                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], f[RW_KY], Kx, Ky, f[RW_Z],   \
+                                   ca, cb, i, ra_u);                                                           \
             else                                                                                               \
-                hipLaunchKernelGGL((k_rf_inner2<T, P, 0>), dim3(grid2), dim3(BLOCK_THREADS), 0, c->s, w.g, e,   \
-                                   m2, f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], f[RW_KY], Kx, Ky,        \
-                                   f[RW_Z], ca, cb, i, ra_u);                                                  \
+                hipLaunchKernelGGL((k_rf_inner2<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
+                                   f[RW_D], r_cur, r_oth, sd_cur, sd_oth, f[RW_W], f[RW_KY], Kx, Ky, f[RW_Z],   \
+                                   ca, cb, i, ra_u);                                                           \
             std::swap(r_cur, r_oth);                                                                           \
             std::swap(sd_cur, sd_oth);                                                                         \
         }                                                                                                      \
         for (; i < inner; ++i) {                                                                               \
             if (i == inner - 1)                                                                                \
-                hipLaunchKernelGGL((k_rf_inner<T, P, 1>), dim3(grid2), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
-                                   m2, f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], f[RW_KY], c->faces, f[RW_Z], ca, \
-                                   cb, i, ra_u);                                                               \
+                hipLaunchKernelGGL((k_rf_inner<T, P, 1>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
+                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], f[RW_KY], c->faces, f[RW_Z], ca, cb, \
+                                   i, ra_u);                                                                   \
             else                                                                                               \
-                hipLaunchKernelGGL((k_rf_inner<T, P, 0>), dim3(grid2), dim3(BLOCK_THREADS), 0, c->s, w.g, e,    \
-                                   m2, f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], f[RW_KY], c->faces, f[RW_Z], ca, \
-                                   cb, i, ra_u);                                                               \
+                hipLaunchKernelGGL((k_rf_inner<T, P, 0>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e,     \
+                                   f[RW_D], r_cur, sd_cur, sd_oth, f[RW_W], f[RW_KY], c->faces, f[RW_Z], ca, cb, \
+                                   i, ra_u);                                                                   \
             std::swap(sd_cur, sd_oth);                                                                         \
         }                                                                                                      \
     } while (0)
@@ -945,7 +849,7 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
     else if (c->prec == PREC_DIAG) TL_RFO(PREC_DIAG);
     else TL_RFO(PREC_NONE);
 #undef TL_RFO
-    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid2);
+    hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
 }
 
 int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body)
