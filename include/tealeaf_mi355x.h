@@ -144,6 +144,18 @@ int tl_initialise_chunk(tl_chunk *c, double xmin, double ymin, double dx, double
 int tl_generate_chunk(tl_chunk *c, int nstates, const double *density, const double *energy,
                       const double *xmin, const double *xmax, const double *ymin, const double *ymax,
                       const double *radius, const int *geometry, int g_rect, int g_circ, int g_point);
+/* generate_chunk for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight loops, harnesses that forbid hipMalloc): no
+ * counterpart in the reference. The same painting as tl_generate_chunk, whose
+ * state arrays are staged in device buffers allocated and freed on every
+ * call; here they are staged in a file-scope device array of
+ * TL_NOALLOC_MAX_STATES states (about 30 KB, one per device, held until the
+ * process ends). TL_ERR_ARG for more states. Calls that overlap on one device
+ * share that array and race. */
+enum { TL_NOALLOC_MAX_STATES = 512 };
+int tl_generate_chunk_noalloc(tl_chunk *c, int nstates, const double *density, const double *energy,
+                              const double *xmin, const double *xmax, const double *ymin, const double *ymax,
+                              const double *radius, const int *geometry, int g_rect, int g_circ, int g_point);
 int tl_set_field(tl_chunk *c);
 int tl_field_summary(tl_chunk *c, double *vol, double *mass, double *ie, double *temp);
 int tl_update_halo(tl_chunk *c, const int *chunk_neighbours, const int *fields, int depth);

@@ -31,6 +31,8 @@ MODE_DROPIN, MODE_NATIVE = 0, 1
 TUNE_PPCG_STEPS_PER_LAUNCH, TUNE_CHEBY_STEPS_PER_LAUNCH = 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 G_RECT, G_CIRC, G_POINT = 1, 2, 3
+# TL_NOALLOC_MAX_STATES: the most states tl_generate_chunk_noalloc stages
+NOALLOC_MAX_STATES = 512
 
 FIELDS = ["density", "energy0", "energy1", "volume", "xarea", "yarea", "u", "u0", "p", "r", "w",
           "Mi", "Kx", "Ky", "sd", "z", "cp", "bfp"]
@@ -86,6 +88,7 @@ _SIGS = {
     "tl_field1d_get": (_I, [_P, _I, _DP, _I]),
     "tl_initialise_chunk": (_I, [_P, _D, _D, _D, _D]),
     "tl_generate_chunk": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _IP, _I, _I, _I]),
+    "tl_generate_chunk_noalloc": (_I, [_P, _I, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _IP, _I, _I, _I]),
     "tl_set_field": (_I, [_P]),
     "tl_field_summary": (_I, [_P, _DP, _DP, _DP, _DP]),
     "tl_update_halo": (_I, [_P, _IP, _IP, _I]),
@@ -233,13 +236,15 @@ class Chunk:
         _check(self.lib.tl_initialise_chunk(self.h, xmin, ymin, dx, dy), "initialise_chunk")
 
     def generate_chunk(self, density, energy, xmin, xmax, ymin, ymax, radius, geometry,
-                       g_rect=G_RECT, g_circ=G_CIRC, g_point=G_POINT):
+                       g_rect=G_RECT, g_circ=G_CIRC, g_point=G_POINT, noalloc=False):
+        """tl_generate_chunk, or with noalloc tl_generate_chunk_noalloc (the
+        state arrays staged without a device allocation)."""
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
                 (density, energy, xmin, xmax, ymin, ymax, radius)]
         geom = np.ascontiguousarray(geometry, dtype=np.int32)
         n = len(arrs[0])
-        _check(self.lib.tl_generate_chunk(self.h, n, *[_dptr(a) for a in arrs], _iptr(geom),
-                                          g_rect, g_circ, g_point), "generate_chunk")
+        fn = self.lib.tl_generate_chunk_noalloc if noalloc else self.lib.tl_generate_chunk
+        _check(fn(self.h, n, *[_dptr(a) for a in arrs], _iptr(geom), g_rect, g_circ, g_point), "generate_chunk")
 
     def set_field(self):
         _check(self.lib.tl_set_field(self.h), "set_field")

@@ -14,6 +14,7 @@
 struct tl_chunk {
     tl::Geom g{};
     int prec = tl::PREC_NONE;
+    int device = 0;                  // the device current at creation
     int profiler_on = 0;
     int solver = tl::SOLVER_JACOBI;  // side-channel solver (informational)
     hipStream_t s = nullptr;

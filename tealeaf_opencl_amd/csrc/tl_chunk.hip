@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -274,6 +275,7 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     c->g.x_max = x_max;
     c->g.y_max = y_max;
     c->g.hd = halo_depth;
+    if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
     c->prec = preconditioner;
     c->ppcg_steps_per_launch = g_ppcg_steps_default;
     c->cheby_steps_per_launch = g_cheby_steps_default;
@@ -437,6 +439,26 @@ extern "C" int tl_initialise_chunk(tl_chunk *c, double xmin, double ymin, double
 }
 
 // generate_chunk_kernel_ocl.cpp:25-91 (all states painted in one pass)
+namespace {
+
+// the state painting of generate_chunk_kernel_ocl.cpp:25-91 from state arrays
+// already on the device (dev: density, energy, xmin, xmax, ymin, ymax, radius,
+// nstates each), then the wait the reference ends with
+int generate_staged(tl_chunk *c, int nstates, const double *dev, const int *dgeom, int g_rect, int g_circ,
+                    int g_point)
+{
+    const double *sarr[7];
+    for (int a = 0; a < 7; ++a) sarr[a] = dev + a * nstates;
+    prof_begin(c);
+    launch_generate(c->s, c->g, ext_interior(c), c->cap, c->f1, c->f2[F_DENSITY], c->f2[F_ENERGY0], c->f2[F_U],
+                    c->f2[F_U0], nstates, sarr, dgeom, g_rect, g_circ, g_point);
+    TL_CHECK_LAUNCH(c, "generate_chunk");
+    prof_end(c, "generate_chunk");
+    return tl_synchronize(c);
+}
+
+}  // namespace
+
 extern "C" int tl_generate_chunk(tl_chunk *c, int nstates, const double *density, const double *energy,
                                  const double *xmin, const double *xmax, const double *ymin,
                                  const double *ymax, const double *radius, const int *geometry,
@@ -458,17 +480,76 @@ extern "C" int tl_generate_chunk(tl_chunk *c, int nstates, const double *density
     for (int a = 0; a < 7; ++a)
         hipMemcpyAsync(dev + a * nstates, src[a], sizeof(double) * nstates, hipMemcpyHostToDevice, c->s);
     hipMemcpyAsync(dgeom, geometry, sizeof(int) * nstates, hipMemcpyHostToDevice, c->s);
-    const double *sarr[7];
-    for (int a = 0; a < 7; ++a) sarr[a] = dev + a * nstates;
-    prof_begin(c);
-    launch_generate(c->s, c->g, ext_interior(c), c->cap, c->f1, c->f2[F_DENSITY], c->f2[F_ENERGY0], c->f2[F_U],
-                    c->f2[F_U0], nstates, sarr, dgeom, g_rect, g_circ, g_point);
-    TL_CHECK_LAUNCH(c, "generate_chunk");
-    prof_end(c, "generate_chunk");
-    const int rc = tl_synchronize(c);
+    const int rc = generate_staged(c, nstates, dev, dgeom, g_rect, g_circ, g_point);
     hipFree(dev);
     hipFree(dgeom);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// generate_chunk without allocation: the state arrays are staged in `scratch`,
+// a file-scope device array (one copy per device, held until the process
+// ends), instead of device buffers allocated and freed per call. Calls that
+// overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_DOUBLES = 7 * (size_t)TL_NOALLOC_MAX_STATES;
+constexpr size_t NOALLOC_BYTES = sizeof(double) * NOALLOC_DOUBLES + sizeof(int) * TL_NOALLOC_MAX_STATES;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+// device address of `scratch` on each device, looked up once per device with
+// that device current (in host code the array's name is only a stand-in)
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int scratch_of(int dev, unsigned char **out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        set_error("tl_generate_chunk_noalloc: device %d outside 0..%d", dev, NOALLOC_MAX_DEVICES - 1);
+        return TL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return hip_fail("hipGetDevice", e);
+        if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return hip_fail("hipSetDevice", e);
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) hipSetDevice(cur);
+        if (e != hipSuccess) return hip_fail("hipGetSymbolAddress(scratch)", e);
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return TL_OK;
+}
+
+}  // namespace
+
+extern "C" int tl_generate_chunk_noalloc(tl_chunk *c, int nstates, const double *density, const double *energy,
+                                         const double *xmin, const double *xmax, const double *ymin,
+                                         const double *ymax, const double *radius, const int *geometry,
+                                         int g_rect, int g_circ, int g_point)
+{
+    if (!c || nstates < 1 || nstates > TL_NOALLOC_MAX_STATES) {
+        set_error("tl_generate_chunk_noalloc: %d states outside 1..%d", nstates, TL_NOALLOC_MAX_STATES);
+        return TL_ERR_ARG;
+    }
+    unsigned char *ptr = nullptr;
+    const int rs = scratch_of(c->device, &ptr);
+    if (rs != TL_OK) return rs;
+    double *dev = reinterpret_cast<double *>(ptr);
+    int *dgeom = reinterpret_cast<int *>(ptr + sizeof(double) * NOALLOC_DOUBLES);
+    const double *src[7] = {density, energy, xmin, xmax, ymin, ymax, radius};
+    hipError_t e = hipSuccess;
+    for (int a = 0; a < 7 && e == hipSuccess; ++a)
+        e = hipMemcpyAsync(dev + a * nstates, src[a], sizeof(double) * nstates, hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dgeom, geometry, sizeof(int) * nstates, hipMemcpyHostToDevice, c->s);
+    if (e != hipSuccess) return hip_fail("tl_generate_chunk_noalloc: state copy", e);
+    return generate_staged(c, nstates, dev, dgeom, g_rect, g_circ, g_point);
 }
 
 // set_field_kernel_ocl.cpp:3-7

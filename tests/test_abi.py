@@ -60,3 +60,9 @@ def test_no_gpu_means_loud_failure(hiplib):
         pytest.skip("a GPU is present")
     with pytest.raises(hiplib.TealeafError):
         hiplib.Chunk(16, 16)
+
+
+def test_python_constants_match_header(hiplib):
+    text = open(os.path.join(ROOT, "include", "tealeaf_mi355x.h")).read()
+    m = re.search(r"TL_NOALLOC_MAX_STATES\s*=\s*(\d+)", text)
+    assert m and int(m.group(1)) == hiplib.NOALLOC_MAX_STATES
