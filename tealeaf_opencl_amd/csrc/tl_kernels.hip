@@ -1348,10 +1348,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby(Geom g, Ext e, Ma
 
 // Two Chebyshev iterations n, n+1 per launch, n odd (tea_solve.f90:223 forms
 // the residual norm only on iterations divisible by 10, so only the second one
-// can need it), as the band march with a one-strip lag of
-// k_fused_ppcg_inner2: iteration n on strip S, then iteration n+1 on strip
+// can need it), as a band march with a one-strip lag: iteration n on 4-row
+// strip S, then iteration n+1 on strip
 // S-1 from iteration n's u of the rows it spans plus one row either side
-// (kept in registers with S-1's new p and u0; its faces are read again).
+// (kept in registers with S-1's new p, u0 and faces).
 // u and p are read from one buffer and written to the other. When the tail of
 // iteration n will stop the solve (max_iters, or the stale-error exit test of
 // tea_solve.f90:322-328, both known from the state before the launch) only
@@ -1390,8 +1390,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, M
         c6[1] = u_in[i + P];
     }
     double ky0 = Ky[IX(g, j, ks)];
-    // iteration n's results for strip S-1 (u row above it in u1p[0])
-    double u1p[STRIP + 1] = {}, p1p[STRIP] = {}, bp[STRIP] = {};
+    // iteration n's results for strip S-1 (u row above it in u1p[0]), and
+    // that strip's faces
+    // (its bottom face is ky0, which the next strip's top face shares)
+    double u1p[STRIP + 1] = {}, p1p[STRIP] = {}, bp[STRIP] = {}, kxp[STRIP] = {}, kyq[STRIP] = {};
     for (; ks <= ks_last + (two ? STRIP : 0); ks += STRIP) {
         const int i0 = IX(g, j, ks);
         // ---- iteration n on strip ks
@@ -1435,9 +1437,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, M
                 double w6[STRIP + 2], w4b[STRIP], e4b[STRIP], wv2[STRIP], uc2[STRIP], rv2[STRIP], pn2[STRIP],
                     un2[STRIP], kx0p[STRIP], kx1p[STRIP], kyp[STRIP + 1];
 #pragma unroll
-                for (int q = 0; q < STRIP; ++q) kx0p[q] = Kx[i2 + q * P];
+                for (int q = 0; q < STRIP; ++q) kx0p[q] = kxp[q];
 #pragma unroll
-                for (int q = 0; q < STRIP + 1; ++q) kyp[q] = Ky[i2 + q * P];
+                for (int q = 0; q < STRIP; ++q) kyp[q] = kyq[q];
+                kyp[STRIP] = ky0;
 #pragma unroll
                 for (int q = 0; q < STRIP + 1; ++q) w6[q] = u1p[q];
                 w6[STRIP + 1] = u1[0];
@@ -1466,6 +1469,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, M
                 u1p[q + 1] = u1[q];
                 p1p[q] = pn[q];
                 bp[q] = b[q];
+                kxp[q] = kx0[q];
+                kyq[q] = ky[q];
             }
         }
         c6[0] = c6[STRIP];
