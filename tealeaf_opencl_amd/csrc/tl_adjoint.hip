@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_adj_seed(Geom g, Ext e, const
 __global__ void __launch_bounds__(FINISH_THREADS) k_adj_sum(const double *partials, int nblocks, RedArgs ra)
 {
     double tot[MAX_RED];
-    sum_partials(partials, nblocks, 1, tot);
+    sum_partials<1>(partials, nblocks, tot);
     if (threadIdx.x == 0) {
         ra.result[0] = tot[0];
         __hip_atomic_store(ra.flag, ra.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);

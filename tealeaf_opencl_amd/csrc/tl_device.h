@@ -197,31 +197,38 @@ __device__ __forceinline__ void block_partial(const double (&v)[NR], double *par
 
 constexpr int FINISH_THREADS = 1024;
 
-// Sum of the first nq partials of nblocks blocks, in a fixed order, by one
-// block of FINISH_THREADS threads; every thread gets the totals. The loads of
-// four blocks' partials are issued before any is added so that a grid of up
-// to 4 * FINISH_THREADS blocks costs one memory round trip.
-__device__ __forceinline__ void sum_partials(const double *partials, int nblocks, int nq, double *tot)
+// Sum of the first NQ partials of nblocks blocks, in a fixed order, by one
+// block of FINISH_THREADS threads; every thread gets the totals (tot[q] for
+// q < NQ, 0 beyond). Thread t adds blocks t, t + FINISH_THREADS, ... in that
+// order whatever the unroll, so the totals do not depend on it. The loads of
+// U blocks' partials are issued before any is added: a grid of up to
+// U * FINISH_THREADS blocks costs one memory round trip (8192 blocks for one
+// or two quantities, 4096 for more).
+template <int NQ>
+__device__ __forceinline__ void sum_partials(const double *partials, int nblocks, double *tot)
 {
-    __shared__ double lds[FINISH_THREADS / 64][MAX_RED];
+    static_assert(NQ >= 1 && NQ <= MAX_RED, "quantities");
+    __shared__ double lds[FINISH_THREADS / 64][NQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double acc[MAX_RED] = {0.0, 0.0, 0.0, 0.0};
-    constexpr int U = 4;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    constexpr int U = NQ <= 2 ? 8 : 4;
     for (int b0 = threadIdx.x; b0 < nblocks; b0 += U * FINISH_THREADS) {
-        double v[U][MAX_RED];
+        double v[U][NQ];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int b = b0 + u * FINISH_THREADS;
 #pragma unroll
-            for (int q = 0; q < MAX_RED; ++q) v[u][q] = (q < nq && b < nblocks) ? partials[q * nblocks + b] : 0.0;
+            for (int q = 0; q < NQ; ++q) v[u][q] = b < nblocks ? partials[q * nblocks + b] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int q = 0; q < MAX_RED; ++q) acc[q] += v[u][q];
+            for (int q = 0; q < NQ; ++q) acc[q] += v[u][q];
     }
 #pragma unroll
-    for (int q = 0; q < MAX_RED; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const double s = wave_sum(acc[q]);
         if (lane == 0) lds[w][q] = s;
     }
@@ -229,7 +236,8 @@ __device__ __forceinline__ void sum_partials(const double *partials, int nblocks
 #pragma unroll
     for (int q = 0; q < MAX_RED; ++q) {
         double t = 0.0;
-        for (int i = 0; i < FINISH_THREADS / 64; ++i) t += lds[i][q];
+        if (q < NQ)
+            for (int i = 0; i < FINISH_THREADS / 64; ++i) t += lds[i][q];
         tot[q] = t;
     }
 }

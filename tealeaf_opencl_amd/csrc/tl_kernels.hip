@@ -27,7 +27,7 @@ namespace tl {
 
 // tea_solve.f90:322-331: error = SQRT(error), exit test, old_error, then the
 // solver-switch test at the top of the next iteration (:151-165)
-__device__ void iteration_tail(SolveState *s, double *hist_e)
+__device__ __forceinline__ void iteration_tail(SolveState *s, double *hist_e)
 {
     s->error = sqrt(s->error);
     if (hist_e) hist_e[s->n] = s->error;
@@ -60,14 +60,18 @@ __device__ __forceinline__ bool cheby_norm_due(const SolveState *s)
     return s->n >= s->est_itc && (s->n % 10) == 0;
 }
 
-__device__ void finish_scalars(const RedArgs &ra, const double *tot)
+__device__ __forceinline__ void finish_results(const RedArgs &ra, const double *tot)
 {
 #pragma unroll
     for (int q = 0; q < MAX_RED; ++q) ra.result[q] = tot[q];
     // the release orders the totals before the flag for the host's reader
     if (ra.flag) __hip_atomic_store(ra.flag, ra.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    SolveState *s = ra.st;
-    if (!s) return;
+}
+
+// The scalar algebra of the mode on s, a copy of the solver state in
+// registers that the caller writes back
+__device__ __forceinline__ void finish_state(const RedArgs &ra, const double *tot, SolveState *s)
+{
     switch (ra.mode) {
     case RED_INIT_RESID:
         s->old_error = tot[0];
@@ -148,13 +152,34 @@ __device__ __forceinline__ bool iteration_mode(int mode)
 
 // Sums the partials of the preceding reduction kernel in block order and runs
 // the scalar algebra of its mode. An iteration-mode finisher of a halted
-// solver does nothing, like the kernel before it.
-__global__ void __launch_bounds__(FINISH_THREADS) k_finish(RedArgs ra, int nblocks, int nq)
+// solver changes nothing, like the kernel before it. The state is copied into
+// registers as the partials are loaded, so that its loads overlap theirs and
+// the algebra makes no round trips to memory; the copy is stored back whole.
+template <int NQ>
+__global__ void __launch_bounds__(FINISH_THREADS) k_finish(RedArgs ra, int nblocks)
 {
-    if (ra.st && iteration_mode(ra.mode) && (ra.st->halt || ra.st->fault)) return;
     double tot[MAX_RED];
-    sum_partials(ra.partials, nblocks, nq, tot);
-    if (threadIdx.x == 0) finish_scalars(ra, tot);
+    // (the four-quantity reduction, the field summary, carries no solver state)
+    if (NQ == MAX_RED || !ra.st) {
+        sum_partials<NQ>(ra.partials, nblocks, tot);
+        if (threadIdx.x == 0) finish_results(ra, tot);
+        return;
+    }
+    SolveState loc = *ra.st;
+    sum_partials<NQ>(ra.partials, nblocks, tot);
+    if (threadIdx.x != 0 || (iteration_mode(ra.mode) && (loc.halt || loc.fault))) return;
+    finish_results(ra, tot);
+    finish_state(ra, tot, &loc);
+    *ra.st = loc;
+}
+
+// the finisher of a reduction kernel with nq sums per block
+void launch_finish(hipStream_t s, const RedArgs &ra, int nblocks, int nq)
+{
+    if (nq <= 1) hipLaunchKernelGGL(k_finish<1>, dim3(1), dim3(FINISH_THREADS), 0, s, ra, nblocks);
+    else if (nq == 2) hipLaunchKernelGGL(k_finish<2>, dim3(1), dim3(FINISH_THREADS), 0, s, ra, nblocks);
+    else if (nq == 3) hipLaunchKernelGGL(k_finish<3>, dim3(1), dim3(FINISH_THREADS), 0, s, ra, nblocks);
+    else hipLaunchKernelGGL(k_finish<MAX_RED>, dim3(1), dim3(FINISH_THREADS), 0, s, ra, nblocks);
 }
 
 __device__ __forceinline__ bool halted(const SolveState *st) { return st && st->halt; }
@@ -1534,7 +1559,7 @@ int march_grid(const March &m, int cap)
         const int grid_ = grid_for(e, cap);                                                        \
         hipLaunchKernelGGL(kern, dim3(grid_), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__);         \
         if (ev) hipEventRecord(ev, stream);                                                        \
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, stream, ra, grid_, nq);      \
+        launch_finish(stream, ra, grid_, nq);                                                      \
     } while (0)
 
 void launch_init_chunk_1d(hipStream_t s, const Geom &g, double xmin, double ymin, double dx, double dy,
@@ -1776,7 +1801,7 @@ void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, cons
     const int grid = march_grid(m, cap);
     hipLaunchKernelGGL(k_fused_cg_w, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, src, pA, pB, W, fc, ra);
     if (ev_main) hipEventRecord(ev_main, s);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
+    launch_finish(s, ra, grid, 1);
 }
 
 void launch_cg_u_fix(hipStream_t s, const Geom &g, const Ext &e, int cap, double *u, const double *p,
@@ -1902,7 +1927,7 @@ void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, in
     else TL_FC2(PREC_NONE);
 #undef TL_FC2
     if (ev_main) hipEventRecord(ev_main, s);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
+    launch_finish(s, ra, grid, 1);
 }
 
 void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, const double *u0,
@@ -1920,7 +1945,7 @@ void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int
     else TL_FC1(PREC_NONE);
 #undef TL_FC1
     if (ev_main) hipEventRecord(ev_main, s);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(FINISH_THREADS), 0, s, ra, grid, 1);
+    launch_finish(s, ra, grid, 1);
 }
 
 }  // namespace tl

@@ -65,7 +65,7 @@ struct RfArgs {
 // replacements in a row without progress after which the solve stops
 constexpr int STALL_LIMIT = 3;
 
-__device__ void rf_scalars(RefineState *s, const RfArgs &ra, const double *tot)
+__device__ __forceinline__ void rf_scalars(RefineState *s, const RfArgs &ra, const double *tot)
 {
     const double rz = tot[0], rr = tot[1];
     switch (ra.mode) {
@@ -127,12 +127,17 @@ __device__ void rf_scalars(RefineState *s, const RfArgs &ra, const double *tot)
     }
 }
 
+// The state is copied into registers as the partials are loaded and stored
+// back whole (as k_finish does); a halted inner iteration changes nothing.
 __global__ void __launch_bounds__(FINISH_THREADS) k_rf_finish(RfArgs ra, int nblocks)
 {
-    if (ra.mode != RF_UPDATE && ra.st->halt) return;
+    RefineState loc;
+    if (threadIdx.x == 0) loc = *ra.st;
     double tot[MAX_RED];
-    sum_partials(ra.partials, nblocks, 2, tot);
-    if (threadIdx.x == 0) rf_scalars(ra.st, ra, tot);
+    sum_partials<2>(ra.partials, nblocks, tot);
+    if (threadIdx.x != 0 || (ra.mode != RF_UPDATE && loc.halt)) return;
+    rf_scalars(&loc, ra, tot);
+    *ra.st = loc;
 }
 
 template <typename T>
