@@ -58,8 +58,10 @@ REF_BYTES = {"cg": 144, "cheby": 104, "ppcg_inner": 112, "ppcg_outer": 208}
 # refined CG kernels: faces from one per-cell field (FP64: the conductivity,
 # FP32: 1/(2w))
 ALG_BYTES_RF = {
-    "fp64": {"k_rf_w": 4 * 8, "k_rf_ur": 7 * 8},
-    "mixed": {"k_rf_w": 4 * 4, "k_rf_ur": 7 * 4},
+    # k_rf_ur: R p, r, w  W r, z, plus every second iteration R d, p_prev  W d
+    # (the lagged update of d); the timed launches alternate: (5 + 8) / 2 values
+    "fp64": {"k_rf_w": 4 * 8, "k_rf_ur": (5 * 8 + 8 * 8) // 2},
+    "mixed": {"k_rf_w": 4 * 4, "k_rf_ur": (5 * 4 + 8 * 4) // 2},
 }
 REF_BYTES_PER_ITER = 144  # reference CG + strip-4 block Jacobi (calc_w + calc_ur + calc_p)
 

@@ -149,13 +149,15 @@ struct RefineState {
     double rr_prev;       // ||r||^2 of the previous true residual
     double theta;         // PPCG: centre of the eigenvalue interval (sd = z / theta)
     int stop_n;           // > 0: halt (without an update) once n reaches it
-    int pad2_;
+    int d_lag;            // 1: d lacks the last iteration's alpha p (added by the next one)
+    double alpha_prev;    // alpha of the last iteration (what d lacks while d_lag)
 };
 
 enum RefineMode {
     RF_UPDATE = 0,        // true residual r.z, r.r after u += d
     RF_PW,                // p.w -> alpha
-    RF_RRN                // recursive r.z, r.r -> beta, update / exit tests
+    RF_RRN,               // recursive r.z, r.r -> beta, update / exit tests
+    RF_RRN_LAG            // the same after a CG iteration whose d update lags one iteration
 };
 
 }  // namespace tl

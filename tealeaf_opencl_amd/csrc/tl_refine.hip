@@ -84,6 +84,7 @@ __device__ __forceinline__ void rf_scalars(RefineState *s, const RfArgs &ra, con
             s->stalls = (rr > 0.25 * s->rr_prev) ? s->stalls + 1 : 0;
         }
         s->rr_prev = rr;
+        s->d_lag = 0;  // d is cleared after the update
         s->rro = rz;
         s->rr_true = rr;
         s->rr = rr;
@@ -105,6 +106,12 @@ __device__ __forceinline__ void rf_scalars(RefineState *s, const RfArgs &ra, con
         s->alpha = s->rro / s->pw;
         if (ra.hist_a && s->n < ra.hist_len) ra.hist_a[s->n] = s->alpha;
         break;
+    case RF_RRN_LAG:
+        // k_rf_ur adds alpha p into d every second iteration (both steps at
+        // once); the PPCG outer step updates d every time (RF_RRN)
+        s->alpha_prev = s->alpha;
+        s->d_lag ^= 1;
+        [[fallthrough]];
     case RF_RRN: {
         s->rz_prev = s->rro;
         s->beta = rz / s->rro;
@@ -363,28 +370,39 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
 
 // d += alpha p, r -= alpha A p (A p recomputed from the stored p),
 // z = M^-1 r with the preconditioner recomputed from the stored
-// coefficients; r.z and r.r of the stored values.
+// coefficients; r.z and r.r of the stored values. The update of d, which
+// nothing in the iteration reads, is made every second iteration: with
+// d_lag clear the kernel leaves d alone, with d_lag set it adds
+// alpha_prev p_prev (still in the other p buffer) and then alpha p, each as
+// one FMA rounded to the storage, so d is what per-iteration updates give.
+// A reliable update or the switch to PPCG first completes d (k_rf_d_fix).
 template <typename T, int PREC, int FACES>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, const T *pA, const T *pB, T *r,
                                                          T *z, const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
     const RefineState *st = ra.st;
     if (st->halt) return;
-    const T *p = (st->n & 1) ? pB : pA;
-    const double alpha = st->alpha;
+    const bool odd = (st->n & 1) != 0;
+    const T *p = odd ? pB : pA;
+    const T *p_prev = odd ? pA : pB;
+    const double alpha = st->alpha, alpha_prev = st->alpha_prev;
+    const bool upd = st->d_lag != 0;
     double acc[2] = {0.0, 0.0};
     TL_TILE_LOOP(e)
     const int i0 = IX(g, j, ks), P = g.pitch;
     double kx0[STRIP], kx1[STRIP], ky[STRIP + 1], wv[STRIP], pc[STRIP], dv[STRIP], rv[STRIP], zv[STRIP];
     rf_strip_faces<T, FACES>(g, fc, W, Wy, i0, P, j, ks, kx0, kx1, ky);
-    col4_nt(d, i0, P, dv);
+    if (upd) col4_nt(d, i0, P, dv);
     col4_nt(r, i0, P, rv);
     strip_matvec(p, kx0, kx1, ky, i0, P, wv, pc);
+    if (upd) {
+        double pp[STRIP];
+        col4_nt(p_prev, i0, P, pp);
 #pragma unroll
-    for (int q = 0; q < STRIP; ++q) {
-        dv[q] += alpha * pc[q];
-        rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
+        for (int q = 0; q < STRIP; ++q) dv[q] = fma(alpha, pc[q], rnd<T>(fma(alpha_prev, pp[q], dv[q])));
     }
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) rv[q] = rnd<T>(rv[q] - alpha * wv[q]);
     mask_rows(ks, e.k1, rv);
     strip_precondition<PREC, inner_fast_rcp<T>()>(g, ks, i0, kx0, kx1, ky, rv, zv);
 #pragma unroll
@@ -392,7 +410,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
         if (ks + q > e.k1) continue;
         const int i = i0 + q * P;
         const T zt = (T)zv[q];
-        store_nt(d + i, dv[q]);
+        if (upd) store_nt(d + i, dv[q]);
         store_nt(r + i, rv[q]);
         if (PREC != PREC_NONE) store_nt(z + i, (double)zt);
         acc[0] += rv[q] * (double)zt;
@@ -400,6 +418,29 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_ur(Geom g, Ext e, T *d, co
     }
     TL_TILE_LOOP_END
     block_partial<2>(acc, ra.partials);
+}
+
+// Completes d after CG iterations of k_rf_ur that left it lagging:
+// d += alpha_prev p of the last iteration (written into buffer (n-1) & 1).
+// Nothing to do when d_lag is clear; the update that follows clears it.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK_THREADS) k_rf_d_fix(Geom g, Ext e, T *d, const T *pA, const T *pB,
+                                                            const RefineState *st)
+{
+    if (!st->d_lag) return;
+    const T *p = ((st->n - 1) & 1) ? pB : pA;
+    const double alpha = st->alpha_prev;
+    TL_TILE_LOOP(e)
+    const int i0 = IX(g, j, ks), P = g.pitch;
+    double dv[STRIP], pc[STRIP];
+    col4_nt(d, i0, P, dv);
+    col4_nt(p, i0, P, pc);
+#pragma unroll
+    for (int q = 0; q < STRIP; ++q) {
+        if (ks + q > e.k1) continue;
+        store_nt(d + i0 + q * P, fma(alpha, pc[q], dv[q]));
+    }
+    TL_TILE_LOOP_END
 }
 
 // PPCG outer step, second half (the refined form of k_fused_cg_ur in PPCG
@@ -657,6 +698,8 @@ void launch_update_f(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, d
     const int grid = grid_of(e, c->cap);
     const RfArgs ra{c->partials, w.st, RF_UPDATE, nullptr, nullptr, 0};
     T *const *f = reinterpret_cast<T *const *>(w.f);
+    hipLaunchKernelGGL(k_rf_d_fix<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], f[RW_P2],
+                       w.st);
 #define TL_RFU(P)                                                                                           \
     hipLaunchKernelGGL((k_rf_update<T, P, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, w.g, e, u_in, \
                        u_out, f[RW_D], c->f2[F_U0], c->f2[F_COND], f[RW_W], f[RW_KY], c->faces, f[RW_R], f[RW_Z], \
@@ -690,7 +733,7 @@ void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
     T *const *f = reinterpret_cast<T *const *>(w.f);
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
-        ra_u{c->partials, w.st, RF_RRN, w.hist, w.hist + RF_HIST, RF_HIST};
+        ra_u{c->partials, w.st, RF_RRN_LAG, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) hipEventRecord(ev[0], c->s);
     hipLaunchKernelGGL((k_rf_w<T, FACES>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
                        f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
@@ -880,12 +923,13 @@ int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::funct
     return TL_OK;
 }
 
-// CG iterations: the first iteration of each batch is launched eagerly (timed)
+// CG iterations: the first two iterations of each batch are launched eagerly
+// (timed), the other RF_BATCH - 2 replayed
 template <typename T>
 int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
 {
     return capture_into(c, w.graph, c->prec * 4 + w.faces, [&]() {
-        for (int b = 0; b < RF_BATCH - 1; ++b) launch_iteration<T>(c, w);
+        for (int b = 0; b < RF_BATCH - 2; ++b) launch_iteration<T>(c, w);
     });
 }
 
@@ -905,17 +949,22 @@ struct KernelTimes {
 };
 
 // Replays `exec` until the device state halts; batch i+1 is queued before the
-// state after batch i is examined. With `timed`, each batch starts with one
-// eagerly launched CG iteration bracketed by events; the events of a batch
-// are read once it has finished (every examined batch follows one that did
-// not halt, so its eager iteration ran).
+// state after batch i is examined. With `timed`, each batch starts with two
+// eagerly launched CG iterations, each kernel bracketed by events; the events
+// of a batch are read once it has finished, and only when it did not halt
+// (then both eager iterations ran).
 template <typename T>
 int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, hipGraphExec_t exec, bool timed, long max_batches,
                KernelTimes *kt)
 {
     hipError_t e;
     auto enqueue = [&](int sl) -> hipError_t {
-        if (timed) launch_iteration<T>(c, w, c->ev_t[sl]);
+        if (timed) {
+            // two iterations, so that both forms of k_rf_ur (with and without
+            // the lagged update of d) are timed
+            launch_iteration<T>(c, w, c->ev_t[sl]);
+            launch_iteration<T>(c, w, c->ev_t[sl] + 4);
+        }
         hipError_t ee = hipGraphLaunch(exec, c->s);
         if (ee != hipSuccess) return ee;
         ee = hipMemcpyAsync(&w.h_st[sl], w.st, sizeof(RefineState), hipMemcpyDeviceToHost, c->s);
@@ -927,14 +976,17 @@ int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, hipGraphExec_t exec, bool t
     for (long guard = 0;; ++guard) {
         if ((e = enqueue(1 - slot)) != hipSuccess) return hip_fail("hipGraphLaunch", e);
         if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
-        float a = 0.f, b = 0.f;
-        if (timed && hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
-            hipEventElapsedTime(&b, c->ev_t[slot][2], c->ev_t[slot][3]) == hipSuccess) {
-            kt->k1_ms += a;
-            kt->k2_ms += b;
-            kt->samples += 1;
-        }
         if (w.h_st[slot].halt) break;
+        // a batch that did not halt ran both timed iterations
+        float a = 0.f, b = 0.f, a2 = 0.f, b2 = 0.f;
+        if (timed && hipEventElapsedTime(&a, c->ev_t[slot][0], c->ev_t[slot][1]) == hipSuccess &&
+            hipEventElapsedTime(&b, c->ev_t[slot][2], c->ev_t[slot][3]) == hipSuccess &&
+            hipEventElapsedTime(&a2, c->ev_t[slot][4], c->ev_t[slot][5]) == hipSuccess &&
+            hipEventElapsedTime(&b2, c->ev_t[slot][6], c->ev_t[slot][7]) == hipSuccess) {
+            kt->k1_ms += a + a2;
+            kt->k2_ms += b + b2;
+            kt->samples += 2;
+        }
         if (guard > max_batches) {
             hipStreamSynchronize(c->s);
             set_error("refined solve did not halt after %ld batches", guard);
@@ -1037,6 +1089,13 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
         calc_ch_coefs(ca, cb, eigmin, eigmax, &theta, inner);
         hipMemcpy(w.coef, ca.data(), sizeof(double) * inner, hipMemcpyHostToDevice);
         hipMemcpy(w.coef + RF_MAX_INNER, cb.data(), sizeof(double) * inner, hipMemcpyHostToDevice);
+        // the outer steps update d at every step: complete the CG steps' d first
+        {
+            T *const *f = reinterpret_cast<T *const *>(w.f);
+            hipLaunchKernelGGL(k_rf_d_fix<T>, dim3(grid_of(ext_interior(c), c->cap)), dim3(BLOCK_THREADS), 0, c->s,
+                               w.g, ext_interior(c), f[RW_D], f[RW_P], f[RW_P2], w.st);
+        }
+        h.d_lag = 0;
         h.theta = theta;
         h.stop_n = 0;
         h.halt = 0;
