@@ -49,7 +49,9 @@ void set_field_kernel_ocl_(void);
 void field_summary_kernel_ocl_(double *vol, double *mass, double *ie, double *temp);
 /* update_halo_kernel_ocl.cpp:6-12 -- reflective fill of density/energy0/energy1. */
 void update_halo_kernel_ocl_(const int *chunk_neighbours, const int *fields, const int *depth);
-/* tea_leaf_kernel_ocl.cpp:233-239 -- rx, ry are outputs; Kx, Ky; walls; u = u0 = e1*density. */
+/* tea_leaf_kernel_ocl.cpp:233-239 -- rx, ry are outputs; Kx, Ky; walls; u = u0 = e1*density.
+ * Aborts (as on a bad coefficient) when an interior density lies outside
+ * [2^-499, 2^499] or is not a number (see tl_common_init). */
 void tea_leaf_common_init_kernel_ocl_(const int *coefficient, double *dt, double *rx, double *ry,
                                       int *zero_boundary, int *reflective_boundary);
 /* tea_leaf_kernel_ocl.cpp:248-252 -- r = u0 - A u. */
@@ -159,6 +161,10 @@ int tl_generate_chunk_noalloc(tl_chunk *c, int nstates, const double *density, c
 int tl_set_field(tl_chunk *c);
 int tl_field_summary(tl_chunk *c, double *vol, double *mass, double *ie, double *temp);
 int tl_update_halo(tl_chunk *c, const int *chunk_neighbours, const int *fields, int depth);
+/* TL_ERR_ARG for an unknown coefficient, or for an interior density outside
+ * [2^-499, 2^499] or not a number: inside that range every divisor the
+ * iteration kernels form lies where their reciprocal-Newton division is the
+ * IEEE quotient (the reference would go on with infinite or NaN faces). */
 int tl_common_init(tl_chunk *c, int coefficient, double dt, double *rx, double *ry,
                    const int *zero_boundary, int reflective_boundary);
 int tl_calc_residual(tl_chunk *c);

@@ -30,6 +30,7 @@ struct tl_chunk {
     double *partials = nullptr;
     double *result = nullptr;
     tl::SolveState *st = nullptr;
+    int *range_bad = nullptr;        // k_conductivity's flag: a density outside the fast division's range
     double *hist_a = nullptr, *hist_b = nullptr, *hist_e = nullptr;
     int hist_len = 0;
     double *ch_a = nullptr, *ch_b = nullptr;

@@ -235,7 +235,7 @@ extern "C" int tl_chunk_destroy(tl_chunk *c)
         if (c->base2[i]) hipFree(c->base2[i]);
     for (int i = 0; i < NUM_FIELDS_1D; ++i)
         if (c->f1[i]) hipFree(c->f1[i]);
-    for (void *p : {(void *)c->partials, (void *)c->result, (void *)c->st,
+    for (void *p : {(void *)c->partials, (void *)c->result, (void *)c->st, (void *)c->range_bad,
                     (void *)c->hist_a, (void *)c->hist_b, (void *)c->hist_e, (void *)c->ch_a,
                     (void *)c->ch_b})
         if (p) hipFree(p);
@@ -313,6 +313,7 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
     if ((e = hipMalloc(&c->partials, sizeof(double) * MAX_RED * c->cap)) != hipSuccess ||
         (e = hipMalloc(&c->result, sizeof(double) * MAX_RED)) != hipSuccess ||
         (e = hipMalloc(&c->st, sizeof(SolveState))) != hipSuccess ||
+        (e = hipMalloc(&c->range_bad, sizeof(int))) != hipSuccess ||
         (e = hipHostMalloc(&c->h_result, sizeof(double) * MAX_RED, hipHostMallocCoherent)) != hipSuccess ||
         (e = hipHostMalloc(&c->h_flag, 64, hipHostMallocCoherent)) != hipSuccess ||
         (e = hipHostGetDevicePointer((void **)&c->d_result, c->h_result, 0)) != hipSuccess ||
@@ -625,8 +626,23 @@ extern "C" int tl_common_init(tl_chunk *c, int coefficient, double dt, double *r
     prof_end(c, "tea_leaf_init_common");
     // the same coefficients in recomputable form for the fused kernels
     c->factors_current = false;
-    launch_conductivity(c->s, c->g, c->f2[F_DENSITY], c->f2[F_COND], coefficient);
+    hipMemsetAsync(c->range_bad, 0, sizeof(int), c->s);
+    launch_conductivity(c->s, c->g, c->f2[F_DENSITY], c->f2[F_COND], coefficient, c->range_bad);
     TL_CHECK_LAUNCH(c, "conductivity");
+    // the iteration kernels' division (div_nr) is the IEEE quotient for the
+    // divisors that densities in [2^-499, 2^499] give; others are refused
+    // (the reference would go on with infinite or NaN coefficients)
+    {
+        int bad = 0;
+        hipError_t e = hipMemcpyAsync(&bad, c->range_bad, sizeof(int), hipMemcpyDeviceToHost, c->s);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+        if (e != hipSuccess) return hip_fail("tl_common_init", e);
+        if (bad) {
+            set_error("tl_common_init: a density outside [2^-499, 2^499] or not a number: the iteration kernels' "
+                      "division is exact only for the divisors such densities give");
+            return TL_ERR_ARG;
+        }
+    }
     Faces nf;
     nf.rx = *rx;
     nf.ry = *ry;

@@ -35,7 +35,7 @@ void launch_halo(hipStream_t s, const Geom &g, double *a, int depth, int face);
 void launch_init_common(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, double *Kx,
                         double *Ky, double rx, double ry, int coef);
 // conductivity (density or 1/density) over the whole padded index space
-void launch_conductivity(hipStream_t s, const Geom &g, const double *rho, double *w, int coef);
+void launch_conductivity(hipStream_t s, const Geom &g, const double *rho, double *w, int coef, int *bad);
 void launch_zero_boundary(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Kx, double *Ky,
                           const int *zb);
 void launch_init_u(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *rho, const double *en,

@@ -390,15 +390,24 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_init_common(Geom g, Ext e, co
 // conductivity of tea_leaf_init_common (tea_leaf_common_cl.cl:56-90: density,
 // or its reciprocal) over the whole padded index space, kept for the fused
 // iteration kernels, which recompute Kx, Ky from it
-__global__ void __launch_bounds__(256) k_conductivity(Geom g, const double *rho, double *w, int coef)
+// (`bad`: set when an interior density lies outside [2^-499, 2^499] or is not
+// a number. Inside that range every divisor the iteration kernels form from
+// the conductivities -- face denominators 2 w_a w_b, and Thomas pivots and
+// diagonals >= 1 -- has a binary exponent within [-999, 999], where div_nr is
+// the IEEE quotient.)
+__global__ void __launch_bounds__(256) k_conductivity(Geom g, const double *rho, double *w, int coef, int *bad)
 {
     const int nx = g.x_max + 2 * g.hd, ny = g.y_max + 2 * g.hd;
     const long n = (long)nx * ny;
+    bool out = false;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const int j = (int)(t % nx), k = (int)(t / nx);
         const double d = rho[IX(g, j, k)];
         w[IX(g, j, k)] = (coef == COEF_CONDUCTIVITY) ? d : 1.0 / d;
+        const bool interior = j >= g.hd && j < g.hd + g.x_max && k >= g.hd && k < g.hd + g.y_max;
+        out |= interior && !(d >= 0x1p-499 && d <= 0x1p499);
     }
+    if (out) *bad = 1;
 }
 
 // tea_leaf_zero_boundary (tea_leaf_common_cl.cl:92-125), launch bounds (-hd, hd)
@@ -1620,9 +1629,9 @@ void launch_init_common(hipStream_t s, const Geom &g, const Ext &e, int cap, con
     TL_LAUNCH(k_init_common, e, cap, s, g, e, rho, Kx, Ky, rx, ry, coef);
 }
 
-void launch_conductivity(hipStream_t s, const Geom &g, const double *rho, double *w, int coef)
+void launch_conductivity(hipStream_t s, const Geom &g, const double *rho, double *w, int coef, int *bad)
 {
-    hipLaunchKernelGGL(k_conductivity, dim3(1024), dim3(256), 0, s, g, rho, w, coef);
+    hipLaunchKernelGGL(k_conductivity, dim3(1024), dim3(256), 0, s, g, rho, w, coef, bad);
 }
 
 void launch_zero_boundary(hipStream_t s, const Geom &g, const Ext &e, int cap, double *Kx, double *Ky,
