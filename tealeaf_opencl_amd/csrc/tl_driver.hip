@@ -485,8 +485,9 @@ void launch_timed(tl_chunk *c, bool prof, const char *name, bool fin, F &&launch
 // update + matvec + p.w, then the fused u/r update + preconditioner + r.z.
 // With `ev`, ev[0..1] bracket the first iteration kernel and ev[2..3] the
 // second (their finishers are outside the brackets). parity 1 / 2: the
-// iteration is known to be odd / even (odd ones take the band-march update
-// kernel); 0: unknown.
+// iteration is known to be odd / even (the update kernel's form compiled for
+// that parity runs); 0: unknown (the form that reads the parity from the
+// state).
 void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false, int parity = 0)
 {
     const Ext in = ext_interior(c);
@@ -581,8 +582,8 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
 
 // 16 CG iterations per replay; with `timed`, 14 plus the eagerly launched
 // odd and even ones of each batch. Every batch starts on an odd iteration
-// (the CG phase starts at n = 1), so the kernels are captured for known
-// parities: odd iterations take the band-march update kernel.
+// (the CG phase starts at n = 1), so the update kernel is captured in its
+// odd and even forms alternately.
 int cg_graph(tl_chunk *c, bool timed)
 {
     const int B = timed ? 14 : 16;
