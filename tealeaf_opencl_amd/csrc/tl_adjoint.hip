@@ -155,8 +155,8 @@ struct Work {
     std::vector<double *> bases;
     ~Work()
     {
-        hipStreamSynchronize(c->s);
-        for (double *b : bases) hipFree(b);
+        (void)hipStreamSynchronize(c->s);
+        for (double *b : bases) (void)hipFree(b);
     }
     int alloc(double **out)
     {
@@ -309,7 +309,7 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
     for (auto &x : ev)
         if (hipEventCreate(&x) != hipSuccess) {
             for (auto &y : ev)
-                if (y) hipEventDestroy(y);
+                if (y) (void)hipEventDestroy(y);
             set_error("tl_adjoint_density: hipEventCreate failed");
             return TL_ERR_HIP;
         }
@@ -317,14 +317,14 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
         hipEvent_t *e;
         ~EvGuard()
         {
-            for (int i = 0; i < 3; ++i) hipEventDestroy(e[i]);
+            for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
         }
     } evg{ev};
 
     adj::Counts cnt;
-    hipEventRecord(ev[0], c->s);
+    TL_HIP_TRY(hipEventRecord(ev[0], c->s), "tl_adjoint_density");
     RC(adj::forward(c, coefficient, dt, nsteps, tol, max_iters, ck.data(), every, &cnt));
-    hipEventRecord(ev[1], c->s);
+    TL_HIP_TRY(hipEventRecord(ev[1], c->s), "tl_adjoint_density");
     double loss = 0.0;
     RC(adj::loss_and_seed(c, uo, gv, &loss));
 
@@ -343,7 +343,7 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
         // walk the segment backwards
         for (int n = end - 1; n >= s; --n) {
             adj::copy_interior(c, gv, c->f2[F_U0]);
-            hipMemsetAsync(c->base2[F_U], 0, sizeof(double) * c->elems2d, c->s);
+            TL_HIP_TRY(hipMemsetAsync(c->base2[F_U], 0, sizeof(double) * c->elems2d, c->s), "tl_adjoint_density");
             RC(adj::solve(c, tol, max_iters, &cnt.adjoint, &cnt, "adjoint", n + 1));
             hipLaunchKernelGGL(adj::k_adj_face_grad, dim3(adj::grid_of(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s,
                                c->g, e, c->f2[F_U], seg[n - s], c->f2[F_COND], c->faces, coefficient, gr);
@@ -352,15 +352,15 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
     }
     hipLaunchKernelGGL(adj::k_adj_initial, dim3(adj::grid_of(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s, c->g, e, gv,
                        e0, gr);
-    hipEventRecord(ev[2], c->s);
+    TL_HIP_TRY(hipEventRecord(ev[2], c->s), "tl_adjoint_density");
     hipError_t err = hipGetLastError();
     if (err != hipSuccess) return hip_fail("launch adjoint", err);
     RC(adj::download_padded(c, grad, gr));
     if (out) {
         memset(out, 0, sizeof *out);
         float a = 0.f, b = 0.f;
-        hipEventElapsedTime(&a, ev[0], ev[1]);
-        hipEventElapsedTime(&b, ev[1], ev[2]);
+        TL_HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]), "tl_adjoint_density");
+        TL_HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]), "tl_adjoint_density");
         out->loss = loss;
         out->forward_s = 1e-3 * a;
         out->adjoint_s = 1e-3 * b;

@@ -116,6 +116,15 @@ inline Ext ext_interior(const tl_chunk *c) { return ext_range(c, 0, 0, 0, 0); }
 RedArgs red_args(tl_chunk *c, int mode, bool with_state);
 void set_error(const char *fmt, ...);
 int hip_fail(const char *what, hipError_t e);
+// Runs a HIP call in a function that returns a TL_ status: on failure the
+// error is recorded and the function returns TL_ERR_HIP. Calls whose result
+// is deliberately dropped (teardown, and profiling events whose errors the
+// next launch check reports) are cast to void instead.
+#define TL_HIP_TRY(call, what)                                                   \
+    do {                                                                         \
+        const hipError_t e_try_ = (call);                                        \
+        if (e_try_ != hipSuccess) return ::tl::hip_fail(what, e_try_);           \
+    } while (0)
 // profiling wrapper bracket (no-ops unless profiler_on)
 void prof_begin(tl_chunk *c);
 void prof_end(tl_chunk *c, const char *name);

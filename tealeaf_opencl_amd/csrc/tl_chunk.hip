@@ -85,7 +85,7 @@ RedArgs red_args(tl_chunk *c, int mode, bool with_state)
 
 void prof_begin(tl_chunk *c)
 {
-    if (c->profiler_on) hipEventRecord(c->pe0, c->s);
+    if (c->profiler_on) (void)hipEventRecord(c->pe0, c->s);
 }
 
 void prof_add(tl_chunk *c, const char *name, double ms)
@@ -98,11 +98,11 @@ void prof_add(tl_chunk *c, const char *name, double ms)
 void prof_end_split(tl_chunk *c, const char *name, const char *second)
 {
     if (!c->profiler_on) return;
-    hipEventRecord(c->pe1, c->s);
-    hipEventSynchronize(c->pe1);
+    (void)hipEventRecord(c->pe1, c->s);
+    (void)hipEventSynchronize(c->pe1);
     float a = 0.f, b = 0.f;
-    hipEventElapsedTime(&a, c->pe0, c->pe_mid);
-    hipEventElapsedTime(&b, c->pe_mid, c->pe1);
+    (void)hipEventElapsedTime(&a, c->pe0, c->pe_mid);
+    (void)hipEventElapsedTime(&b, c->pe_mid, c->pe1);
     prof_add(c, name, a);
     prof_add(c, second, b);
 }
@@ -110,10 +110,10 @@ void prof_end_split(tl_chunk *c, const char *name, const char *second)
 void prof_end(tl_chunk *c, const char *name)
 {
     if (!c->profiler_on) return;
-    hipEventRecord(c->pe1, c->s);
-    hipEventSynchronize(c->pe1);
+    (void)hipEventRecord(c->pe1, c->s);
+    (void)hipEventSynchronize(c->pe1);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, c->pe0, c->pe1);
+    (void)hipEventElapsedTime(&ms, c->pe0, c->pe1);
     auto &e = c->prof[name];
     e.first += ms;
     e.second += 1;
@@ -124,8 +124,8 @@ int ensure_coef_capacity(tl_chunk *c, int n)
     if (n <= c->ch_cap) return TL_OK;
     const int cap = std::max(n, 64);
     hipError_t e;
-    if (c->ch_a) hipFree(c->ch_a);
-    if (c->ch_b) hipFree(c->ch_b);
+    if (c->ch_a) (void)hipFree(c->ch_a);
+    if (c->ch_b) (void)hipFree(c->ch_b);
     c->ch_a = c->ch_b = nullptr;
     if ((e = hipMalloc(&c->ch_a, sizeof(double) * cap)) != hipSuccess) return hip_fail("hipMalloc coefs", e);
     if ((e = hipMalloc(&c->ch_b, sizeof(double) * cap)) != hipSuccess) return hip_fail("hipMalloc coefs", e);
@@ -133,7 +133,7 @@ int ensure_coef_capacity(tl_chunk *c, int n)
     // captured batches hold the old coefficient pointers
     for (auto *gs : {&c->g_cheby, &c->g_ppcg})
         if (gs->exec) {
-            hipGraphExecDestroy(gs->exec);
+            (void)hipGraphExecDestroy(gs->exec);
             gs->exec = nullptr;
             gs->key = -1;
         }
@@ -146,7 +146,7 @@ void drop_graphs(tl_chunk *c)
                                     &c->rf[0].graph_ppcg, &c->rf[1].graph, &c->rf[1].graph_ppcg};
     for (auto *gs : slots)
         if (gs->exec) {
-            hipGraphExecDestroy(gs->exec);
+            (void)hipGraphExecDestroy(gs->exec);
             gs->exec = nullptr;
             gs->key = -1;
         }
@@ -158,15 +158,15 @@ int ensure_hist_capacity(tl_chunk *c, int n)
     const int cap = std::max(n, 1024);
     hipError_t e;
     for (double **p : {&c->hist_a, &c->hist_b, &c->hist_e}) {
-        if (*p) hipFree(*p);
+        if (*p) (void)hipFree(*p);
         *p = nullptr;
         if ((e = hipMalloc(p, sizeof(double) * cap)) != hipSuccess) return hip_fail("hipMalloc history", e);
-        hipMemsetAsync(*p, 0, sizeof(double) * cap, c->s);
+        TL_HIP_TRY(hipMemsetAsync(*p, 0, sizeof(double) * cap, c->s), "hipMemsetAsync history");
     }
     c->hist_len = cap;
     for (auto *gs : {&c->g_cg, &c->g_cheby, &c->g_ppcg, &c->g_jacobi})
         if (gs->exec) {
-            hipGraphExecDestroy(gs->exec);
+            (void)hipGraphExecDestroy(gs->exec);
             gs->exec = nullptr;
             gs->key = -1;
         }
@@ -227,27 +227,28 @@ extern "C" int tl_device_count(int *count)
 extern "C" int tl_chunk_destroy(tl_chunk *c)
 {
     if (!c) return TL_OK;
-    if (c->s) hipStreamSynchronize(c->s);
+    // teardown: every resource is released whatever the earlier calls return
+    if (c->s) (void)hipStreamSynchronize(c->s);
     for (auto *gs : {&c->g_cg, &c->g_cheby, &c->g_ppcg, &c->g_jacobi})
-        if (gs->exec) hipGraphExecDestroy(gs->exec);
+        if (gs->exec) (void)hipGraphExecDestroy(gs->exec);
     refine_release(c);
     for (int i = 0; i < NUM_FIELDS_2D; ++i)
-        if (c->base2[i]) hipFree(c->base2[i]);
+        if (c->base2[i]) (void)hipFree(c->base2[i]);
     for (int i = 0; i < NUM_FIELDS_1D; ++i)
-        if (c->f1[i]) hipFree(c->f1[i]);
+        if (c->f1[i]) (void)hipFree(c->f1[i]);
     for (void *p : {(void *)c->partials, (void *)c->result, (void *)c->st, (void *)c->range_bad,
                     (void *)c->hist_a, (void *)c->hist_b, (void *)c->hist_e, (void *)c->ch_a,
                     (void *)c->ch_b})
-        if (p) hipFree(p);
-    if (c->h_result) hipHostFree(c->h_result);
-    if (c->h_flag) hipHostFree(c->h_flag);
-    if (c->h_st) hipHostFree(c->h_st);
+        if (p) (void)hipFree(p);
+    if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
+    if (c->h_st) (void)hipHostFree(c->h_st);
     for (hipEvent_t ev : {c->pe0, c->pe1, c->pe_mid, c->ev_batch[0], c->ev_batch[1]})
-        if (ev) hipEventDestroy(ev);
+        if (ev) (void)hipEventDestroy(ev);
     for (auto &set : c->ev_t)
         for (hipEvent_t ev : set)
-            if (ev) hipEventDestroy(ev);
-    if (c->s) hipStreamDestroy(c->s);
+            if (ev) (void)hipEventDestroy(ev);
+    if (c->s) (void)hipStreamDestroy(c->s);
     if (g_default == c) g_default = nullptr;
     delete c;
     return TL_OK;
@@ -299,7 +300,10 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
             return hip_fail("hipMalloc field", e);
         }
         // every buffer starts at zero, halos included (ocl_buffers.cpp:3-19)
-        hipMemsetAsync(c->base2[i], 0, sizeof(double) * c->elems2d, c->s);
+        if ((e = hipMemsetAsync(c->base2[i], 0, sizeof(double) * c->elems2d, c->s)) != hipSuccess) {
+            tl_chunk_destroy(c);
+            return hip_fail("hipMemsetAsync field", e);
+        }
         c->f2[i] = c->base2[i] + c->off;
     }
     c->len1d = std::max(x_max, y_max) + 2 * halo_depth + 8;
@@ -308,7 +312,10 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
             tl_chunk_destroy(c);
             return hip_fail("hipMalloc 1d", e);
         }
-        hipMemsetAsync(c->f1[i], 0, sizeof(double) * c->len1d, c->s);
+        if ((e = hipMemsetAsync(c->f1[i], 0, sizeof(double) * c->len1d, c->s)) != hipSuccess) {
+            tl_chunk_destroy(c);
+            return hip_fail("hipMemsetAsync 1d", e);
+        }
     }
     if ((e = hipMalloc(&c->partials, sizeof(double) * MAX_RED * c->cap)) != hipSuccess ||
         (e = hipMalloc(&c->result, sizeof(double) * MAX_RED)) != hipSuccess ||
@@ -322,7 +329,10 @@ extern "C" int tl_chunk_create(int x_max, int y_max, int halo_depth, int precond
         tl_chunk_destroy(c);
         return hip_fail("hipMalloc scratch", e);
     }
-    hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s);
+    if ((e = hipMemsetAsync(c->st, 0, sizeof(SolveState), c->s)) != hipSuccess) {
+        tl_chunk_destroy(c);
+        return hip_fail("hipMemsetAsync state", e);
+    }
     *c->h_flag = 0u;
     for (hipEvent_t *ev : {&c->pe0, &c->pe1, &c->pe_mid})
         if ((e = hipEventCreate(ev)) != hipSuccess) {
@@ -475,15 +485,18 @@ extern "C" int tl_generate_chunk(tl_chunk *c, int nstates, const double *density
     hipError_t e;
     if ((e = hipMalloc(&dev, sizeof(double) * 7 * nstates)) != hipSuccess) return hip_fail("hipMalloc states", e);
     if ((e = hipMalloc(&dgeom, sizeof(int) * nstates)) != hipSuccess) {
-        hipFree(dev);
+        (void)hipFree(dev);
         return hip_fail("hipMalloc states", e);
     }
-    for (int a = 0; a < 7; ++a)
-        hipMemcpyAsync(dev + a * nstates, src[a], sizeof(double) * nstates, hipMemcpyHostToDevice, c->s);
-    hipMemcpyAsync(dgeom, geometry, sizeof(int) * nstates, hipMemcpyHostToDevice, c->s);
-    const int rc = generate_staged(c, nstates, dev, dgeom, g_rect, g_circ, g_point);
-    hipFree(dev);
-    hipFree(dgeom);
+    for (int a = 0; a < 7 && e == hipSuccess; ++a)
+        e = hipMemcpyAsync(dev + a * nstates, src[a], sizeof(double) * nstates, hipMemcpyHostToDevice, c->s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dgeom, geometry, sizeof(int) * nstates, hipMemcpyHostToDevice, c->s);
+    const int rc = e == hipSuccess ? generate_staged(c, nstates, dev, dgeom, g_rect, g_circ, g_point)
+                                   : hip_fail("generate_chunk: state copy", e);
+    // the copies and the kernel are done before the arrays are freed
+    // (hipFree waits for the device)
+    (void)hipFree(dev);
+    (void)hipFree(dgeom);
     return rc;
 }
 
@@ -520,7 +533,7 @@ int scratch_of(int dev, unsigned char **out)
         if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return hip_fail("hipSetDevice", e);
         void *p = nullptr;
         e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
-        if (cur != dev) hipSetDevice(cur);
+        if (cur != dev) (void)hipSetDevice(cur);
         if (e != hipSuccess) return hip_fail("hipGetSymbolAddress(scratch)", e);
         g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
     }
@@ -626,7 +639,7 @@ extern "C" int tl_common_init(tl_chunk *c, int coefficient, double dt, double *r
     prof_end(c, "tea_leaf_init_common");
     // the same coefficients in recomputable form for the fused kernels
     c->factors_current = false;
-    hipMemsetAsync(c->range_bad, 0, sizeof(int), c->s);
+    TL_HIP_TRY(hipMemsetAsync(c->range_bad, 0, sizeof(int), c->s), "tl_common_init");
     launch_conductivity(c->s, c->g, c->f2[F_DENSITY], c->f2[F_COND], coefficient, c->range_bad);
     TL_CHECK_LAUNCH(c, "conductivity");
     // the iteration kernels' division (div_nr) is the IEEE quotient for the
@@ -837,8 +850,10 @@ extern "C" int tl_cheby_init(tl_chunk *c, const double *ch_alphas, const double 
     }
     int rc = ensure_coef_capacity(c, n_coefs);
     if (rc != TL_OK) return rc;
-    hipMemcpyAsync(c->ch_a, ch_alphas, sizeof(double) * n_coefs, hipMemcpyHostToDevice, c->s);
-    hipMemcpyAsync(c->ch_b, ch_betas, sizeof(double) * n_coefs, hipMemcpyHostToDevice, c->s);
+    TL_HIP_TRY(hipMemcpyAsync(c->ch_a, ch_alphas, sizeof(double) * n_coefs, hipMemcpyHostToDevice, c->s),
+               "tl_cheby_init: coefficient copy");
+    TL_HIP_TRY(hipMemcpyAsync(c->ch_b, ch_betas, sizeof(double) * n_coefs, hipMemcpyHostToDevice, c->s),
+               "tl_cheby_init: coefficient copy");
     c->n_coefs = n_coefs;
     c->rx = rx;
     c->ry = ry;
@@ -888,8 +903,10 @@ extern "C" int tl_ppcg_init(tl_chunk *c, const double *ch_alphas, const double *
     }
     int rc = ensure_coef_capacity(c, n_inner_steps);
     if (rc != TL_OK) return rc;
-    hipMemcpyAsync(c->ch_a, ch_alphas, sizeof(double) * n_inner_steps, hipMemcpyHostToDevice, c->s);
-    hipMemcpyAsync(c->ch_b, ch_betas, sizeof(double) * n_inner_steps, hipMemcpyHostToDevice, c->s);
+    TL_HIP_TRY(hipMemcpyAsync(c->ch_a, ch_alphas, sizeof(double) * n_inner_steps, hipMemcpyHostToDevice, c->s),
+               "tl_ppcg_init: coefficient copy");
+    TL_HIP_TRY(hipMemcpyAsync(c->ch_b, ch_betas, sizeof(double) * n_inner_steps, hipMemcpyHostToDevice, c->s),
+               "tl_ppcg_init: coefficient copy");
     c->n_coefs = n_inner_steps;
     c->theta = theta;
     return tl_synchronize(c);
@@ -974,10 +991,8 @@ extern "C" int tl_jacobi_solve(tl_chunk *c, double *error)
     return read_result(c, error, 1);
 }
 
-// Attainable streaming bandwidth on the chunk's layout: `reps` launches of a
-// copy (kind 0: 8 B/cell read + 8 B/cell written) or of the 4-read/3-write
-// mix of the CG update kernel (kind 1: 56 B/cell) over the interior, on
-// temporary fields, timed with events on the chunk's stream.
+// The iteration kernels' division (div_nr) and the IEEE quotient of n operand
+// pairs, both computed on the device, copied back for comparison.
 extern "C" int tl_check_division(const double *num, const double *den, double *quot, double *fast, long n)
 {
     if (!num || !den || !quot || !fast || n < 1 || n > (1L << 26)) {
@@ -998,10 +1013,14 @@ extern "C" int tl_check_division(const double *num, const double *den, double *q
     if (e == hipSuccess) e = hipMemcpy(quot, d[2], bytes, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(fast, d[3], bytes, hipMemcpyDeviceToHost);
     for (double *p : d)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     return e == hipSuccess ? TL_OK : hip_fail("tl_check_division", e);
 }
 
+// Attainable streaming bandwidth on the chunk's layout: `reps` launches of a
+// copy (kind 0: 8 B/cell read + 8 B/cell written) or of the 4-read/3-write
+// mix of the CG update kernel (kind 1: 56 B/cell) over the interior, on
+// temporary fields, timed with events on the chunk's stream.
 extern "C" int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *bytes_per_s)
 {
     if (!c || (kind != 0 && kind != 1) || reps < 1 || !bytes_per_s) {
@@ -1027,21 +1046,22 @@ extern "C" int tl_stream_bandwidth(tl_chunk *c, int kind, int reps, double *byte
             else launch_stream_mix(c->s, c->g, in, c->cap, f[0], f[1], f[2], f[3], f[4]);
         };
         once();
-        hipEventRecord(t0, c->s);
-        for (int r = 0; r < reps; ++r) once();
-        hipEventRecord(t1, c->s);
-        e = hipEventSynchronize(t1);
+        e = hipEventRecord(t0, c->s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) once();
+        if (e == hipSuccess) e = hipEventRecord(t1, c->s);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
         const double cells = (double)c->g.x_max * c->g.y_max;
         *bytes_per_s = cells * (kind == 0 ? 16.0 : 56.0) * reps / (1e-3 * ms);
     }
+    if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) rc = hip_fail("tl_stream_bandwidth", e);
-    hipStreamSynchronize(c->s);
-    if (t0) hipEventDestroy(t0);
-    if (t1) hipEventDestroy(t1);
+    (void)hipStreamSynchronize(c->s);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
     for (double *b : base)
-        if (b) hipFree(b);
+        if (b) (void)hipFree(b);
     return rc;
 }
 

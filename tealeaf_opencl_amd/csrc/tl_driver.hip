@@ -83,7 +83,8 @@ double wtime() { return std::chrono::duration<double>(std::chrono::steady_clock:
 // host time of a device phase: the stream is drained first (profiler_on only)
 double phase_time(tl_chunk *c)
 {
-    if (g_fprof) hipStreamSynchronize(c->s);
+    // an error here is reported by the phase's own launch checks
+    if (g_fprof) (void)hipStreamSynchronize(c->s);
     return wtime();
 }
 
@@ -444,7 +445,7 @@ int capture(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<v
 {
     if (gs.exec && gs.key == key) return TL_OK;
     if (gs.exec) {
-        hipGraphExecDestroy(gs.exec);
+        (void)hipGraphExecDestroy(gs.exec);
         gs.exec = nullptr;
     }
     hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
@@ -454,7 +455,7 @@ int capture(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<v
     e = hipStreamEndCapture(c->s, &graph);
     if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
     e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
         gs.exec = nullptr;
         return hip_fail("hipGraphInstantiate", e);
@@ -492,12 +493,12 @@ void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false, int parity = 0
 {
     const Ext in = ext_interior(c);
     const double *src = (c->prec == PREC_NONE) ? c->f2[F_R] : c->f2[F_Z];
-    if (ev) hipEventRecord(ev[0], c->s);
+    if (ev) (void)hipEventRecord(ev[0], c->s);
     launch_timed(c, prof, "k_fused_cg_w", true, [&](hipEvent_t mid) {
         launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
                           red_args(c, RED_CG_PW, true), ev ? ev[1] : mid);
     });
-    if (ev) hipEventRecord(ev[2], c->s);
+    if (ev) (void)hipEventRecord(ev[2], c->s);
     launch_timed(c, prof, "k_fused_cg_ur", true, [&](hipEvent_t mid) {
         launch_fused_cg_ur(c->s, c->g, in, c->cap, c->prec, 0, c->f2[F_U], c->f2[F_P], c->f2[F_P2], c->f2[F_R],
                            c->f2[F_R], c->f2[F_Z], c->f2[F_SD], c->f2[F_COND], c->faces,
@@ -548,7 +549,7 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
         if ((e = hipEventSynchronize(c->ev_batch[slot])) != hipSuccess) return hip_fail("batch wait", e);
         const SolveState &h = c->h_st[slot];
         if (h.fault) {
-            hipStreamSynchronize(c->s);
+            (void)hipStreamSynchronize(c->s);
             set_error("an iteration kernel captured for one iteration parity ran on the other");
             return TL_ERR_SOLVER;
         }
@@ -569,7 +570,7 @@ int run_batches(tl_chunk *c, hipGraphExec_t exec, bool timed, long max_batches)
         }
         if (h.halt) break;
         if (guard > max_batches) {
-            hipStreamSynchronize(c->s);
+            (void)hipStreamSynchronize(c->s);
             set_error("solver state did not halt after %ld batches", guard);
             return TL_ERR_SOLVER;
         }
@@ -810,8 +811,10 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         if (sv.first) {
             std::vector<double> a(n > 1 ? n - 1 : 1), b(n > 1 ? n - 1 : 1);
             if (n > 1) {
-                hipMemcpy(a.data(), c->hist_a + 1, sizeof(double) * (n - 1), hipMemcpyDeviceToHost);
-                hipMemcpy(b.data(), c->hist_b + 1, sizeof(double) * (n - 1), hipMemcpyDeviceToHost);
+                TL_HIP_TRY(hipMemcpy(a.data(), c->hist_a + 1, sizeof(double) * (n - 1), hipMemcpyDeviceToHost),
+                           "CG history copy");
+                TL_HIP_TRY(hipMemcpy(b.data(), c->hist_b + 1, sizeof(double) * (n - 1), hipMemcpyDeviceToHost),
+                           "CG history copy");
             }
             const int info = calc_eigenvalues(a.data(), b.data(), n - 1, &sv.eigmin, &sv.eigmax);
             sv.first = 0;
@@ -848,7 +851,7 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
             h.est_itc = est_itc;
             h.theta = sv.theta;
             h.error = sqrt(h.error);
-            hipMemcpy(c->hist_e + n, &h.error, sizeof(double), hipMemcpyHostToDevice);
+            TL_HIP_TRY(hipMemcpy(c->hist_e + n, &h.error, sizeof(double), hipMemcpyHostToDevice), "history copy");
             h.halt = 0;
             if (fabs(h.error) < h.eps * h.initial_residual) {
                 h.done = h.halt = 1;
@@ -911,7 +914,8 @@ int solve_native(tl_chunk *c, const Config &cfg, Save &sv, double dt, int step, 
         fprintf(lg, "INITRES %d %.17e\n", step, h.initial_residual);
         std::vector<double> he(executed + 1, 0.0);
         if (executed > 0)
-            hipMemcpy(he.data() + 1, c->hist_e + 1, sizeof(double) * executed, hipMemcpyDeviceToHost);
+            TL_HIP_TRY(hipMemcpy(he.data() + 1, c->hist_e + 1, sizeof(double) * executed, hipMemcpyDeviceToHost),
+                       "residual history copy");
         for (int i = 1; i <= executed; ++i) fprintf(lg, "RESID %d %d %.17e\n", step, i, he[i]);
     }
     if (lg)
@@ -1020,7 +1024,11 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
     while (rc == TL_OK) {
         step++;
         if (warmup >= 0 && step == warmup + 1) {
-            hipDeviceSynchronize();
+            const hipError_t e = hipDeviceSynchronize();
+            if (e != hipSuccess) {
+                rc = hip_fail("hipDeviceSynchronize", e);
+                break;
+            }
             c->t_k1_ms = c->t_k2_ms = 0;
             c->t_samples = 0;
             stats = Stats();
@@ -1049,7 +1057,11 @@ int run(const Config &cfg, FILE *lg, int mode, int max_steps, int warmup, RunOut
             emit_summary(lg, c, step, time, false, cfg, nullptr);
         if (done) {
             if (warmup >= 0 && out) {
-                hipDeviceSynchronize();
+                const hipError_t e = hipDeviceSynchronize();
+                if (e != hipSuccess) {
+                    rc = hip_fail("hipDeviceSynchronize", e);
+                    break;
+                }
                 out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 out->stats = stats;
             }

@@ -1518,7 +1518,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, M
 }
 
 // ===========================================================================
-// host launchers
+// host launchers. They only enqueue: a failed launch or event record is
+// reported by the caller's hipGetLastError check after the launcher returns.
 // ===========================================================================
 static inline int grid_for(const Ext &e, int cap)
 {
@@ -1576,7 +1577,7 @@ int march_grid(const March &m, int cap)
     do {                                                                                           \
         const int grid_ = grid_for(e, cap);                                                        \
         hipLaunchKernelGGL(kern, dim3(grid_), dim3(BLOCK_THREADS), 0, stream, __VA_ARGS__);         \
-        if (ev) hipEventRecord(ev, stream);                                                        \
+        if (ev) (void)hipEventRecord(ev, stream);                                                  \
         launch_finish(stream, ra, grid_, nq);                                                      \
     } while (0)
 
@@ -1818,7 +1819,7 @@ void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, cons
     const March m = make_march(e, MARCH_COLS);
     const int grid = march_grid(m, cap);
     hipLaunchKernelGGL(k_fused_cg_w, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, src, pA, pB, W, fc, ra);
-    if (ev_main) hipEventRecord(ev_main, s);
+    if (ev_main) (void)hipEventRecord(ev_main, s);
     launch_finish(s, ra, grid, 1);
 }
 
@@ -1944,7 +1945,7 @@ void launch_fused_cheby2(hipStream_t s, const Geom &g, const Ext &e, int cap, in
     else if (prec == PREC_DIAG) TL_FC2(PREC_DIAG);
     else TL_FC2(PREC_NONE);
 #undef TL_FC2
-    if (ev_main) hipEventRecord(ev_main, s);
+    if (ev_main) (void)hipEventRecord(ev_main, s);
     launch_finish(s, ra, grid, 1);
 }
 
@@ -1962,7 +1963,7 @@ void launch_fused_cheby(hipStream_t s, const Geom &g, const Ext &e, int cap, int
     else if (prec == PREC_DIAG) TL_FC1(PREC_DIAG);
     else TL_FC1(PREC_NONE);
 #undef TL_FC1
-    if (ev_main) hipEventRecord(ev_main, s);
+    if (ev_main) (void)hipEventRecord(ev_main, s);
     launch_finish(s, ra, grid, 1);
 }
 

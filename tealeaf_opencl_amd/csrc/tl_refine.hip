@@ -710,8 +710,9 @@ void launch_update_f(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, d
     else TL_RFU(PREC_NONE);
 #undef TL_RFU
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra, grid);
-    // d = 0 (never inside a captured graph: updates are launched eagerly)
-    hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
+    // d = 0 (never inside a captured graph: updates are launched eagerly; a
+    // failure is returned by the callers' hipGetLastError check)
+    (void)hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
 }
 
 // with the faces of the solve in progress (w.faces)
@@ -735,12 +736,12 @@ void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
         ra_u{c->partials, w.st, RF_RRN_LAG, w.hist, w.hist + RF_HIST, RF_HIST};
-    if (ev) hipEventRecord(ev[0], c->s);
+    if (ev) (void)hipEventRecord(ev[0], c->s);
     hipLaunchKernelGGL((k_rf_w<T, FACES>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
                        f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
-    if (ev) hipEventRecord(ev[1], c->s);
+    if (ev) (void)hipEventRecord(ev[1], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid_m);
-    if (ev) hipEventRecord(ev[2], c->s);
+    if (ev) (void)hipEventRecord(ev[2], c->s);
 #define TL_RFI(P)                                                                                       \
     hipLaunchKernelGGL((k_rf_ur<T, P, FACES>), dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], \
                        f[RW_P2], f[RW_R], f[RW_Z], f[RW_W], f[RW_KY], c->faces, ra_u)
@@ -748,7 +749,7 @@ void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
     else if (c->prec == PREC_DIAG) TL_RFI(PREC_DIAG);
     else TL_RFI(PREC_NONE);
 #undef TL_RFI
-    if (ev) hipEventRecord(ev[3], c->s);
+    if (ev) (void)hipEventRecord(ev[3], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
 }
 
@@ -764,12 +765,12 @@ constexpr int RF_BATCH = 16;
 void release_work(tl_chunk::RefineWork &w)
 {
     for (auto *gs : {&w.graph, &w.graph_ppcg})
-        if (gs->exec) hipGraphExecDestroy(gs->exec);
+        if (gs->exec) (void)hipGraphExecDestroy(gs->exec);
     for (void *p : w.base)
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     for (void *p : {(void *)w.st, (void *)w.hist, (void *)w.coef})
-        if (p) hipFree(p);
-    if (w.h_st) hipHostFree(w.h_st);
+        if (p) (void)hipFree(p);
+    if (w.h_st) (void)hipHostFree(w.h_st);
     w = tl_chunk::RefineWork();
 }
 
@@ -790,7 +791,7 @@ int rf_alloc_impl(tl_chunk *c, int which)
         w.elems = rows * w.g.pitch + colpad + 64;
         for (int i = 0; i < RW_N; ++i) {
             if ((e = hipMalloc(&w.base[i], sizeof(float) * w.elems)) != hipSuccess) return hip_fail("hipMalloc refine", e);
-            hipMemsetAsync(w.base[i], 0, sizeof(float) * w.elems, c->s);
+            TL_HIP_TRY(hipMemsetAsync(w.base[i], 0, sizeof(float) * w.elems, c->s), "refine alloc");
             w.f[i] = static_cast<float *>(w.base[i]) + w.off;
         }
     } else {
@@ -798,7 +799,7 @@ int rf_alloc_impl(tl_chunk *c, int which)
         w.off = c->off;
         w.elems = c->elems2d;
         if ((e = hipMalloc(&w.base[RW_D], sizeof(double) * w.elems)) != hipSuccess) return hip_fail("hipMalloc refine", e);
-        hipMemsetAsync(w.base[RW_D], 0, sizeof(double) * w.elems, c->s);
+        TL_HIP_TRY(hipMemsetAsync(w.base[RW_D], 0, sizeof(double) * w.elems, c->s), "refine alloc");
         w.f[RW_D] = static_cast<double *>(w.base[RW_D]) + w.off;
         w.f[RW_W] = c->f2[F_COND];
         w.f[RW_R] = c->f2[F_R];
@@ -812,8 +813,8 @@ int rf_alloc_impl(tl_chunk *c, int which)
     if ((e = hipMalloc(&w.hist, sizeof(double) * 2 * RF_HIST)) != hipSuccess ||
         (e = hipMalloc(&w.coef, sizeof(double) * 2 * RF_MAX_INNER)) != hipSuccess)
         return hip_fail("hipMalloc refine history", e);
-    hipMemsetAsync(w.hist, 0, sizeof(double) * 2 * RF_HIST, c->s);
-    hipMemsetAsync(w.coef, 0, sizeof(double) * 2 * RF_MAX_INNER, c->s);
+    TL_HIP_TRY(hipMemsetAsync(w.hist, 0, sizeof(double) * 2 * RF_HIST, c->s), "refine alloc");
+    TL_HIP_TRY(hipMemsetAsync(w.coef, 0, sizeof(double) * 2 * RF_MAX_INNER, c->s), "refine alloc");
     if ((e = hipMalloc(&w.st, sizeof(RefineState))) != hipSuccess) return hip_fail("hipMalloc refine state", e);
     if ((e = hipHostMalloc(&w.h_st, sizeof(RefineState) * 2, hipHostMallocDefault)) != hipSuccess)
         return hip_fail("hipHostMalloc refine state", e);
@@ -908,7 +909,7 @@ int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::funct
 {
     if (gs.exec && gs.key == key) return TL_OK;
     if (gs.exec) {
-        hipGraphExecDestroy(gs.exec);
+        (void)hipGraphExecDestroy(gs.exec);
         gs.exec = nullptr;
     }
     hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
@@ -918,7 +919,7 @@ int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::funct
     e = hipStreamEndCapture(c->s, &graph);
     if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
     e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (e != hipSuccess) {
         gs.exec = nullptr;
         return hip_fail("hipGraphInstantiate", e);
@@ -992,7 +993,7 @@ int rf_batches(tl_chunk *c, tl_chunk::RefineWork &w, hipGraphExec_t exec, bool t
             kt->samples += 2;
         }
         if (guard > max_batches) {
-            hipStreamSynchronize(c->s);
+            (void)hipStreamSynchronize(c->s);
             set_error("refined solve did not halt after %ld batches", guard);
             return TL_ERR_SOLVER;
         }
@@ -1037,12 +1038,16 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
     hipError_t e;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if ((e = hipEventCreate(&t0)) != hipSuccess || (e = hipEventCreate(&t1)) != hipSuccess) {
-        if (t0) hipEventDestroy(t0);
+        if (t0) (void)hipEventDestroy(t0);
         return hip_fail("hipEventCreate", e);
     }
     struct EvGuard {
         hipEvent_t a, b;
-        ~EvGuard() { hipEventDestroy(a); hipEventDestroy(b); }
+        ~EvGuard()
+        {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
     } guard{t0, t1};
     w.faces = inner > 0 ? ppcg_faces<T>() : cg_faces<T>();
     if ((rc = rf_capture<T>(c, w)) != TL_OK) return rc;
@@ -1056,9 +1061,9 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
     h.stop_n = inner > 0 ? presteps : 0;
     if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("refine start", e);
     w.h_st[0] = h;
-    hipEventRecord(t0, c->s);
-    hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
-    hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s);
+    TL_HIP_TRY(hipEventRecord(t0, c->s), "refine start");
+    TL_HIP_TRY(hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s), "refine start");
+    TL_HIP_TRY(hipMemsetAsync(w.base[RW_D], 0, (size_t)w.bytes * w.elems, c->s), "refine start");
     if (w.faces == RF_FACES_HALF)
         hipLaunchKernelGGL(k_rf_narrow_half, dim3(1024), dim3(256), 0, c->s, c->g, w.g, c->f2[F_COND],
                            static_cast<float *>(w.f[RW_W]));
@@ -1080,8 +1085,9 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
         // eigenvalue bounds from the CG coefficients, Chebyshev coefficients
         const int ns = h.n < RF_HIST ? h.n : RF_HIST;
         std::vector<double> a(ns), b(ns);
-        hipMemcpy(a.data(), w.hist, sizeof(double) * ns, hipMemcpyDeviceToHost);
-        hipMemcpy(b.data(), w.hist + RF_HIST, sizeof(double) * ns, hipMemcpyDeviceToHost);
+        TL_HIP_TRY(hipMemcpy(a.data(), w.hist, sizeof(double) * ns, hipMemcpyDeviceToHost), "refine history copy");
+        TL_HIP_TRY(hipMemcpy(b.data(), w.hist + RF_HIST, sizeof(double) * ns, hipMemcpyDeviceToHost),
+                   "refine history copy");
         double eigmin = 0, eigmax = 0, theta = 0;
         if (calc_eigenvalues(a.data(), b.data(), ns, &eigmin, &eigmax) != 0) {
             set_error("refined PPCG: eigenvalue estimate from %d CG steps failed", ns);
@@ -1091,8 +1097,9 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
         eigmax *= 1.05;
         std::vector<double> ca, cb;
         calc_ch_coefs(ca, cb, eigmin, eigmax, &theta, inner);
-        hipMemcpy(w.coef, ca.data(), sizeof(double) * inner, hipMemcpyHostToDevice);
-        hipMemcpy(w.coef + RF_MAX_INNER, cb.data(), sizeof(double) * inner, hipMemcpyHostToDevice);
+        TL_HIP_TRY(hipMemcpy(w.coef, ca.data(), sizeof(double) * inner, hipMemcpyHostToDevice), "refine coefficients");
+        TL_HIP_TRY(hipMemcpy(w.coef + RF_MAX_INNER, cb.data(), sizeof(double) * inner, hipMemcpyHostToDevice),
+                   "refine coefficients");
         // the outer steps update d at every step: complete the CG steps' d first
         {
             T *const *f = reinterpret_cast<T *const *>(w.f);
@@ -1104,16 +1111,18 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
         h.stop_n = 0;
         h.halt = 0;
         w.h_st[0] = h;
-        hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s);
+        TL_HIP_TRY(hipMemcpyAsync(w.st, &w.h_st[0], sizeof(RefineState), hipMemcpyHostToDevice, c->s),
+                   "refine state copy");
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail("launch refine d fix", e);
         outer0 = h.n;
         const long mb = (max_iters - h.n) / ppcg_batch(inner) + 4;
         if ((rc = rf_phase<T>(c, w, w.graph_ppcg.exec, false, mb, &kt, ubuf, &cur, &h)) != TL_OK) return rc;
     }
     if (cur == 1) launch_copy(c->s, c->g, ext_interior(c), c->cap, c->f2[F_U2], c->f2[F_U]);
-    hipEventRecord(t1, c->s);
+    TL_HIP_TRY(hipEventRecord(t1, c->s), "refine finish");
     if ((e = hipStreamSynchronize(c->s)) != hipSuccess) return hip_fail("refine finish", e);
     float ms = 0.f;
-    hipEventElapsedTime(&ms, t0, t1);
+    TL_HIP_TRY(hipEventElapsedTime(&ms, t0, t1), "refine finish");
     if (out) {
         memset(out, 0, sizeof *out);
         out->seconds = 1e-3 * ms;
