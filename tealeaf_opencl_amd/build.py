@@ -18,7 +18,8 @@ HEADERS = ["tl_common.h", "tl_device.h", "tl_kernels.h", "tl_chunk.h", "tl_deck.
 
 # packed FP32 instructions are disabled on every device compile
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Werror=shadow",
+# an ignored hipError_t (a nodiscard value) fails the build
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Werror=shadow", "-Werror=unused-value",
           "-I", os.path.join(HERE, "..", "include")]
 
 
