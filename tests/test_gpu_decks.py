@@ -241,7 +241,7 @@ def test_config2_full_size_vs_oracle(hiplib, tmp_path, name, slack):
     on the device-resident driver against the FP64 CPU oracle's field
     summaries after every timestep (fixtures written by
     tests/golden/make_full_size.py: CG and Chebyshev over 20 timesteps, PPCG
-    over 2), to the 1e-11 bar, with the iteration counts."""
+    over 6), to the 1e-11 bar, with the iteration counts."""
     import json
     path = os.path.join(os.path.dirname(__file__), "golden", "config2_4096_%s.json" % name)
     if not os.path.exists(path):
