@@ -137,12 +137,6 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_adj_initial(Geom g, Ext e, co
     TL_TILE_LOOP_END
 }
 
-inline int grid_of(const Ext &e, int cap)
-{
-    const int tiles = e.ntx * e.nty;
-    return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
-}
-
 #define RC(x)                          \
     do {                               \
         int rc_ = (x);                 \
@@ -237,7 +231,7 @@ int forward(tl_chunk *c, int coef, double dt, int nsteps, double tol, int max_it
 int loss_and_seed(tl_chunk *c, const double *u_obs, double *gvec, double *loss)
 {
     const Ext e = ext_interior(c);
-    const int grid = grid_of(e, c->cap);
+    const int grid = grid_for(e, c->cap);
     hipLaunchKernelGGL(k_adj_seed, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, c->g, e, c->f2[F_U], u_obs, gvec,
                        c->partials);
     hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(FINISH_THREADS), 0, c->s, c->partials, grid,
@@ -345,12 +339,12 @@ extern "C" int tl_adjoint_density(tl_chunk *c, int coefficient, double dt, int n
             adj::copy_interior(c, gv, c->f2[F_U0]);
             TL_HIP_TRY(hipMemsetAsync(c->base2[F_U], 0, sizeof(double) * c->elems2d, c->s), "tl_adjoint_density");
             RC(adj::solve(c, tol, max_iters, &cnt.adjoint, &cnt, "adjoint", n + 1));
-            hipLaunchKernelGGL(adj::k_adj_face_grad, dim3(adj::grid_of(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s,
+            hipLaunchKernelGGL(adj::k_adj_face_grad, dim3(grid_for(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s,
                                c->g, e, c->f2[F_U], seg[n - s], c->f2[F_COND], c->faces, coefficient, gr);
             adj::copy_interior(c, c->f2[F_U], gv);
         }
     }
-    hipLaunchKernelGGL(adj::k_adj_initial, dim3(adj::grid_of(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s, c->g, e, gv,
+    hipLaunchKernelGGL(adj::k_adj_initial, dim3(grid_for(e, c->cap)), dim3(BLOCK_THREADS), 0, c->s, c->g, e, gv,
                        e0, gr);
     TL_HIP_TRY(hipEventRecord(ev[2], c->s), "tl_adjoint_density");
     hipError_t err = hipGetLastError();

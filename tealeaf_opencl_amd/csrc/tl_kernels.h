@@ -10,6 +10,14 @@
 
 namespace tl {
 
+// blocks of a tile launch (TL_TILE_LOOP): one per 64x16 tile, at most `cap`
+// (blocks then loop over several tiles); INT_MAX for one block per tile
+inline int grid_for(const Ext &e, int cap)
+{
+    const int tiles = e.ntx * e.nty;
+    return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
+}
+
 void launch_init_chunk_1d(hipStream_t s, const Geom &g, double xmin, double ymin, double dx, double dy,
                           double *const *f1);
 void launch_init_chunk_2d(hipStream_t s, const Geom &g, const Ext &e, int cap, double dx, double dy,

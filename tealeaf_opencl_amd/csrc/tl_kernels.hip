@@ -1521,12 +1521,6 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cheby2(Geom g, Ext e, M
 // host launchers. They only enqueue: a failed launch or event record is
 // reported by the caller's hipGetLastError check after the launcher returns.
 // ===========================================================================
-static inline int grid_for(const Ext &e, int cap)
-{
-    const int tiles = e.ntx * e.nty;
-    return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
-}
-
 March make_march(const Ext &e, int cols)
 {
     March m;

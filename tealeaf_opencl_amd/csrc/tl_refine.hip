@@ -686,17 +686,11 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_inner2(Geom g, Ext e, T *d
 constexpr int RF_HIST = 1024;
 constexpr int RF_MAX_INNER = 256;
 
-inline int grid_of(const Ext &e, int cap)
-{
-    const int tiles = e.ntx * e.nty;
-    return tiles < cap ? (tiles > 0 ? tiles : 1) : cap;
-}
-
 template <typename T, int FACES>
 void launch_update_f(tl_chunk *c, tl_chunk::RefineWork &w, const double *u_in, double *u_out)
 {
     const Ext e = ext_interior(c);
-    const int grid = grid_of(e, c->cap);
+    const int grid = grid_for(e, c->cap);
     const RfArgs ra{c->partials, w.st, RF_UPDATE, nullptr, nullptr, 0};
     T *const *f = reinterpret_cast<T *const *>(w.f);
     hipLaunchKernelGGL(k_rf_d_fix<T>, dim3(grid), dim3(BLOCK_THREADS), 0, c->s, w.g, e, f[RW_D], f[RW_P], f[RW_P2],
@@ -729,7 +723,7 @@ template <typename T, int FACES>
 void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
 {
     const Ext e = ext_interior(c);
-    const int grid = grid_of(e, c->cap);
+    const int grid = grid_for(e, c->cap);
     const March m = make_march(e, MARCH_COLS);
     const int grid_m = march_grid(m, c->cap);
     T *const *f = reinterpret_cast<T *const *>(w.f);
@@ -847,10 +841,10 @@ template <typename T>
 void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
 {
     const Ext e = ext_interior(c);
-    const int grid = grid_of(e, c->cap);
+    const int grid = grid_for(e, c->cap);
     // one block per tile for the kernels that write no partials (2.6 % faster
     // FP64 refined PPCG at 4096^2 than blocks looping over two tiles each)
-    const int grid_nr = grid_of(e, INT_MAX);
+    const int grid_nr = grid_for(e, INT_MAX);
     const March m = make_march(e, MARCH_COLS);
     const int grid_m = march_grid(m, c->cap);
     T *const *f = reinterpret_cast<T *const *>(w.f);
@@ -1103,7 +1097,7 @@ int rf_solve(tl_chunk *c, int which, double tol, int max_iters, double delta, in
         // the outer steps update d at every step: complete the CG steps' d first
         {
             T *const *f = reinterpret_cast<T *const *>(w.f);
-            hipLaunchKernelGGL(k_rf_d_fix<T>, dim3(grid_of(ext_interior(c), c->cap)), dim3(BLOCK_THREADS), 0, c->s,
+            hipLaunchKernelGGL(k_rf_d_fix<T>, dim3(grid_for(ext_interior(c), c->cap)), dim3(BLOCK_THREADS), 0, c->s,
                                w.g, ext_interior(c), f[RW_D], f[RW_P], f[RW_P2], w.st);
         }
         h.d_lag = 0;
