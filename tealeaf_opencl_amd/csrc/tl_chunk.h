@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <string>
 #include <utility>
@@ -144,5 +145,8 @@ void refine_release(tl_chunk *c);
 // destroys every captured iteration batch (native and refined): they hold the
 // face parameters (rx, ry, walls) and buffer addresses by value
 void drop_graphs(tl_chunk *c);
+// captures the launches `body` enqueues on the chunk's stream into `gs`,
+// unless gs already holds a batch captured under the same key
+int capture_graph(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body);
 
 }  // namespace tl

@@ -152,6 +152,29 @@ void drop_graphs(tl_chunk *c)
         }
 }
 
+int capture_graph(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body)
+{
+    if (gs.exec && gs.key == key) return TL_OK;
+    if (gs.exec) {
+        (void)hipGraphExecDestroy(gs.exec);
+        gs.exec = nullptr;
+    }
+    hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
+    body();
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(c->s, &graph);
+    if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
+    e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        gs.exec = nullptr;
+        return hip_fail("hipGraphInstantiate", e);
+    }
+    gs.key = key;
+    return TL_OK;
+}
+
 int ensure_hist_capacity(tl_chunk *c, int n)
 {
     if (n <= c->hist_len) return TL_OK;

@@ -441,29 +441,6 @@ int state_upload(tl_chunk *c, const SolveState &h)
     return TL_OK;
 }
 
-int capture(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body)
-{
-    if (gs.exec && gs.key == key) return TL_OK;
-    if (gs.exec) {
-        (void)hipGraphExecDestroy(gs.exec);
-        gs.exec = nullptr;
-    }
-    hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
-    body();
-    hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(c->s, &graph);
-    if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
-    e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-        gs.exec = nullptr;
-        return hip_fail("hipGraphInstantiate", e);
-    }
-    gs.key = key;
-    return TL_OK;
-}
-
 // One launch of the device-resident driver. With prof (profiler_on: the
 // batches are then launched eagerly instead of replayed) it is bracketed by
 // events and its device time goes to the chunk's per-kernel table under the
@@ -589,7 +566,7 @@ int cg_graph(tl_chunk *c, bool timed)
 {
     const int B = timed ? 14 : 16;
     const int key = c->prec * 10 + (timed ? 1 : 0);
-    return capture(c, c->g_cg, key, [&]() {
+    return capture_graph(c, c->g_cg, key, [&]() {
         for (int b = 0; b < B; ++b) cg_iteration(c, nullptr, false, (b & 1) ? 2 : 1);
     });
 }
@@ -622,7 +599,7 @@ void cheby_launch(tl_chunk *c, bool pairs, long k, bool prof)
 int cheby_graph(tl_chunk *c, bool pairs)
 {
     const int B = 16;
-    return capture(c, c->g_cheby, c->prec * 4 + (pairs ? 2 : 1), [&]() {
+    return capture_graph(c, c->g_cheby, c->prec * 4 + (pairs ? 2 : 1), [&]() {
         for (int b = 0; b < (pairs ? B / 2 : B); ++b) cheby_launch(c, pairs, b, false);
     });
 }
@@ -678,7 +655,7 @@ int ppcg_graph(tl_chunk *c, int inner)
     const int B = std::max(1, 32 / (inner + 2));
     const int per = c->ppcg_steps_per_launch;
     const int last = c->ppcg_fresh_z ? 2 : 1;
-    return capture(c, c->g_ppcg, ((c->prec * 100000 + inner) * 4 + per) * 2 + (last - 1), [&]() {
+    return capture_graph(c, c->g_ppcg, ((c->prec * 100000 + inner) * 4 + per) * 2 + (last - 1), [&]() {
         for (int b = 0; b < B; ++b) ppcg_outer(c, inner, false);
     });
 }
@@ -731,7 +708,7 @@ void jacobi_launch(tl_chunk *c, bool prof)
 int jacobi_graph(tl_chunk *c)
 {
     const int B = 16;
-    return capture(c, c->g_jacobi, 1, [&]() {
+    return capture_graph(c, c->g_jacobi, 1, [&]() {
         for (int b = 0; b < B; ++b) jacobi_launch(c, false);
     });
 }

@@ -899,35 +899,12 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_u, grid);
 }
 
-int capture_into(tl_chunk *c, tl_chunk::GraphSlot &gs, int key, const std::function<void()> &body)
-{
-    if (gs.exec && gs.key == key) return TL_OK;
-    if (gs.exec) {
-        (void)hipGraphExecDestroy(gs.exec);
-        gs.exec = nullptr;
-    }
-    hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail("hipStreamBeginCapture", e);
-    body();
-    hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(c->s, &graph);
-    if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
-    e = hipGraphInstantiate(&gs.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-        gs.exec = nullptr;
-        return hip_fail("hipGraphInstantiate", e);
-    }
-    gs.key = key;
-    return TL_OK;
-}
-
 // CG iterations: the first two iterations of each batch are launched eagerly
 // (timed), the other RF_BATCH - 2 replayed
 template <typename T>
 int rf_capture(tl_chunk *c, tl_chunk::RefineWork &w)
 {
-    return capture_into(c, w.graph, c->prec * 4 + w.faces, [&]() {
+    return capture_graph(c, w.graph, c->prec * 4 + w.faces, [&]() {
         for (int b = 0; b < RF_BATCH - 2; ++b) launch_iteration<T>(c, w);
     });
 }
@@ -937,7 +914,7 @@ int ppcg_batch(int inner) { return inner + 2 >= 32 ? 1 : 32 / (inner + 2); }
 template <typename T>
 int rf_capture_ppcg(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
 {
-    return capture_into(c, w.graph_ppcg, (c->prec * 1000 + inner) * 4 + c->ppcg_steps_per_launch, [&]() {
+    return capture_graph(c, w.graph_ppcg, (c->prec * 1000 + inner) * 4 + c->ppcg_steps_per_launch, [&]() {
         for (int b = 0; b < ppcg_batch(inner); ++b) launch_ppcg_outer<T>(c, w, inner);
     });
 }
