@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 # solver name -> (deck keyword, timesteps)
-RUNS = {"cg": ("tl_use_cg", 20), "chebyshev": ("tl_use_chebyshev", 20), "ppcg": ("tl_use_ppcg", 2),
+RUNS = {"cg": ("tl_use_cg", 20), "chebyshev": ("tl_use_chebyshev", 20), "ppcg": ("tl_use_ppcg", 6),
         "config3": ("tl_use_cg", 1)}
 N = 4096
 
