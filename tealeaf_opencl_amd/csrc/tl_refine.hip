@@ -294,6 +294,16 @@ __device__ __forceinline__ double rf_march_ky(const Geom &g, const Faces &f, boo
     return march_ky(g, f, inner, j, k, wd, wc);
 }
 
+// z in k_rf_w: nontemporal loads for FP64 fields (as k_fused_cg_w: the
+// caches then keep more of p_old and the faces; 0.104 -> 0.098 ms per launch
+// at 4096^2), plain loads for FP32 fields (the hint measured 2 % slower there)
+template <typename T>
+__device__ __forceinline__ T ld_z(const T *p)
+{
+    if (sizeof(T) == 8) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
 // p = z + beta p_old (rounded to the storage), w = A p, p.w. p is
 // double-buffered by the parity of the iteration counter. A band march
 // (TL_MARCH_LOOP) as k_fused_cg_w: one load per row of z, p_old and the face
@@ -319,10 +329,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
     const bool out_lane = lane >= 1 && lane <= MARCH_COLS && j <= e.j1;
     const bool inner_x = jb - 1 > hd && jb + MARCH_COLS < hd + g.x_max;
     int i = IX(g, j, k_begin - 1);
-    double pa = pn((double)src[i], (double)p_old[i]);
+    double pa = pn((double)ld_z(src + i), (double)p_old[i]);
     const double ca = (double)W[i];
     i += P;
-    double pb = pn((double)src[i], (double)p_old[i]);
+    double pb = pn((double)ld_z(src + i), (double)p_old[i]);
     double cb = (double)W[i];  // K32: Kx of row k; else the per-cell face field of row k
     double kyb = K32 ? (double)Wy[i] : rf_march_ky<FACES>(g, fc, false, j, k_begin, ca, cb);
     constexpr int PF = 2;
@@ -330,7 +340,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
 #pragma unroll
     for (int q = 0; q < PF; ++q) {
         const int ii = i + (q + 1) * P;
-        zr[q] = (double)src[ii];
+        zr[q] = (double)ld_z(src + ii);
         pr[q] = (double)p_old[ii];
         cr[q] = (double)W[ii];
         yr[q] = K32 ? (double)Wy[ii] : 0.0;
@@ -343,7 +353,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
             const double cc = cr[q], yc = yr[q];
             {
                 const int ii = IX(g, j, kk + 1 + PF);
-                zr[q] = (double)src[ii];
+                zr[q] = (double)ld_z(src + ii);
                 pr[q] = (double)p_old[ii];
                 cr[q] = (double)W[ii];
                 yr[q] = K32 ? (double)Wy[ii] : 0.0;
