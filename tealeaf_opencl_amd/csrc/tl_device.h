@@ -514,6 +514,14 @@ struct March {
     int segs_per;  // segments per slab
 };
 
+// a load with the nontemporal hint when NT is set
+template <bool NT>
+__device__ __forceinline__ double ld_nt(const double *p)
+{
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
 // lane l gets lane l-1's v (lane 0 keeps `old`)
 __device__ __forceinline__ double dpp_shr1(double v, double old)
 {

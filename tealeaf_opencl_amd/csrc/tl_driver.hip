@@ -619,7 +619,7 @@ void ppcg_outer(tl_chunk *c, int inner, bool prof)
     const int last = c->ppcg_fresh_z ? 2 : 1;  // the last step's r.z: fresh or stale z
     launch_timed(c, prof, "k_fused_cg_w", true, [&](hipEvent_t mid) {
         launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
-                          red_args(c, RED_PPCG_PW, true), mid);
+                          red_args(c, RED_PPCG_PW, true), mid, false);
     });
     double *r_cur = (pairs & 1) ? c->f2[F_R2] : c->f2[F_R];
     double *r_oth = (pairs & 1) ? c->f2[F_R] : c->f2[F_R2];

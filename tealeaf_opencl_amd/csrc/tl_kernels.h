@@ -100,12 +100,14 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
 // which buffer is read and which is written. ev_main, when given, is recorded
 // after the iteration kernel and before its finisher. The fused kernels
 // recompute Kx, Ky from the conductivity W and the face parameters.
+// stream_z: read z (src) with nontemporal loads (CG iterations; the PPCG
+// outer step, which follows the inner steps, measured faster without).
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
                        double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
-                       hipEvent_t ev_main = nullptr);
+                       hipEvent_t ev_main = nullptr, bool stream_z = true);
 // parity 1 / 2: the launch is known to run an odd / even iteration (CG: the
-// band-march kernels, the odd one leaving u alone); 0: either parity
-// (decided by the tile kernel)
+// update kernel compiled for that parity, the odd one leaving u alone); 0:
+// either parity (decided by the kernel from the state)
 void launch_fused_cg_ur(hipStream_t s, const Geom &g, const Ext &e, int cap, int prec, int ppcg, double *u,
                         const double *pA, const double *pB, const double *r, double *r_out, double *z,
                         double *sd, const double *W, const Faces &fc, const RedArgs &ra,

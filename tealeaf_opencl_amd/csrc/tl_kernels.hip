@@ -904,13 +904,16 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const d
 // w = A p and p.w (calc_w, :45-69); p is double-buffered by the iteration
 // counter and w is not stored. A band march (TL_MARCH_LOOP): each row of
 // z, p_old and the conductivity is loaded once per band, the new p of the
-// rows above and below and the face below are kept in registers. z is read
-// with nontemporal loads: the caches then keep more of p_old and the
+// rows above and below and the face below are kept in registers. NTZ: z is
+// read with nontemporal loads, so that the caches keep more of p_old and the
 // conductivity, whose halo rows and overlap columns neighbouring segments
-// and bands read again (at 4096^2: 105.6 -> 98.6 us per launch, and the
-// update kernel that follows 0.152 -> 0.149 ms; the same hint on p_old or on
-// the conductivity as well measured slower).
+// and bands read again: in CG iterations (at 4096^2) 105.6 -> 98.6 us per
+// launch, and the update kernel that follows 0.152 -> 0.149 ms; the same hint
+// on p_old or on the conductivity as well measured slower, and in the PPCG
+// outer step, after the inner steps, the hint made the kernel slower (113 ->
+// 122 us), so that launch reads z with plain loads.
 // Algorithmic bytes: R z, p_old, cond, W p = 32 B/cell.
+template <bool NTZ>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, March m, const double *src,
                                                               double *pA, double *pB, const double *W, Faces fc,
                                                               RedArgs ra)
@@ -932,10 +935,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, Mar
     // rows k-1 (pa) and k (pb) of the new p, conductivity of row k, the face
     // between them (kyb)
     int i = IX(g, j, k_begin - 1);
-    double pa = __builtin_nontemporal_load(src + i) + beta * p_old[i];
+    double pa = ld_nt<NTZ>(src + i) + beta * p_old[i];
     const double ca = W[i];
     i += P;
-    double pb = __builtin_nontemporal_load(src + i) + beta * p_old[i];
+    double pb = ld_nt<NTZ>(src + i) + beta * p_old[i];
     double cb = W[i];
     double kyb = ky_at(g, fc, j, k_begin, ca, cb);
     // loads of the next two rows in flight
@@ -944,7 +947,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, Mar
 #pragma unroll
     for (int q = 0; q < PF; ++q) {
         const int ii = i + (q + 1) * P;
-        zr[q] = __builtin_nontemporal_load(src + ii);
+        zr[q] = ld_nt<NTZ>(src + ii);
         pr[q] = p_old[ii];
         cr[q] = W[ii];
     }
@@ -956,7 +959,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, Mar
             const double cc = cr[q];
             {
                 const int ii = IX(g, j, kk + 1 + PF);
-                zr[q] = __builtin_nontemporal_load(src + ii);
+                zr[q] = ld_nt<NTZ>(src + ii);
                 pr[q] = p_old[ii];
                 cr[q] = W[ii];
             }
@@ -1813,11 +1816,15 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
 
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
                        double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
-                       hipEvent_t ev_main)
+                       hipEvent_t ev_main, bool stream_z)
 {
     const March m = make_march(e, MARCH_COLS);
     const int grid = march_grid(m, cap);
-    hipLaunchKernelGGL(k_fused_cg_w, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, src, pA, pB, W, fc, ra);
+    if (stream_z)
+        hipLaunchKernelGGL(k_fused_cg_w<true>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, src, pA, pB, W, fc, ra);
+    else
+        hipLaunchKernelGGL(k_fused_cg_w<false>, dim3(grid), dim3(BLOCK_THREADS), 0, s, g, e, m, src, pA, pB, W, fc,
+                           ra);
     if (ev_main) (void)hipEventRecord(ev_main, s);
     launch_finish(s, ra, grid, 1);
 }

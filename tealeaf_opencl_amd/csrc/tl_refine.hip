@@ -294,13 +294,15 @@ __device__ __forceinline__ double rf_march_ky(const Geom &g, const Faces &f, boo
     return march_ky(g, f, inner, j, k, wd, wc);
 }
 
-// z in k_rf_w: nontemporal loads for FP64 fields (as k_fused_cg_w: the
-// caches then keep more of p_old and the faces; 0.104 -> 0.098 ms per launch
-// at 4096^2), plain loads for FP32 fields (the hint measured 2 % slower there)
-template <typename T>
+// z in k_rf_w with NTZ: nontemporal loads for FP64 fields (as k_fused_cg_w:
+// the caches then keep more of p_old and the faces; 0.104 -> 0.098 ms per CG
+// iteration's launch at 4096^2), plain loads for FP32 fields (the hint
+// measured 2 % slower there). The PPCG outer step launches it without the
+// hint (with it: 115 -> 120 us after the inner steps).
+template <bool NTZ, typename T>
 __device__ __forceinline__ T ld_z(const T *p)
 {
-    if (sizeof(T) == 8) return __builtin_nontemporal_load(p);
+    if (NTZ && sizeof(T) == 8) return __builtin_nontemporal_load(p);
     return *p;
 }
 
@@ -308,7 +310,7 @@ __device__ __forceinline__ T ld_z(const T *p)
 // double-buffered by the parity of the iteration counter. A band march
 // (TL_MARCH_LOOP) as k_fused_cg_w: one load per row of z, p_old and the face
 // field(s), neighbours by lane exchange, each face formed once.
-template <typename T, int FACES>
+template <typename T, int FACES, bool NTZ>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, const T *src, T *pA, T *pB,
                                                         const T *W, const T *Wy, Faces fc, RfArgs ra)
 {
@@ -329,10 +331,10 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
     const bool out_lane = lane >= 1 && lane <= MARCH_COLS && j <= e.j1;
     const bool inner_x = jb - 1 > hd && jb + MARCH_COLS < hd + g.x_max;
     int i = IX(g, j, k_begin - 1);
-    double pa = pn((double)ld_z(src + i), (double)p_old[i]);
+    double pa = pn((double)ld_z<NTZ>(src + i), (double)p_old[i]);
     const double ca = (double)W[i];
     i += P;
-    double pb = pn((double)ld_z(src + i), (double)p_old[i]);
+    double pb = pn((double)ld_z<NTZ>(src + i), (double)p_old[i]);
     double cb = (double)W[i];  // K32: Kx of row k; else the per-cell face field of row k
     double kyb = K32 ? (double)Wy[i] : rf_march_ky<FACES>(g, fc, false, j, k_begin, ca, cb);
     constexpr int PF = 2;
@@ -340,7 +342,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
 #pragma unroll
     for (int q = 0; q < PF; ++q) {
         const int ii = i + (q + 1) * P;
-        zr[q] = (double)ld_z(src + ii);
+        zr[q] = (double)ld_z<NTZ>(src + ii);
         pr[q] = (double)p_old[ii];
         cr[q] = (double)W[ii];
         yr[q] = K32 ? (double)Wy[ii] : 0.0;
@@ -353,7 +355,7 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_rf_w(Geom g, Ext e, March m, 
             const double cc = cr[q], yc = yr[q];
             {
                 const int ii = IX(g, j, kk + 1 + PF);
-                zr[q] = (double)ld_z(src + ii);
+                zr[q] = (double)ld_z<NTZ>(src + ii);
                 pr[q] = (double)p_old[ii];
                 cr[q] = (double)W[ii];
                 yr[q] = K32 ? (double)Wy[ii] : 0.0;
@@ -741,7 +743,7 @@ void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
     const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
         ra_u{c->partials, w.st, RF_RRN_LAG, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) (void)hipEventRecord(ev[0], c->s);
-    hipLaunchKernelGGL((k_rf_w<T, FACES>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
+    hipLaunchKernelGGL((k_rf_w<T, FACES, true>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
                        f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
     if (ev) (void)hipEventRecord(ev[1], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid_m);
@@ -861,7 +863,7 @@ void launch_ppcg_outer(tl_chunk *c, tl_chunk::RefineWork &w, int inner)
     const T *src = (c->prec == PREC_NONE) ? f[RW_R] : f[RW_Z];
     const RfArgs ra_w{c->partials, w.st, RF_PW, nullptr, nullptr, 0},
         ra_u{c->partials, w.st, RF_RRN, nullptr, nullptr, 0};
-    hipLaunchKernelGGL((k_rf_w<T, ppcg_faces<T>()>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src,
+    hipLaunchKernelGGL((k_rf_w<T, ppcg_faces<T>(), false>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src,
                        f[RW_P], f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid_m);
     const double *ca = w.coef, *cb = w.coef + RF_MAX_INNER;
