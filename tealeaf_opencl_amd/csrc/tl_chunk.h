@@ -142,6 +142,12 @@ int read_result(tl_chunk *c, double *out, int nq);
 int precondition_r(tl_chunk *c);
 // frees the refined-solve workspaces (tl_refine.hip)
 void refine_release(tl_chunk *c);
+// Whether the CG iterations' w kernel reads z with nontemporal loads: when an
+// FP64 field is at most 256 MB (the size of MI355X's last-level cache, so
+// that p_old and the conductivity, which the hint leaves cached, span about
+// twice it). Measured: at 4096^2 (134 MB fields) the hint makes k_fused_cg_w
+// 7 % faster, at 8192^2 (537 MB) 8 % slower.
+inline bool stream_z_pays(const tl_chunk *c) { return sizeof(double) * c->elems2d <= (256L << 20); }
 // destroys every captured iteration batch (native and refined): they hold the
 // face parameters (rx, ry, walls) and buffer addresses by value
 void drop_graphs(tl_chunk *c);

@@ -473,7 +473,7 @@ void cg_iteration(tl_chunk *c, hipEvent_t *ev, bool prof = false, int parity = 0
     if (ev) (void)hipEventRecord(ev[0], c->s);
     launch_timed(c, prof, "k_fused_cg_w", true, [&](hipEvent_t mid) {
         launch_fused_cg_w(c->s, c->g, in, c->cap, src, c->f2[F_P], c->f2[F_P2], c->f2[F_COND], c->faces,
-                          red_args(c, RED_CG_PW, true), ev ? ev[1] : mid);
+                          red_args(c, RED_CG_PW, true), ev ? ev[1] : mid, stream_z_pays(c));
     });
     if (ev) (void)hipEventRecord(ev[2], c->s);
     launch_timed(c, prof, "k_fused_cg_ur", true, [&](hipEvent_t mid) {

@@ -100,8 +100,9 @@ void launch_jacobi(hipStream_t s, const Geom &g, const Ext &e, int cap, const do
 // which buffer is read and which is written. ev_main, when given, is recorded
 // after the iteration kernel and before its finisher. The fused kernels
 // recompute Kx, Ky from the conductivity W and the face parameters.
-// stream_z: read z (src) with nontemporal loads (CG iterations; the PPCG
-// outer step, which follows the inner steps, measured faster without).
+// stream_z: read z (src) with nontemporal loads (CG iterations on fields that
+// stream_z_pays accepts; the PPCG outer step, which follows the inner steps,
+// and larger fields measured faster without).
 void launch_fused_cg_w(hipStream_t s, const Geom &g, const Ext &e, int cap, const double *src,
                        double *pA, double *pB, const double *W, const Faces &fc, const RedArgs &ra,
                        hipEvent_t ev_main = nullptr, bool stream_z = true);

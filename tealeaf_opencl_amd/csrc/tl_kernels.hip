@@ -911,7 +911,8 @@ __global__ void __launch_bounds__(BLOCK_THREADS) k_jacobi(Geom g, Ext e, const d
 // launch, and the update kernel that follows 0.152 -> 0.149 ms; the same hint
 // on p_old or on the conductivity as well measured slower, and in the PPCG
 // outer step, after the inner steps, the hint made the kernel slower (113 ->
-// 122 us), so that launch reads z with plain loads.
+// 122 us), as it did at 8192^2 (457 -> 495 us): those launches read z with
+// plain loads.
 // Algorithmic bytes: R z, p_old, cond, W p = 32 B/cell.
 template <bool NTZ>
 __global__ void __launch_bounds__(BLOCK_THREADS) k_fused_cg_w(Geom g, Ext e, March m, const double *src,

@@ -298,7 +298,8 @@ __device__ __forceinline__ double rf_march_ky(const Geom &g, const Faces &f, boo
 // the caches then keep more of p_old and the faces; 0.104 -> 0.098 ms per CG
 // iteration's launch at 4096^2), plain loads for FP32 fields (the hint
 // measured 2 % slower there). The PPCG outer step launches it without the
-// hint (with it: 115 -> 120 us after the inner steps).
+// hint (with it: 115 -> 120 us after the inner steps), and so do the CG
+// iterations on fields larger than stream_z_pays allows.
 template <bool NTZ, typename T>
 __device__ __forceinline__ T ld_z(const T *p)
 {
@@ -743,8 +744,12 @@ void launch_iteration_f(tl_chunk *c, tl_chunk::RefineWork &w, hipEvent_t *ev)
     const RfArgs ra_w{c->partials, w.st, RF_PW, w.hist, w.hist + RF_HIST, RF_HIST},
         ra_u{c->partials, w.st, RF_RRN_LAG, w.hist, w.hist + RF_HIST, RF_HIST};
     if (ev) (void)hipEventRecord(ev[0], c->s);
-    hipLaunchKernelGGL((k_rf_w<T, FACES, true>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src, f[RW_P],
-                       f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
+    if (stream_z_pays(c))
+        hipLaunchKernelGGL((k_rf_w<T, FACES, true>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src,
+                           f[RW_P], f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
+    else
+        hipLaunchKernelGGL((k_rf_w<T, FACES, false>), dim3(grid_m), dim3(BLOCK_THREADS), 0, c->s, w.g, e, m, src,
+                           f[RW_P], f[RW_P2], f[RW_W], f[RW_KY], c->faces, ra_w);
     if (ev) (void)hipEventRecord(ev[1], c->s);
     hipLaunchKernelGGL(k_rf_finish, dim3(1), dim3(FINISH_THREADS), 0, c->s, ra_w, grid_m);
     if (ev) (void)hipEventRecord(ev[2], c->s);
